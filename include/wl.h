@@ -1,0 +1,134 @@
+/*
+ * wl.h -- C ABI of the MI355X (gfx950) WaterLily time-step library (libwaterlily_hip.so).
+ *
+ * Plain pointers and sizes only.  Every host buffer is in the memory layout Julia uses for the
+ * corresponding WaterLily array: column-major, x fastest, ghost layers included, the vector
+ * component index last (u[x,y,z,i], mu1[x,y,z,i,j]).  A `ccall` can therefore pass
+ * `pointer(A)` of the reference's own arrays.
+ *
+ * All entry points return 0 on success and a negative WL_E* code on failure; the message of the
+ * last failure on the calling thread is returned by wl_last_error().  Calls on one handle are not
+ * re-entrant; separate handles are independent.  Every call is synchronous for the host outputs
+ * it returns.
+ *
+ * Reference interfaces replaced (WaterLily.jl v1.8.0; paths relative to the WaterLily repository):
+ *   wl_flow_create      Flow(N, uBC; Δt, ν, g, λ, T, perdir, exitBC)      src/Flow.jl:133-147
+ *   wl_flow_upload /    the fields u, u⁰, f, p, σ, V, μ₀, μ₁ of Flow          src/Flow.jl:114-125
+ *     wl_flow_download
+ *   wl_flow_measure     measure!(flow, body; t, ϵ) for analytic bodies     src/Body.jl:28-51
+ *   wl_flow_pois_update update!(::MultiLevelPoisson)                        src/MultiLevelPoisson.jl:79-86
+ *   wl_mom_step         mom_step!(flow, pois) incl. CFL                     src/Flow.jl:156-167,234-244
+ *   wl_pressure_force   pressure_force(flow, body)                          src/Metrics.jl:126-133
+ *   wl_pois_create      Poisson(x,L,z) / MultiLevelPoisson(x,L,z)          src/Poisson.jl:32-38,
+ *                                                                           src/MultiLevelPoisson.jl:61-77
+ *   wl_pois_mult        mult!(pois, x)                                      src/Poisson.jl:63-69
+ *   wl_pois_solve       solver!(pois; tol, itmx)                            src/Poisson.jl:204-214,
+ *                                                                           src/MultiLevelPoisson.jl:108-127
+ *   wl_pois_update      update!(pois)                                       src/Poisson.jl:47
+ */
+#ifndef WL_H
+#define WL_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- error codes ---- */
+#define WL_OK 0
+#define WL_EINVAL (-1)   /* bad argument: shape, size, enum (the reference's @assert cases) */
+#define WL_EHIP (-2)     /* a HIP runtime call failed */
+#define WL_ENAN (-3)     /* a NaN/Inf reached the CFL reduction */
+#define WL_ENOMEM (-4)   /* device allocation failed */
+
+/* ---- enums ---- */
+enum wl_dtype { WL_F32 = 0, WL_F64 = 1 };
+/* convective scheme λ(u,c,d): quick (default), vanLeer, cds -- src/Flow.jl:4-6 */
+enum wl_scheme { WL_QUICK = 0, WL_VANLEER = 1, WL_CDS = 2 };
+/* Flow fields -- src/Flow.jl:114-125 */
+enum wl_field { WL_U = 0, WL_U0 = 1, WL_F = 2, WL_P = 3, WL_SIGMA = 4, WL_V = 5, WL_MU0 = 6, WL_MU1 = 7 };
+/* Poisson level fields -- src/Poisson.jl:22-31 */
+enum wl_pfield { WL_PX = 0, WL_PL = 1, WL_PD = 2, WL_PID = 3, WL_PEPS = 4, WL_PR = 5, WL_PZ = 6 };
+/* Pressure solver modes */
+enum wl_solver_mode {
+  WL_SOLVER_PLAIN = 0, /* MultiLevelPoisson exactly as the reference: FP32 V-cycles, GS-RB at every level */
+  WL_SOLVER_MIXED = 1  /* FP32 V-cycles inside an FP64 defect-correction loop */
+};
+/* Analytic bodies.  A body list is a union (SetBody(min,...), src/Body.jl:91-103). */
+enum wl_body_kind {
+  WL_BODY_SPHERE = 1,    /* AutoBody((x,t)->√sum(abs2,x.-c)-r); a circle in 2D */
+  WL_BODY_ELLIPSOID = 2  /* AutoBody((x,t)->(√sum(abs2,(x.-c)./a)-1)*minimum(a)) */
+};
+
+typedef struct {
+  int32_t kind;       /* wl_body_kind */
+  int32_t reserved;
+  double center[3];
+  double radius;      /* sphere radius */
+  double axes[3];     /* ellipsoid semi-axes */
+} wl_body;
+
+/* Flow(N, uBC; ...) arguments, src/Flow.jl:133-134 and src/WaterLily.jl:93-98.
+ * The boundary velocity is uniform in space: uBC(i,x,t) = uBC[i] + uBC_rate[i]*t.
+ * The body force is uniform in space: g(i,x,t) = g0[i] + g1[i]*t (has_g = 0 means g=nothing). */
+typedef struct {
+  int32_t D;            /* 2 or 3 */
+  int32_t n[3];         /* interior cells per direction (n[2] ignored when D==2) */
+  int32_t dtype;        /* wl_dtype */
+  int32_t scheme;       /* wl_scheme */
+  int32_t perdir_mask;  /* bit j set: direction j+1 periodic */
+  int32_t exit_bc;      /* convective exit in direction 1 */
+  int32_t has_g;
+  int32_t has_uBC_rate;
+  double uBC[3];
+  double uBC_rate[3];
+  double g0[3];
+  double g1[3];
+  double nu;
+  double dt0;
+} wl_flow_desc;
+
+typedef struct wl_flow_s* wl_flow_t;
+typedef struct wl_pois_s* wl_pois_t;
+
+const char* wl_last_error(void);
+int wl_device_count(int* n);
+
+/* ---- Flow: src/Flow.jl ---- */
+int wl_flow_create(wl_flow_t* out, const wl_flow_desc* desc);
+int wl_flow_destroy(wl_flow_t h);
+/* element count of a field (ghosted cells x components) */
+int wl_flow_field_size(wl_flow_t h, int field, size_t* count);
+int wl_flow_upload(wl_flow_t h, int field, const void* src, size_t nbytes);
+int wl_flow_download(wl_flow_t h, int field, void* dst, size_t nbytes);
+int wl_flow_measure(wl_flow_t h, const wl_body* bodies, int nbodies, double t, double eps);
+int wl_flow_pois_update(wl_flow_t h);
+int wl_flow_set_solver(wl_flow_t h, int mode, double tol, int itmx);
+/* One mom_step!: dt = Δt[end], t0 = t1 - dt, t1 = sum(Δt).  Returns the CFL time step that the
+ * caller pushes onto Δt, and the V-cycle counts of the two projections (pois.n entries). */
+int wl_mom_step(wl_flow_t h, double dt, double t0, double t1, double* dt_next, int16_t* n_iter);
+/* F = Σ p·nds(body) with the product in T and the sum in Float64; F has D entries. */
+int wl_pressure_force(wl_flow_t h, const wl_body* bodies, int nbodies, double t, double* F);
+/* Σr² residual history of the last mom_step! (both projections); returns the count written. */
+int wl_flow_last_residuals(wl_flow_t h, double* r2, int cap, int* count);
+
+/* ---- Poisson / MultiLevelPoisson: src/Poisson.jl, src/MultiLevelPoisson.jl ---- */
+/* Ng: ghosted sizes; L: Ng x D coefficients (host, layout as above); x, z start at zero. */
+int wl_pois_create(wl_pois_t* out, int D, const int32_t* Ng, int dtype, const void* L, int multilevel,
+                   int perdir_mask);
+int wl_pois_destroy(wl_pois_t h);
+int wl_pois_nlevels(wl_pois_t h, int* n);
+int wl_pois_level_dims(wl_pois_t h, int level, int32_t* Ng);
+int wl_pois_upload(wl_pois_t h, int level, int pfield, const void* src, size_t nbytes);
+int wl_pois_download(wl_pois_t h, int level, int pfield, void* dst, size_t nbytes);
+/* z = A x on level 1 (x given on the host; z written to the level's z and to z_out) */
+int wl_pois_mult(wl_pois_t h, const void* x, void* z_out, size_t nbytes);
+int wl_pois_update(wl_pois_t h);
+int wl_pois_solve(wl_pois_t h, double tol, int itmx, int32_t* n_iter);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* WL_H */
