@@ -1,0 +1,155 @@
+#!/usr/bin/env python3
+"""Benchmark of the gfx950 WaterLily time step (one GPU).
+
+A step is one sim_step! (mom_step!: predictor, two pressure projections with the multigrid to
+Σr² < 1e-4, corrector, CFL) of the 3D flow past a sphere, 256x128x128 interior cells, Float32,
+U=(1,0,0), radius 16 at (64,64,64), Re=3700 (ν = U·32/Re), static body (remeasure=false, which
+gives the same fields as remeasuring a static body).  Inputs are resident in HBM before timing.
+
+Prints one JSON line: Mcell-steps/s (interior cells x steps / s), the per-kernel roofline of the
+dominant kernel from HIP events recorded on the library's stream during the timed steps, and the
+oracle's CPU rate on a bounded sample of the same workload.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+from __graft_entry__ import ROOT, load_package
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "Mcell-steps/s & sim_step!/s, plain vs learned-warm-start MG at equal residual"
+
+
+def sphere_sim(wl, n, R, center, Re, L):
+    return wl.Simulation(n, (1, 0, 0), L, ν=L / Re, body=wl.Sphere(center, R), T=np.float32)
+
+
+def run_gpu(wl, n, R, center, Re, L, steps, warmup, profile=True):
+    sim = sphere_sim(wl, n, R, center, Re, L)
+    for _ in range(warmup):
+        wl.sim_step(sim, remeasure=False)
+    n0 = len(sim.pois.n)
+    if profile:
+        sim.flow.profile(True)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        wl.sim_step(sim, remeasure=False)
+    t1 = time.perf_counter()  # every sim_step returns after its CFL reduction reached the host
+    stats = sim.flow.profile_read() if profile else []
+    if profile:
+        sim.flow.profile(False)
+    cells = int(np.prod(n))
+    sec = t1 - t0
+    nv = sim.pois.n[n0:]
+    return dict(sec=sec, ms_per_step=1e3 * sec / steps, mcells=cells * steps / sec / 1e6, steps_per_s=steps / sec,
+                vcycles_per_step=sum(nv) / steps, stats=stats, sim=sim)
+
+
+def roofline(stats, traffic_file):
+    rows = []
+    for s in stats:
+        if s["launches"] == 0:
+            continue
+        avg = s["total_ms"] / s["launches"]
+        gbs = s["bytes_per_launch"] / (avg * 1e-3) / 1e9 if s["bytes_per_launch"] > 0 else None
+        rows.append(dict(kernel=f'{s["name"]}@L{s["level"]}', launches=s["launches"], total_ms=round(s["total_ms"], 3),
+                         avg_us=round(avg * 1e3, 2), alg_MB=round(s["bytes_per_launch"] / 1e6, 3),
+                         GBs=None if gbs is None else round(gbs, 1)))
+    rows.sort(key=lambda r: -r["total_ms"])
+    dom = next(r for r in rows if r["GBs"] is not None)
+    traffic = None
+    if os.path.exists(traffic_file):
+        with open(traffic_file) as fh:
+            tt = json.load(fh)
+        traffic = tt.get(dom["kernel"])
+    total_b = sum(s["bytes_per_launch"] * s["launches"] for s in stats)
+    total_ms = sum(s["total_ms"] for s in stats)
+    return dict(bound="hbm", kernel=dom["kernel"], achieved=dom["GBs"], peak=HBM_PEAK_GBS, unit="GB/s",
+                frac=round(dom["GBs"] / HBM_PEAK_GBS, 4), traffic=traffic,
+                step_alg_GBs=round(total_b / (total_ms * 1e-3) / 1e9, 1) if total_ms > 0 else None), rows
+
+
+def run_cpu(n, R, center, Re, L, budget_s=15.0):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import wl_oracle as O
+    threads = os.cpu_count() or 1
+    O.lib().wlo_threads(threads)
+    sim = O.OSimulation(n, (1, 0, 0), L, nu=L / Re, bodies=[{"kind": 1, "center": center, "radius": R}])
+    sim.step(remeasure=False)  # warm-up
+    k, t0 = 0, time.perf_counter()
+    while True:
+        sim.step(remeasure=False)
+        k += 1
+        el = time.perf_counter() - t0
+        if (el > budget_s and k >= 2) or k >= 200:
+            break
+    return dict(value=round(int(np.prod(n)) * k / el / 1e6, 3), unit="Mcell-steps/s", cores=threads, kind="port",
+                sample=f"{k} sim_step! of the same 256x128x128 sphere flow after 1 warm-up step "
+                       f"({el:.1f} s), oracle built -O3 -fopenmp on {threads} threads")
+
+
+def copy_calibration(nbytes=1 << 30, reps=5):
+    """A plain device copy of known size, for scaling rocprofv3 FETCH_SIZE/WRITE_SIZE."""
+    import torch
+    a = torch.empty(nbytes // 4, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    a.fill_(1.0)
+    for _ in range(reps):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes": nbytes, "reps": reps}), flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-c3", action="store_true")
+    ap.add_argument("--c3-steps", type=int, default=20)
+    ap.add_argument("--copy-calib", action="store_true")
+    ap.add_argument("--quiet-kernels", action="store_true")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(f"bench.py measures one GPU: --gpus {a.gpus} is not supported (the flow is stepped on a single "
+              f"MI355X; see the sharding design note)", file=sys.stderr)
+        sys.exit(2)
+    if a.copy_calib:
+        copy_calibration()
+    wl = load_package()
+    wl.load_library()
+    n, R, center, Re, L = (256, 128, 128), 16, (64, 64, 64), 3700.0, 32
+    r = run_gpu(wl, n, R, center, Re, L, a.steps, a.warmup)
+    roof, rows = roofline(r["stats"], os.path.join(ROOT, "profiles", "hbm_traffic.json"))
+    out = dict(metric=METRIC, value=round(r["mcells"], 2), unit="Mcell-steps/s", n_gpus=1, steps=a.steps,
+               warmup=a.warmup, ms_per_step=round(r["ms_per_step"], 4), higher_is_better=True, scaling="weak",
+               vs_baseline=None, dtype="f32", data="synthetic",
+               config=dict(workload="3D flow past a sphere 256x128x128, Re=3700, plain MultiLevelPoisson, "
+                                    "Float32, tol Σr²<1e-4", grid=list(n), sphere_radius=R, Re=Re,
+                           method="plain", remeasure=False),
+               sim_steps_per_s=round(r["steps_per_s"], 3), vcycles_per_step=round(r["vcycles_per_step"], 3),
+               roofline=roof)
+    if not a.quiet_kernels:
+        out["kernels"] = rows[:14]
+    if not a.no_c3:
+        n3, R3, c3, Re3, L3 = (512, 256, 256), 32, (128, 128, 128), 1e4, 64
+        r3 = run_gpu(wl, n3, R3, c3, Re3, L3, a.c3_steps, 3)
+        roof3, _ = roofline(r3["stats"], os.path.join(ROOT, "profiles", "hbm_traffic_c3.json"))
+        out["c3"] = dict(workload="3D sphere 512x256x256, r=32, Re=1e4, plain FP32 MG", steps=a.c3_steps,
+                         mcells=round(r3["mcells"], 2), ms_per_step=round(r3["ms_per_step"], 3),
+                         vcycles_per_step=round(r3["vcycles_per_step"], 3), roofline=roof3)
+        del r3
+    if not a.no_cpu:
+        out["cpu_baseline"] = run_cpu(n, R, center, Re, L)
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -114,6 +114,21 @@ int wl_pressure_force(wl_flow_t h, const wl_body* bodies, int nbodies, double t,
 /* Σr² residual history of the last mom_step! (both projections); returns the count written. */
 int wl_flow_last_residuals(wl_flow_t h, double* r2, int cap, int* count);
 
+/* Instrumentation, no counterpart in the reference: per-kernel HIP-event timing on the handle's
+ * stream.  wl_flow_profile(h,1) starts recording (and clears), wl_flow_profile_read aggregates
+ * launches, total milliseconds and the algorithmic bytes per launch by kernel name and level,
+ * then clears the records. */
+typedef struct {
+  char name[40];
+  int32_t level;
+  int32_t reserved;
+  int64_t launches;
+  double total_ms;
+  double bytes_per_launch;
+} wl_kernel_stat;
+int wl_flow_profile(wl_flow_t h, int enable);
+int wl_flow_profile_read(wl_flow_t h, wl_kernel_stat* out, int cap, int* count);
+
 /* ---- Poisson / MultiLevelPoisson: src/Poisson.jl, src/MultiLevelPoisson.jl ---- */
 /* Ng: ghosted sizes; L: Ng x D coefficients (host, layout as above); x, z start at zero. */
 int wl_pois_create(wl_pois_t* out, int D, const int32_t* Ng, int dtype, const void* L, int multilevel,

@@ -1,0 +1,231 @@
+"""GPU parity: the gfx950 path through the C ABI against the oracle on the same inputs.
+
+Bar: results within relative L2 1e-5 of the Float32 oracle after a step,
+identical V-cycle counts; element-wise kernels are additionally checked for bit equality where
+both sides evaluate the same IEEE operations in the same order.
+"""
+import math
+import sys
+
+import numpy as np
+import pytest
+
+import wl_oracle as O
+from __graft_entry__ import load_package
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def wl():
+    m = load_package()
+    m.load_library()
+    return m
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def rel(a, b):
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    nb = np.linalg.norm(b)
+    return np.linalg.norm(a - b) / (nb if nb > 0 else 1.0)
+
+
+def oracle_pair(wl, n, uBC, R, center, nu, T=np.float32, L=None):
+    dt = O.WL_F64 if T == np.float64 else O.WL_F32
+    sim = wl.Simulation(n, uBC, L or 2 * R, ν=nu, body=wl.Sphere(center, R), T=T)
+    osim = O.OSimulation(n, uBC, L or 2 * R, nu=nu, dtype=dt,
+                         bodies=[{"kind": 1, "center": tuple(center), "radius": R}])
+    return sim, osim
+
+
+@pytest.mark.parametrize("D", [2, 3])
+def test_construction_and_measure(wl, D):
+    say(f"case: construction+measure D={D}")
+    n = (40, 24) if D == 2 else (40, 24, 24)
+    c = (14, 12) if D == 2 else (14, 12, 12)
+    sim, osim = oracle_pair(wl, n, (1,) + (0,) * (D - 1), 5, c, 0.01)
+    for name in ("u", "u0", "V"):
+        assert np.array_equal(sim.flow.get(name), osim.flow.get(name)), name
+    for name in ("mu0", "mu1", "sigma"):
+        a, b = sim.flow.get(name), osim.flow.get(name)
+        assert np.max(np.abs(a.astype(np.float64) - b)) <= 2e-7, name
+        say(f"  {name}: bit-equal fraction {np.mean(a == b):.6f}")
+
+
+def test_one_step_bitwise_3d(wl):
+    """Same μ₀, μ₁, V on both sides: one mom_step! agrees bit for bit, including Δt and n."""
+    say("case: one step 3D 48x32x32, shared BDIM fields")
+    sim, osim = oracle_pair(wl, (48, 32, 32), (1, 0, 0), 6, (16, 16, 16), 12 / 500)
+    for name in ("mu0", "mu1", "V"):
+        sim.flow.set(name, osim.flow.get(name))
+    sim.pois.update()
+    wl.mom_step(sim.flow)
+    osim.step(remeasure=False)
+    for name in ("u", "u0", "p", "f", "sigma"):
+        a, b = sim.flow.get(name), osim.flow.get(name)
+        say(f"  {name}: rel-L2 {rel(a, b):.3e}  bit-equal {np.mean(a == b):.6f}")
+        assert rel(a, b) < 1e-6, name
+    assert sim.flow.Δt == osim.flow.dt
+    assert sim.pois.n == osim.n
+    say("  n =", sim.pois.n, " residuals gpu", sim.flow.residuals()[:4], " oracle", osim.flow.residuals()[:4])
+
+
+def test_c1_circle_trajectory(wl):
+    """2D circle of the CPU config (192x128, Re 250): 30 steps against the Float32 oracle."""
+    say("case: 2D circle 192x128, 30 steps")
+    sim, osim = oracle_pair(wl, (192, 128), (1, 0), 16, (63, 63), 32 / 250, L=32)
+    worst = 0.0
+    for k in range(30):
+        wl.sim_step(sim, remeasure=False)
+        osim.step(remeasure=False)
+        F, Fo = wl.pressure_force(sim), osim.pressure_force()
+        worst = max(worst, float(np.max(np.abs(F - Fo)) / max(np.max(np.abs(Fo)), 1e-12)))
+    eu, ep = rel(sim.flow.u, osim.flow.get("u")), rel(sim.flow.p, osim.flow.get("p"))
+    say(f"  rel-L2 u {eu:.3e} p {ep:.3e}; worst relative force error {worst:.3e}; n {sim.pois.n[-6:]}")
+    assert eu < 1e-5 and ep < 1e-5 and worst < 1e-3
+    assert sim.pois.n == osim.n
+
+
+def test_c2_sphere_one_step(wl):
+    """3D sphere 256x128x128, Re 3700: one step against the Float32 oracle; error against the
+    Float64 oracle reported."""
+    say("case: 3D sphere 256x128x128 one step")
+    n, nu = (256, 128, 128), 32 / 3700
+    sim, osim = oracle_pair(wl, n, (1, 0, 0), 16, (64, 64, 64), nu)
+    wl.sim_step(sim, remeasure=False)
+    osim.step(remeasure=False)
+    u, p = sim.flow.u, sim.flow.p
+    eu, ep = rel(u, osim.flow.get("u")), rel(p, osim.flow.get("p"))
+    o64 = O.OSimulation(n, (1, 0, 0), 32, nu=nu, dtype=O.WL_F64,
+                        bodies=[{"kind": 1, "center": (64, 64, 64), "radius": 16}])
+    o64.step(remeasure=False)
+    say(f"  vs F32 oracle: u {eu:.3e} p {ep:.3e}; vs F64 oracle: u {rel(u, o64.flow.get('u')):.3e} "
+        f"p {rel(p, o64.flow.get('p')):.3e}; n gpu {sim.pois.n} f32 {osim.n} f64 {o64.n}")
+    assert eu < 1e-5 and ep < 1e-5
+    assert sim.pois.n == osim.n
+
+
+# ---- the reference's own integration tests, on the GPU path ----
+def test_impulsive_box(wl):
+    say("case: impulsive box")
+    U = (2 / 3, -1 / 3)
+    f = wl.Flow((16, 16), U)
+    wl.mom_step(f)
+    u = f.u
+    assert wl.L2(u[:, :, 0] - np.float32(U[0])) < 2e-5 and wl.L2(u[:, :, 1] - np.float32(U[1])) < 1e-5
+
+
+def test_accelerating_circle(wl):
+    say("case: accelerating circle 1024x1024")
+    radius, H = 32, 16
+    sim = wl.Simulation((2 * H * radius,) * 2, ((0.0, 0.0), (1.0, 0.0)), radius, U=1,
+                        body=wl.Circle((H * radius, H * radius), radius))
+    wl.sim_step(sim)
+    assert np.allclose(wl.pressure_force(sim) / (math.pi * sim.L ** 2), [-1, 0], atol=0.04)
+    u = sim.flow.u
+    assert u.max() / u[1, 1, 0] > 1.91
+    for _ in range(3):
+        wl.sim_step(sim)
+    assert all(v <= 2 for v in sim.pois.n)
+
+
+def test_constant_jerk_periodic(wl):
+    say("case: constant jerk, periodic x, Float64")
+    N, jerk = 8, 4
+    sim = wl.Simulation((N, N), (math.sqrt(N), 0.0), N, ν=0.001, Δt=0.001, perdir=(1,), T=np.float64,
+                        g=((0.0, 0.0), (float(jerk), 0.0)))
+    wl.sim_step(sim, 1.0)
+    u = sim.flow.u
+    uf = math.sqrt(N) + 0.5 * jerk * wl.time(sim.flow) ** 2
+    assert wl.L2(u[:, :, 0] - uf) < 1e-4 and wl.L2(u[:, :, 1]) < 1e-4
+    osim = O.OSimulation((N, N), (math.sqrt(N), 0.0), N, nu=0.001, dt=0.001, perdir=(1,), dtype=O.WL_F64,
+                         g=((0.0, 0.0), (float(jerk), 0.0)))
+    osim.sim_step(1.0)
+    assert rel(u, osim.flow.get("u")) < 1e-12
+
+
+def test_semicoarsening_sims(wl):
+    say("case: semi-coarsening channel/duct")
+    H, R = 2 ** 4, 4
+    sim = wl.Simulation((8 * H, H), (1, 0), R, ν=R / 100, body=wl.Circle((4 * H, H // 2), R))
+    for _ in range(4):
+        wl.sim_step(sim, remeasure=False)
+    assert all(v <= 10 for v in sim.pois.n)
+    H, R = 2 ** 3, 2
+    sim = wl.Simulation((8 * H, H, H), (1, 0, 0), R, ν=R / 100, body=wl.Sphere((4 * H, H // 2, H // 2), R))
+    for _ in range(4):
+        wl.sim_step(sim, remeasure=False)
+    assert all(v <= 12 for v in sim.pois.n)
+
+
+# ---- Poisson / MultiLevelPoisson API (test/test_poisson.jl) ----
+def unit_coeffs(N, dtype=np.float32):
+    c = np.ones(tuple(N) + (len(N),), dtype=dtype, order="F")
+    return O.bc(c, (0.0,) * len(N))
+
+
+def poisson_setup(wl, cls, N, dtype=np.float32):
+    c = unit_coeffs(N, dtype)
+    x = np.zeros(N, dtype=dtype, order="F")
+    pois = cls(x, c, x.copy())
+    soln = np.zeros(N, dtype=dtype, order="F")
+    soln[...] = (np.arange(N[0]) + 1).reshape((-1,) + (1,) * (len(N) - 1))
+    I0 = (1,) * len(N)
+    soln -= soln[I0]
+    z = wl.mult(pois, soln)
+    assert np.array_equal(z, pois.z)
+    wl.solver(pois)
+    xs = pois.x
+    xs -= xs[I0]
+    return wl.L2(xs - soln) / wl.L2(soln), pois
+
+
+def test_poisson_api(wl):
+    say("case: Poisson / MultiLevelPoisson API")
+    err, pois = poisson_setup(wl, wl.Poisson, (5, 5))
+    Dm = np.array([[0, 0, 0, 0, 0], [0, -2, -3, -2, 0], [0, -3, -4, -3, 0], [0, -2, -3, -2, 0], [0, 0, 0, 0, 0]],
+                  dtype=np.float32)
+    assert np.array_equal(pois.D, Dm) and err < 1e-5
+    err, pois = poisson_setup(wl, wl.Poisson, (2 ** 6 + 2,) * 2)
+    assert err < 1e-6 and pois.n[-1] < 310
+    err, pois = poisson_setup(wl, wl.Poisson, (2 ** 4 + 2,) * 3)
+    assert err < 1e-6 and pois.n[-1] < 35
+    with pytest.raises(AssertionError, match="MultiLevelPoisson requires size=a2ⁿ, where n>2"):
+        poisson_setup(wl, wl.MultiLevelPoisson, (15 + 2, 3 ** 4 + 2))
+    err, pois = poisson_setup(wl, wl.MultiLevelPoisson, (10, 10))
+    assert np.array_equal(pois.levels[2].D, np.array([[0, 0, 0, 0], [0, -2, -2, 0], [0, -2, -2, 0], [0, 0, 0, 0]],
+                                                      dtype=np.float32))
+    assert err < 1e-5
+    L = pois.L
+    L[4:6, :, 0] = 0
+    pois.set("L", L)
+    wl.update(pois)
+    assert np.array_equal(pois.levels[2].D, np.array([[0, 0, 0, 0], [0, -1, -1, 0], [0, -1, -1, 0], [0, 0, 0, 0]],
+                                                      dtype=np.float32))
+    for N in ((2 ** 6 + 2,) * 2, (2 ** 4 + 2,) * 3):
+        err, pois = poisson_setup(wl, wl.MultiLevelPoisson, N)
+        assert err < 1e-6 and pois.n[-1] <= 3
+
+
+def test_mg_random_matches_oracle(wl):
+    """Variable coefficients and a random source: the GPU multigrid and the oracle's agree."""
+    say("case: MG random coefficients 66x34x34")
+    rng = np.random.default_rng(7)
+    N = (66, 34, 34)
+    L = unit_coeffs(N)
+    L[1:-1, 1:-1, 1:-1, :] *= rng.uniform(0.1, 1.0, size=(64, 32, 32, 3)).astype(np.float32)
+    O.bc(L, (0.0, 0.0, 0.0))
+    z = np.zeros(N, dtype=np.float32, order="F")
+    z[1:-1, 1:-1, 1:-1] = rng.normal(size=(64, 32, 32))
+    g = wl.MultiLevelPoisson(np.zeros(N, np.float32, order="F"), L, z)
+    o = O.OPois(L)
+    o.set("z", z)
+    wl.solver(g)
+    o.solve()
+    say(f"  n gpu {g.n} oracle {o.n}; rel-L2 x {rel(g.x, o.get('x')):.3e}")
+    assert g.n == o.n and rel(g.x, o.get("x")) < 1e-5

@@ -1,0 +1,12 @@
+"""MI355X (gfx950) WaterLily time-step path: HIP kernels behind a C ABI (include/wl.h) and a
+host-side mirror of the reference's Simulation / Flow / MultiLevelPoisson API.
+
+The package directory is named `waterlily.jl_amd`; load it with `load_package()` from
+`__graft_entry__` (it registers the module as `waterlily_amd`).
+"""
+from ._abi import LIB_PATH, WaterLilyError, load as load_library  # noqa: F401
+from .api import (  # noqa: F401
+    AbstractBody, BodyUnion, Circle, Ellipsoid, Flow, FlowPoisson, L2, MultiLevelPoisson, NoBody, Poisson,
+    Simulation, Sphere, cds, coarsen_mask, measure, measure_flow, mom_step, mult, pressure_force, quick, sim_step,
+    sim_time, solver, time, update, vanLeer,
+)

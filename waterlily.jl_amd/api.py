@@ -1,0 +1,447 @@
+"""Host-side mirror of WaterLily.jl's time-step API over the gfx950 C ABI.
+
+Names and argument meanings follow the reference (WaterLily.jl v1.8.0): `Flow`, `mom_step`
+(mom_step!), `MultiLevelPoisson`, `Poisson`, `solver` (solver!), `mult` (mult!), `update`
+(update!), `Simulation`, `sim_step` (sim_step!), `sim_time`, `measure` (measure!) and
+`pressure_force`.  Arrays cross the boundary in Julia's layout: column-major numpy arrays of
+shape (Nx, Ny[, Nz][, D[, D]]), ghosts included.  All compute happens in libwaterlily_hip.so.
+"""
+import ctypes as C
+import math
+
+import numpy as np
+
+from . import _abi as A
+
+quick, vanLeer, cds = A.WL_QUICK, A.WL_VANLEER, A.WL_CDS
+
+
+def _dtype_code(T):
+    T = np.dtype(T)
+    if T == np.float32:
+        return A.WL_F32
+    if T == np.float64:
+        return A.WL_F64
+    raise AssertionError(f"element type {T} is not supported (Float32 or Float64)")
+
+
+def _perdir_mask(perdir):
+    return sum(1 << (int(j) - 1) for j in perdir)
+
+
+# ---------------------------------------------------------------------------------------------
+# Bodies: analytic AutoBody SDFs (src/AutoBody.jl), unions as SetBody(min,...) (src/Body.jl:91-103)
+# ---------------------------------------------------------------------------------------------
+class AbstractBody:
+    def parts(self):
+        raise NotImplementedError
+
+    def __add__(self, other):
+        return BodyUnion(self.parts() + other.parts())
+
+    __or__ = __add__
+
+
+class Sphere(AbstractBody):
+    """AutoBody((x,t)->√sum(abs2,x.-center)-radius); a circle in 2D."""
+
+    def __init__(self, center, radius):
+        self.center, self.radius = tuple(float(c) for c in center), float(radius)
+
+    def parts(self):
+        return [self]
+
+    def sdf(self, x):
+        return math.sqrt(sum((a - b) ** 2 for a, b in zip(x, self.center))) - self.radius
+
+
+Circle = Sphere
+
+
+class Ellipsoid(AbstractBody):
+    """AutoBody((x,t)->(√sum(abs2,(x.-center)./axes)-1)*minimum(axes))."""
+
+    def __init__(self, center, axes):
+        self.center, self.axes = tuple(float(c) for c in center), tuple(float(a) for a in axes)
+
+    def parts(self):
+        return [self]
+
+    def sdf(self, x):
+        q = math.sqrt(sum(((a - b) / s) ** 2 for a, b, s in zip(x, self.center, self.axes)))
+        return (q - 1) * min(self.axes)
+
+
+class BodyUnion(AbstractBody):
+    def __init__(self, bodies):
+        if not 1 <= len(bodies) <= 4:
+            raise AssertionError("between 1 and 4 bodies are supported")
+        self.bodies = list(bodies)
+
+    def parts(self):
+        return list(self.bodies)
+
+
+class NoBody(AbstractBody):
+    def parts(self):
+        return []
+
+
+def _body_array(body):
+    parts = body.parts()
+    arr = (A.WlBody * max(1, len(parts)))()
+    for k, b in enumerate(parts):
+        if isinstance(b, Sphere):
+            arr[k].kind = A.WL_BODY_SPHERE
+            arr[k].radius = b.radius
+            for q, c in enumerate(b.center):
+                arr[k].center[q] = c
+        else:
+            arr[k].kind = A.WL_BODY_ELLIPSOID
+            for q, (c, a) in enumerate(zip(b.center, b.axes)):
+                arr[k].center[q] = c
+                arr[k].axes[q] = a
+    return arr, len(parts)
+
+
+# ---------------------------------------------------------------------------------------------
+# Flow (src/Flow.jl:114-148)
+# ---------------------------------------------------------------------------------------------
+class Flow:
+    """Flow(N, uBC; Δt=0.25, ν=0, g=None, λ=quick, T=float32, perdir=(), exitBC=False).
+
+    uBC: a tuple (constant), or a pair (U, dU/dt) for a boundary velocity uniform in space that
+    varies linearly in time.  g: None, or a pair (g0, g1) for g(i,x,t) = g0[i] + g1[i]*t.
+    The flow owns its MultiLevelPoisson (x≡p, L≡μ₀, z≡σ), exposed as `flow.pois`.
+    """
+
+    def __init__(self, N, uBC, Δt=0.25, ν=0.0, g=None, λ=quick, T=np.float32, perdir=(), exitBC=False,
+                 dt=None, nu=None):
+        if dt is not None:
+            Δt = dt
+        if nu is not None:
+            ν = nu
+        self.D = len(N)
+        self.N = tuple(int(v) for v in N)
+        self.Ng = tuple(v + 2 for v in self.N)
+        self.T = np.dtype(T).type
+        d = A.WlFlowDesc()
+        d.D = self.D
+        for k in range(self.D):
+            d.n[k] = self.N[k]
+        d.dtype = _dtype_code(T)
+        d.scheme = int(λ)
+        d.perdir_mask = _perdir_mask(perdir)
+        d.exit_bc = int(bool(exitBC))
+        if len(uBC) == 2 and hasattr(uBC[0], "__len__"):
+            U, rate = uBC
+            d.has_uBC_rate = 1
+            for k in range(self.D):
+                d.uBC_rate[k] = float(rate[k])
+        else:
+            U = uBC
+        for k in range(self.D):
+            d.uBC[k] = float(U[k])
+        if g is not None:
+            d.has_g = 1
+            for k in range(self.D):
+                d.g0[k], d.g1[k] = float(g[0][k]), float(g[1][k])
+        d.nu, d.dt0 = float(ν), float(Δt)
+        self.desc = d
+        self.uBC = tuple(float(U[k]) for k in range(self.D))
+        self.ν = self.T(ν)
+        self.perdir = tuple(perdir)
+        self.exitBC = bool(exitBC)
+        self.Δt = [float(self.T(Δt))]  # time-step history on the host (src/Flow.jl:127)
+        self.h = C.c_void_p()
+        A.check(A.load().wl_flow_create(C.byref(self.h), C.byref(d)))
+        self.pois = FlowPoisson(self)
+
+    # fields, downloaded in Julia layout
+    def _shape(self, name):
+        if name in ("p", "sigma"):
+            return self.Ng
+        if name == "mu1":
+            return self.Ng + (self.D, self.D)
+        return self.Ng + (self.D,)
+
+    def get(self, name):
+        a = np.zeros(self._shape(name), dtype=self.T, order="F")
+        A.check(A.load().wl_flow_download(self.h, A.FIELDS[name], a.ctypes.data, a.nbytes))
+        return a
+
+    def set(self, name, a):
+        a = np.asfortranarray(a, dtype=self.T)
+        if a.shape != self._shape(name):
+            raise AssertionError(f"{name} must have shape {self._shape(name)}")
+        A.check(A.load().wl_flow_upload(self.h, A.FIELDS[name], a.ctypes.data, a.nbytes))
+
+    u = property(lambda s: s.get("u"), lambda s, v: s.set("u", v))
+    u0 = property(lambda s: s.get("u0"), lambda s, v: s.set("u0", v))
+    f = property(lambda s: s.get("f"), lambda s, v: s.set("f", v))
+    p = property(lambda s: s.get("p"), lambda s, v: s.set("p", v))
+    sigma = property(lambda s: s.get("sigma"), lambda s, v: s.set("sigma", v))
+    V = property(lambda s: s.get("V"), lambda s, v: s.set("V", v))
+    mu0 = property(lambda s: s.get("mu0"), lambda s, v: s.set("mu0", v))
+    mu1 = property(lambda s: s.get("mu1"), lambda s, v: s.set("mu1", v))
+
+    def residuals(self):
+        """Σr² after residual! and after every V-cycle of the last mom_step (both projections)."""
+        buf = (C.c_double * 512)()
+        cnt = C.c_int()
+        A.check(A.load().wl_flow_last_residuals(self.h, buf, 512, C.byref(cnt)))
+        return [buf[k] for k in range(min(cnt.value, 512))]
+
+    def profile(self, enable=True):
+        """Start (clearing) or stop per-kernel HIP-event timing on the flow's stream."""
+        A.check(A.load().wl_flow_profile(self.h, int(enable)))
+
+    def profile_read(self):
+        """Per-kernel stats since profile(True): dicts of name, level, launches, total_ms,
+        bytes_per_launch (algorithmic bytes)."""
+        buf = (A.WlKernelStat * 256)()
+        cnt = C.c_int()
+        A.check(A.load().wl_flow_profile_read(self.h, buf, 256, C.byref(cnt)))
+        return [dict(name=buf[k].name.decode(), level=buf[k].level, launches=buf[k].launches,
+                     total_ms=buf[k].total_ms, bytes_per_launch=buf[k].bytes_per_launch)
+                for k in range(min(cnt.value, 256))]
+
+    def __del__(self):
+        try:
+            if getattr(self, "h", None):
+                A.load().wl_flow_destroy(self.h)
+                self.h = C.c_void_p()
+        except Exception:
+            pass
+
+
+class FlowPoisson:
+    """The MultiLevelPoisson(flow.p, flow.μ₀, flow.σ) attached to a Flow (src/WaterLily.jl:97)."""
+
+    def __init__(self, flow):
+        self.flow = flow
+        self.n = []  # pois.n (src/MultiLevelPoisson.jl:66)
+
+    def update(self):
+        A.check(A.load().wl_flow_pois_update(self.flow.h))
+
+    def set_solver(self, tol=1e-4, itmx=32, mode=A.WL_SOLVER_PLAIN):
+        A.check(A.load().wl_flow_set_solver(self.flow.h, mode, tol, itmx))
+
+
+def time(flow):
+    """time(a::AbstractFlow) = sum(Δt[1:end-1]) (src/Flow.jl:174)."""
+    return float(np.sum(np.asarray(flow.Δt[:-1], dtype=flow.T), dtype=flow.T)) if len(flow.Δt) > 1 else 0.0
+
+
+def mom_step(flow, pois=None):
+    """mom_step!(a::Flow, b::MultiLevelPoisson) (src/Flow.jl:156-167): one predictor-corrector
+    step on the device, then push!(Δt, CFL)."""
+    pois = flow.pois if pois is None else pois
+    dt = flow.Δt[-1]
+    t1 = float(np.sum(np.asarray(flow.Δt, dtype=flow.T), dtype=flow.T))
+    t0 = float(flow.T(t1) - flow.T(dt))
+    out = C.c_double()
+    nit = (C.c_int16 * 2)()
+    rc = A.load().wl_mom_step(flow.h, dt, t0, t1, C.byref(out), nit)
+    if rc == A.WL_ENAN:
+        raise FloatingPointError(A.load().wl_last_error().decode())
+    A.check(rc)
+    flow.Δt.append(float(flow.T(out.value)))
+    pois.n.extend([int(nit[0]), int(nit[1])])
+    return pois
+
+
+# ---------------------------------------------------------------------------------------------
+# Poisson / MultiLevelPoisson on host arrays (src/Poisson.jl:22-39, src/MultiLevelPoisson.jl:61-77)
+# ---------------------------------------------------------------------------------------------
+class _DevicePoisson:
+    multilevel = True
+
+    def __init__(self, x, L, z, perdir=()):
+        x = np.asfortranarray(x)
+        L = np.asfortranarray(L, dtype=x.dtype)
+        z = np.asfortranarray(z, dtype=x.dtype)
+        if not (x.shape == z.shape and L.shape[:-1] == x.shape and L.shape[-1] == x.ndim):
+            raise AssertionError("axes(x) == axes(z) == front(axes(L)) and last(axes(L)) == 1:ndims(x)")
+        self.T = x.dtype.type
+        self.nd = x.ndim
+        self.Ng = x.shape
+        Ng = (C.c_int32 * 3)(*(list(self.Ng) + [1] * (3 - self.nd)))
+        self.h = C.c_void_p()
+        A.check(A.load().wl_pois_create(C.byref(self.h), self.nd, Ng, _dtype_code(self.T), L.ctypes.data,
+                                        int(self.multilevel), _perdir_mask(perdir)))
+        self.n = []
+        if np.any(x):
+            self.set("x", x)
+        if np.any(z):
+            self.set("z", z)
+
+    @property
+    def nlevels(self):
+        v = C.c_int()
+        A.check(A.load().wl_pois_nlevels(self.h, C.byref(v)))
+        return v.value
+
+    def level_dims(self, level=0):
+        Ng = (C.c_int32 * 3)()
+        A.check(A.load().wl_pois_level_dims(self.h, level, Ng))
+        return tuple(Ng[k] for k in range(self.nd))
+
+    def get(self, name, level=0):
+        shp = self.level_dims(level) + ((self.nd,) if name == "L" else ())
+        a = np.zeros(shp, dtype=self.T, order="F")
+        A.check(A.load().wl_pois_download(self.h, level, A.PFIELDS[name], a.ctypes.data, a.nbytes))
+        return a
+
+    def set(self, name, a, level=0):
+        a = np.asfortranarray(a, dtype=self.T)
+        A.check(A.load().wl_pois_upload(self.h, level, A.PFIELDS[name], a.ctypes.data, a.nbytes))
+
+    x = property(lambda s: s.get("x"))
+    z = property(lambda s: s.get("z"))
+    r = property(lambda s: s.get("r"))
+    L = property(lambda s: s.get("L"))
+    D = property(lambda s: s.get("D"))
+    iD = property(lambda s: s.get("iD"))
+
+    def __del__(self):
+        try:
+            if getattr(self, "h", None):
+                A.load().wl_pois_destroy(self.h)
+                self.h = C.c_void_p()
+        except Exception:
+            pass
+
+
+class Poisson(_DevicePoisson):
+    """Poisson(x, L, z): Jacobi-preconditioned CG solver (src/Poisson.jl:22-39,166-214)."""
+    multilevel = False
+
+
+class MultiLevelPoisson(_DevicePoisson):
+    """MultiLevelPoisson(x, L, z): geometric multigrid with GS-RB smoothing
+    (src/MultiLevelPoisson.jl:61-127)."""
+    multilevel = True
+
+    class _LevelView:
+        def __init__(self, ml, level):
+            self.ml, self.level = ml, level
+
+        def __getattr__(self, name):
+            if name in A.PFIELDS:
+                return self.ml.get(name, self.level)
+            raise AttributeError(name)
+
+    @property
+    def levels(self):
+        return [MultiLevelPoisson._LevelView(self, l) for l in range(self.nlevels)]
+
+
+def mult(pois, x):
+    """mult!(p, x): z = A x (ghosts 0), stored in p.z and returned (src/Poisson.jl:63-69)."""
+    x = np.asfortranarray(x, dtype=pois.T)
+    z = np.zeros_like(x, order="F")
+    A.check(A.load().wl_pois_mult(pois.h, x.ctypes.data, z.ctypes.data, x.nbytes))
+    return z
+
+
+def update(pois):
+    """update!(p) (src/Poisson.jl:47, src/MultiLevelPoisson.jl:79-86)."""
+    if isinstance(pois, FlowPoisson):
+        return pois.update()
+    A.check(A.load().wl_pois_update(pois.h))
+
+
+def solver(pois, tol=1e-4, itmx=None):
+    """solver!(p; tol, itmx): itmx defaults to 32 (multigrid) or 1000 (pcg)."""
+    if itmx is None:
+        itmx = 32 if pois.multilevel else 1000
+    n = C.c_int32()
+    A.check(A.load().wl_pois_solve(pois.h, float(tol), int(itmx), C.byref(n)))
+    pois.n.append(n.value)
+    return pois
+
+
+# ---------------------------------------------------------------------------------------------
+# Simulation (src/WaterLily.jl:86-161)
+# ---------------------------------------------------------------------------------------------
+class Simulation:
+    """Simulation(dims, uBC, L; U, Δt, ν, ϵ, g, perdir, exitBC, λ, body, T)."""
+
+    def __init__(self, dims, uBC, L, U=None, Δt=0.25, ν=0.0, ϵ=1, g=None, perdir=(), exitBC=False, λ=quick,
+                 body=None, T=np.float32, dt=None, nu=None, eps=None):
+        if dt is not None:
+            Δt = dt
+        if nu is not None:
+            ν = nu
+        if eps is not None:
+            ϵ = eps
+        funcBC = len(uBC) == 2 and hasattr(uBC[0], "__len__")
+        if U is None:
+            if funcBC:
+                raise AssertionError("`U` (velocity scale) must be specified if boundary conditions `uBC` is a `Function`")
+            U = math.sqrt(sum(float(v) ** 2 for v in uBC))
+        self.U, self.L, self.ϵ = U, L, ϵ
+        self.body = body if body is not None else NoBody()
+        self.flow = Flow(dims, uBC, Δt=Δt, ν=ν, g=g, λ=λ, T=T, perdir=perdir, exitBC=exitBC)
+        measure_flow(self.flow, self.body, t=0.0, ϵ=ϵ)
+        self.pois = self.flow.pois
+        self.pois.update()
+
+
+def measure_flow(flow, body, t=0.0, ϵ=1):
+    """measure!(flow, body; t, ϵ) (src/Body.jl:28-51) for analytic bodies; NoBody skips."""
+    if isinstance(body, NoBody):
+        return
+    arr, nb = _body_array(body)
+    A.check(A.load().wl_flow_measure(flow.h, arr, nb, float(t), float(ϵ)))
+
+
+def measure(sim, t=None):
+    """measure!(sim, t=sum(Δt)): re-measure the body and update! the Poisson coefficients."""
+    t = float(sum(sim.flow.Δt)) if t is None else t
+    if isinstance(sim.body, NoBody):
+        sim.pois.update()
+        return
+    measure_flow(sim.flow, sim.body, t=t, ϵ=sim.ϵ)
+    sim.pois.update()
+
+
+def sim_time(sim):
+    """tU/L with t = sum(Δt[1:end-1]) (src/WaterLily.jl:117)."""
+    return time(sim.flow) * sim.U / sim.L
+
+
+def sim_step(sim, t_end=None, remeasure=True, max_steps=None):
+    """sim_step!(sim[, t_end]; remeasure, max_steps) (src/WaterLily.jl:128-139)."""
+    if t_end is None:
+        if remeasure:
+            measure(sim)
+        mom_step(sim.flow, sim.pois)
+        return sim
+    steps0 = len(sim.flow.Δt)
+    while sim_time(sim) < t_end and (max_steps is None or len(sim.flow.Δt) - steps0 < max_steps):
+        sim_step(sim, remeasure=remeasure)
+    return sim
+
+
+def pressure_force(sim):
+    """pressure_force(sim) (src/Metrics.jl:126-133): Σ p·nds(body) in Float64."""
+    F = (C.c_double * 3)()
+    arr, nb = _body_array(sim.body)
+    if nb == 0:
+        return np.zeros(sim.flow.D)
+    A.check(A.load().wl_pressure_force(sim.flow.h, arr, nb, time(sim.flow), F))
+    return np.array([F[k] for k in range(sim.flow.D)])
+
+
+def L2(a):
+    """L₂(a) = Σ a² over inside(a) (src/Poisson.jl:188), on a host array."""
+    a = np.asarray(a)
+    return float(np.sum(np.square(a[tuple(slice(1, -1) for _ in range(a.ndim))], dtype=np.float64)))
+
+
+def coarsen_mask(N):
+    """coarsen_mask(N) = map(divisible, N) (src/MultiLevelPoisson.jl:29-31,52)."""
+    return tuple(n % 2 == 0 and n > 4 for n in N)

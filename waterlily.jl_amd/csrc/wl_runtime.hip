@@ -1,0 +1,1016 @@
+// Host runtime of the gfx950 WaterLily time step and the C ABI declared in include/wl.h.
+//
+// One handle owns its device buffers and one HIP stream.  mom_step! runs entirely on the device;
+// the only device->host traffic per step is the Poisson residual Σr² once per V-cycle (the
+// reference's stop test, src/MultiLevelPoisson.jl:115-123) and the CFL maximum.
+#include <hip/hip_runtime.h>
+#include <cstdio>
+#include <cstring>
+#include <cmath>
+#include <string>
+#include <vector>
+#include <memory>
+#include <algorithm>
+#include "wl_kernels.h"
+
+using namespace wl;
+
+// ---------------------------------------------------------------------------------------------
+// errors
+// ---------------------------------------------------------------------------------------------
+static thread_local std::string g_err;
+static int set_err(int code, const std::string& m) {
+  g_err = m;
+  return code;
+}
+#define HIPCK(x)                                                                          \
+  do {                                                                                    \
+    hipError_t e_ = (x);                                                                  \
+    if (e_ != hipSuccess)                                                                 \
+      throw WlErr(WL_EHIP, std::string(#x) + ": " + hipGetErrorString(e_));               \
+  } while (0)
+#define KCK() HIPCK(hipGetLastError())
+
+struct WlErr {
+  int code;
+  std::string msg;
+  WlErr(int c, std::string m) : code(c), msg(std::move(m)) {}
+};
+#define API_BEGIN try {
+#define API_END                                   \
+  }                                               \
+  catch (const WlErr& e) { return set_err(e.code, e.msg); } \
+  catch (const std::bad_alloc&) { return set_err(WL_ENOMEM, "host allocation failed"); } \
+  return 0;
+
+// ---------------------------------------------------------------------------------------------
+// launch helpers
+// ---------------------------------------------------------------------------------------------
+static Box box_all(const Geo& g) {
+  Box b;
+  for (int d = 0; d < 3; ++d) { b.lo[d] = 0; b.hi[d] = g.N[d] - 1; }
+  return b;
+}
+static Box box_inside(const Geo& g) {
+  Box b;
+  for (int d = 0; d < 3; ++d) {
+    if (d < g.D) { b.lo[d] = 1; b.hi[d] = g.N[d] - 2; } else { b.lo[d] = b.hi[d] = 0; }
+  }
+  return b;
+}
+static Box box_half(const Geo& g) {  // half_rangek, src/Poisson.jl:130-132
+  Box b = box_inside(g);
+  b.hi[g.D - 1] = g.N[g.D - 1] / 2 - 1;
+  return b;
+}
+static bool box_empty(const Box& b) {
+  for (int d = 0; d < 3; ++d) if (b.hi[d] < b.lo[d]) return true;
+  return false;
+}
+static dim3 grid_of(const Box& b) {
+  const int nx = b.hi[0] - b.lo[0] + 1, ny = b.hi[1] - b.lo[1] + 1, nz = b.hi[2] - b.lo[2] + 1;
+  return dim3((unsigned)((nx + BX - 1) / BX), (unsigned)((ny + BY - 1) / BY), (unsigned)nz);
+}
+static int64_t nblocks(const Box& b) {
+  if (box_empty(b)) return 0;
+  dim3 gd = grid_of(b);
+  return (int64_t)gd.x * gd.y * gd.z;
+}
+static const dim3 kBlock(BX, BY, 1);
+
+template <class T>
+static T* dalloc(int64_t n) {
+  void* p = nullptr;
+  if (n <= 0) n = 1;
+  hipError_t e = hipMalloc(&p, (size_t)n * sizeof(T));
+  if (e != hipSuccess) throw WlErr(WL_ENOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+  HIPCK(hipMemset(p, 0, (size_t)n * sizeof(T)));
+  HIPCK(hipDeviceSynchronize());  // handles use non-blocking streams: the zero fill must be done
+  return (T*)p;
+}
+static void dfree(void* p) {
+  if (p) (void)hipFree(p);
+}
+
+// ---------------------------------------------------------------------------------------------
+// Per-kernel timing with HIP events on the handle's stream (enabled by wl_flow_profile).  Each
+// record carries the kernel's algorithmic bytes (distinct elements read + written) per launch.
+// ---------------------------------------------------------------------------------------------
+struct Prof {
+  bool on = false;
+  struct Rec {
+    const char* key;
+    int level;
+    double bytes;
+    hipEvent_t a, b;
+  };
+  std::vector<Rec> recs;
+  std::vector<hipEvent_t> pool;
+  size_t used = 0;
+  hipEvent_t ev() {
+    if (used == pool.size()) {
+      hipEvent_t e;
+      HIPCK(hipEventCreate(&e));
+      pool.push_back(e);
+    }
+    return pool[used++];
+  }
+  ~Prof() {
+    for (auto e : pool) (void)hipEventDestroy(e);
+  }
+};
+struct ProfScope {
+  Prof* P;
+  hipStream_t st;
+  size_t idx = 0;
+  ProfScope(Prof* P_, hipStream_t st_, const char* key, int level, double bytes) : P(P_), st(st_) {
+    if (P && P->on) {
+      Prof::Rec r{key, level, bytes, P->ev(), P->ev()};
+      HIPCK(hipEventRecord(r.a, st));
+      P->recs.push_back(r);
+      idx = P->recs.size() - 1;
+    }
+  }
+  ~ProfScope() {
+    if (P && P->on) (void)hipEventRecord(P->recs[idx].b, st);
+  }
+};
+static int64_t ncells_inside(const Geo& g) {
+  int64_t m = 1;
+  for (int d = 0; d < g.D; ++d) m *= g.N[d] - 2;
+  return m;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Multigrid hierarchy (src/MultiLevelPoisson.jl) and the single-level Poisson (src/Poisson.jl)
+// ---------------------------------------------------------------------------------------------
+template <class T>
+struct Level {
+  Geo g;
+  T *L = nullptr, *x = nullptr, *r = nullptr, *e = nullptr, *z = nullptr;
+  bool ownL = true, ownx = true, ownz = true;
+  int cmask = 0;  // coarsening mask towards the next level
+};
+
+struct Bufs {  // reduction scratch shared by a handle
+  double* partial = nullptr;
+  int64_t npartial = 0;
+  double* dsc = nullptr;   // device doubles
+  void* tsc = nullptr;     // device T scalars
+  double* hsc = nullptr;   // pinned host doubles
+  void* htsc = nullptr;    // pinned host T
+};
+
+template <class T>
+struct MG {
+  std::vector<Level<T>> lv;
+  int perdir = 0;
+  bool multilevel = true;
+  hipStream_t st = nullptr;
+  Bufs* B = nullptr;
+  Prof* P = nullptr;
+
+  ~MG() {
+    for (auto& l : lv) {
+      if (l.ownL) dfree(l.L);
+      if (l.ownx) dfree(l.x);
+      if (l.ownz) dfree(l.z);
+      dfree(l.r);
+      dfree(l.e);
+    }
+  }
+
+  static bool divisible(int N) { return N % 2 == 0 && N > 4; }  // src/MultiLevelPoisson.jl:52
+
+  // Level sizes (src/MultiLevelPoisson.jl:33-41,61-77); level 0 arrays may be supplied.
+  void build(int D, const int* Ng, T* L0, T* x0, T* z0, int maxlevels = 10) {
+    Level<T> l0;
+    l0.g = make_geo(D, Ng);
+    l0.L = L0 ? L0 : dalloc<T>(l0.g.n * D);
+    l0.ownL = !L0;
+    l0.x = x0 ? x0 : dalloc<T>(l0.g.n);
+    l0.ownx = !x0;
+    l0.z = z0 ? z0 : dalloc<T>(l0.g.n);
+    l0.ownz = !z0;
+    l0.r = dalloc<T>(l0.g.n);
+    l0.e = dalloc<T>(l0.g.n);
+    lv.push_back(l0);
+    if (!multilevel) return;
+    while ((int)lv.size() <= maxlevels) {
+      const Geo& f = lv.back().g;
+      int cm = 0, Na[3] = {1, 1, 1};
+      for (int j = 0; j < D; ++j) {
+        const bool c = divisible(f.N[j]);
+        if (c) cm |= 1 << j;
+        Na[j] = c ? 1 + f.N[j] / 2 : f.N[j];
+      }
+      if (!cm) break;
+      lv.back().cmask = cm;
+      Level<T> lc;
+      lc.g = make_geo(D, Na);
+      lc.L = dalloc<T>(lc.g.n * D);
+      lc.x = dalloc<T>(lc.g.n);
+      lc.z = dalloc<T>(lc.g.n);
+      lc.r = dalloc<T>(lc.g.n);
+      lc.e = dalloc<T>(lc.g.n);
+      lv.push_back(lc);
+    }
+    if (lv.size() <= 2) throw WlErr(WL_EINVAL, "MultiLevelPoisson requires size=a2ⁿ, where n>2");
+  }
+
+  void bc_vec(const Geo& g, T* a, const double* U, int saveexit) {
+    ProfScope ps(P, st, "bc", 0, 0.0);
+    int maxplane = 0;
+    for (int d = 0; d < g.D; ++d) {
+      int p = 1;
+      for (int q = 0; q < g.D; ++q) if (q != d) p *= g.N[q];
+      maxplane = std::max(maxplane, p);
+    }
+    dim3 gd((maxplane + 255) / 256, g.D, 2 * g.D + 1);
+    hipLaunchKernelGGL(k_bc<T>, gd, dim3(256), 0, st, g, a, 1, U[0], U[1], g.D > 2 ? U[2] : 0.0, saveexit, perdir);
+    KCK();
+  }
+  void perbc(const Geo& g, T* a) {
+    if (!perdir) return;
+    int maxplane = 0;
+    for (int d = 0; d < g.D; ++d) {
+      int p = 1;
+      for (int q = 0; q < g.D; ++q) if (q != d) p *= g.N[q];
+      maxplane = std::max(maxplane, p);
+    }
+    dim3 gd((maxplane + 255) / 256, 1, 2 * g.D);
+    hipLaunchKernelGGL(k_bc<T>, gd, dim3(256), 0, st, g, a, 0, 0.0, 0.0, 0.0, 0, perdir);
+    KCK();
+  }
+
+  // update!(ml): coarse L by restrictL!, then BC!(L,0) (src/MultiLevelPoisson.jl:42-48,79-86)
+  void update() {
+    const double zero[3] = {0, 0, 0};
+    for (size_t l = 1; l < lv.size(); ++l) {
+      const Level<T>& f = lv[l - 1];
+      Level<T>& c = lv[l];
+      Box b = box_inside(c.g);
+      if (!box_empty(b)) {
+        hipLaunchKernelGGL(k_restrictL<T>, grid_of(b), kBlock, 0, st, f.g, c.g, b, f.cmask, f.L, c.L);
+        KCK();
+      }
+      bc_vec(c.g, c.L, zero, 0);
+    }
+  }
+
+  double finish_sum(int64_t np, int slot, int mode = 0, double ninside = 1) {
+    hipLaunchKernelGGL(k_finish_sum<T>, dim3(1), dim3(256), 0, st, B->partial, np, B->dsc + slot, (T*)B->tsc, mode,
+                       ninside);
+    KCK();
+    return 0;
+  }
+  double read_dsc(int slot) {
+    HIPCK(hipMemcpyAsync(B->hsc, B->dsc + slot, sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCK(hipStreamSynchronize(st));
+    return B->hsc[0];
+  }
+
+  // Σ a·b over the whole array (whole) or inside, rounded to T (src/Poisson.jl:156-157,189)
+  T dot(int l, const T* a, const T* b, bool whole) {
+    const Geo& g = lv[l].g;
+    Box bx = whole ? box_all(g) : box_inside(g);
+    const int64_t nb = nblocks(bx);
+    hipLaunchKernelGGL(k_dot<T>, grid_of(bx), kBlock, 0, st, g, bx, a, b, B->partial);
+    KCK();
+    finish_sum(nb, 2);
+    return (T)read_dsc(2);
+  }
+
+  // ---- residual! on level 0 (src/Poisson.jl:92-98); Σr² into slot 1 ----
+  void residual_generic() {
+    Level<T>& p = lv[0];
+    perbc(p.g, p.x);
+    Box b = box_inside(p.g);
+    const int64_t nb = nblocks(b);
+    hipLaunchKernelGGL((k_residual<T, false>), grid_of(b), kBlock, 0, st, p.g, b, p.L, p.x, p.x, T(1), 0, p.z,
+                       (const T*)nullptr, p.r, B->partial);
+    KCK();
+    finish_sum(nb, 0, 1, (double)ninside(p.g));
+    submean_l2();
+  }
+  void submean_l2() {
+    Level<T>& p = lv[0];
+    Box b = box_inside(p.g);
+    const int64_t nb = nblocks(b);
+    ProfScope ps(P, st, "submean_l2", 0, (double)sizeof(T) * (p.g.D + 3) * ncells_inside(p.g));
+    hipLaunchKernelGGL(k_submean_l2<T>, grid_of(b), kBlock, 0, st, p.g, b, p.L, p.r, p.e, (const T*)B->tsc,
+                       B->partial);
+    KCK();
+    finish_sum(nb, 1);
+  }
+  static int64_t ninside(const Geo& g) {
+    int64_t m = 1;
+    for (int d = 0; d < g.D; ++d) m *= g.N[d] - 2;
+    return m;
+  }
+
+  void eps_init(int l) {
+    Level<T>& p = lv[l];
+    Box b = box_inside(p.g);
+    ProfScope ps(P, st, "eps_init", l, (double)sizeof(T) * (p.g.D + 2) * ncells_inside(p.g));
+    hipLaunchKernelGGL(k_eps_init<T>, grid_of(b), kBlock, 0, st, p.g, b, p.L, p.r, p.e);
+    KCK();
+  }
+  void gs_sweeps(int l) {
+    Level<T>& p = lv[l];
+    perbc(p.g, p.e);
+    Box b = box_half(p.g);
+    if (box_empty(b)) return;
+    for (int it = 1; it <= 4; ++it) {
+      ProfScope ps(P, st, "gs_half", l, (double)sizeof(T) * (p.g.D + 1.5) * ncells_inside(p.g));
+      hipLaunchKernelGGL(k_gs_half<T>, grid_of(b), kBlock, 0, st, p.g, b, p.L, p.r, p.e, it);
+      KCK();
+    }
+  }
+  void increment(int l, T w, bool l2) {
+    Level<T>& p = lv[l];
+    perbc(p.g, p.e);
+    Box b = box_inside(p.g);
+    ProfScope ps(P, st, "increment", l, (double)sizeof(T) * (p.g.D + 5) * ncells_inside(p.g));
+    if (l2)
+      hipLaunchKernelGGL((k_increment<T, true>), grid_of(b), kBlock, 0, st, p.g, b, p.L, p.r, p.x, p.e, w, B->partial);
+    else
+      hipLaunchKernelGGL((k_increment<T, false>), grid_of(b), kBlock, 0, st, p.g, b, p.L, p.r, p.x, p.e, w,
+                         B->partial);
+    KCK();
+    if (l2) finish_sum(nblocks(b), 1);
+  }
+
+  // Vcycle! (src/MultiLevelPoisson.jl:88-101) from the state "ϵ(0) = r·iD"
+  void vcycle(T w) {
+    const int Lm = (int)lv.size() - 1;
+    for (int l = 0; l < Lm; ++l) {  // down-leg: Jacobi! + restrict! + fill!
+      Level<T>& f = lv[l];
+      Level<T>& c = lv[l + 1];
+      perbc(f.g, f.e);
+      Box bc = box_inside(c.g);
+      ProfScope ps(P, st, "inc_restrict", l,
+                   (double)sizeof(T) * ((f.g.D + 5) * ncells_inside(f.g) + 3 * ncells_inside(c.g)));
+      hipLaunchKernelGGL(k_inc_restrict<T>, grid_of(bc), kBlock, 0, st, f.g, c.g, bc, f.cmask, f.L, f.r, f.x, f.e,
+                         c.L, c.r, c.x, c.e);
+      KCK();
+    }
+    for (int c = Lm; c >= 1; --c) {  // up-leg: smooth!(coarse) then prolongate! + increment!(fine)
+      gs_sweeps(c);
+      increment(c, w, false);
+      Level<T>& f = lv[c - 1];
+      Box bf = box_inside(f.g);
+      ProfScope ps(P, st, "prolong_inc", c - 1,
+                   (double)sizeof(T) * ((f.g.D + 5) * ncells_inside(f.g) + ncells_inside(lv[c].g)));
+      hipLaunchKernelGGL(k_prolong_inc<T>, grid_of(bf), kBlock, 0, st, f.g, lv[c].g, bf, f.cmask, perdir, f.L, f.r,
+                         f.x, f.e, (const T*)lv[c].x, w);
+      KCK();
+    }
+  }
+
+  // solver!(ml) loop (src/MultiLevelPoisson.jl:108-127) after residual!: returns V-cycle count
+  int mg_loop(double tol, int itmx, std::vector<double>* hist) {
+    T r2 = (T)read_dsc(1);
+    if (hist) hist->push_back((double)r2);
+    T w = T(1);
+    int n = 0;
+    while (n < itmx) {
+      if (n > 0) eps_init(0);
+      vcycle(w);
+      gs_sweeps(0);
+      increment(0, w, true);
+      const T rnew = (T)read_dsc(1);
+      if (hist) hist->push_back((double)rnew);
+      n++;
+      if (rnew >= r2) w = (T)std::max(0.2, 0.9 * (double)w);
+      else if (rnew < r2) w = (T)std::min(1.0, 1.02 * (double)w);
+      r2 = rnew;
+      if ((double)r2 < tol) break;
+    }
+    perbc(lv[0].g, lv[0].x);
+    return n;
+  }
+
+  // pcg! (src/Poisson.jl:166-186) on level 0
+  void pcg(int it = 6) {
+    Level<T>& p = lv[0];
+    const Geo& g = p.g;
+    Box bi = box_inside(g);
+    const T eps10 = T(10) * std::numeric_limits<T>::epsilon();
+    hipLaunchKernelGGL(k_pcg_init<T>, grid_of(bi), kBlock, 0, st, g, bi, p.L, p.r, p.e, p.z);
+    KCK();
+    T rho = dot(0, p.r, p.z, true);
+    if (std::fabs(rho) < eps10) return;
+    for (int i = 1; i <= it; ++i) {
+      perbc(g, p.e);
+      hipLaunchKernelGGL(k_mult<T>, grid_of(bi), kBlock, 0, st, g, bi, p.L, p.e, p.z);
+      KCK();
+      const T alpha = rho / dot(0, p.z, p.e, perdir == 0);
+      if ((double)std::fabs(alpha) < 1e-2 || (double)std::fabs(alpha) > 1e2) return;
+      hipLaunchKernelGGL(k_pcg_update<T>, grid_of(bi), kBlock, 0, st, g, bi, p.x, p.r, p.e, p.z, alpha);
+      KCK();
+      if (i == it) return;
+      hipLaunchKernelGGL(k_pcg_precond<T>, grid_of(bi), kBlock, 0, st, g, bi, p.L, p.r, p.z);
+      KCK();
+      const T rho2 = dot(0, p.r, p.z, true);
+      if (std::fabs(rho2) < eps10) return;
+      const T beta = rho2 / rho;
+      hipLaunchKernelGGL(k_pcg_dir<T>, grid_of(bi), kBlock, 0, st, g, bi, p.e, p.z, beta);
+      KCK();
+      rho = rho2;
+    }
+  }
+  int pcg_loop(double tol, int itmx) {
+    T r2 = (T)read_dsc(1);
+    int n = 0;
+    while (n < itmx) {
+      pcg();
+      r2 = dot(0, lv[0].r, lv[0].r, true);
+      n++;
+      if ((double)r2 < tol) break;
+    }
+    perbc(lv[0].g, lv[0].x);
+    return n;
+  }
+};
+
+static void alloc_bufs(Bufs& B, int64_t np, size_t tsize) {
+  B.npartial = np;
+  B.partial = dalloc<double>(np);
+  B.dsc = dalloc<double>(8);
+  B.tsc = dalloc<double>(8);
+  (void)tsize;
+  HIPCK(hipHostMalloc((void**)&B.hsc, 8 * sizeof(double), hipHostMallocDefault));
+  HIPCK(hipHostMalloc(&B.htsc, 8 * sizeof(double), hipHostMallocDefault));
+}
+static void free_bufs(Bufs& B) {
+  dfree(B.partial);
+  dfree(B.dsc);
+  dfree(B.tsc);
+  if (B.hsc) (void)hipHostFree(B.hsc);
+  if (B.htsc) (void)hipHostFree(B.htsc);
+}
+
+// ---------------------------------------------------------------------------------------------
+// Flow (src/Flow.jl:114-148) with its MultiLevelPoisson (x ≡ p·dt work buffer, L ≡ μ₀, z ≡ σ)
+// ---------------------------------------------------------------------------------------------
+struct FlowBase {
+  virtual ~FlowBase() {}
+  wl_flow_desc desc;
+  int dtype = WL_F32;
+  std::vector<double> r2hist;
+  virtual size_t field_count(int field) = 0;
+  virtual void* field_ptr(int field) = 0;
+  virtual void measure(const wl_body* b, int nb, double t, double eps) = 0;
+  virtual void pois_update() = 0;
+  virtual int step(double dt, double t0, double t1, double* dt_next, int16_t* nit) = 0;
+  virtual void pressure_force(const wl_body* b, int nb, double t, double* F) = 0;
+  virtual void sync() = 0;
+  virtual Prof& profiler() = 0;
+  double tol = 1e-4;
+  int itmx = 32;
+  int mode = WL_SOLVER_PLAIN;
+};
+
+static DBodies to_dbodies(const wl_body* b, int nb) {
+  if (nb < 1 || nb > 4) throw WlErr(WL_EINVAL, "between 1 and 4 bodies are supported");
+  DBodies d;
+  d.nb = nb;
+  for (int q = 0; q < nb; ++q) {
+    if (b[q].kind != WL_BODY_SPHERE && b[q].kind != WL_BODY_ELLIPSOID) throw WlErr(WL_EINVAL, "unknown body kind");
+    d.b[q].kind = b[q].kind;
+    for (int k = 0; k < 3; ++k) { d.b[q].c[k] = b[q].center[k]; d.b[q].a[k] = b[q].axes[k]; }
+    d.b[q].r = b[q].radius;
+  }
+  return d;
+}
+
+template <class T>
+struct FlowT : FlowBase {
+  Geo g;
+  int D;
+  T *u, *u0, *f, *p, *sigma, *V, *mu0, *mu1, *xw;
+  MG<T> mg;
+  Bufs B;
+  hipStream_t st;
+  Prof prof;
+
+  FlowT(const wl_flow_desc& d) {
+    desc = d;
+    dtype = d.dtype;
+    D = d.D;
+    int Ng[3] = {1, 1, 1};
+    for (int k = 0; k < D; ++k) Ng[k] = d.n[k] + 2;
+    g = make_geo(D, Ng);
+    HIPCK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    const int64_t n = g.n;
+    u = dalloc<T>(n * D); u0 = dalloc<T>(n * D); f = dalloc<T>(n * D); V = dalloc<T>(n * D);
+    mu0 = dalloc<T>(n * D); mu1 = dalloc<T>(n * D * D);
+    p = dalloc<T>(n); sigma = dalloc<T>(n); xw = dalloc<T>(n);
+    alloc_bufs(B, 3 * nblocks(box_all(g)) + 16, sizeof(T));
+    mg.st = st;
+    mg.B = &B;
+    mg.P = &prof;
+    mg.perdir = d.perdir_mask;
+    mg.multilevel = true;
+    mg.build(D, Ng, mu0, xw, sigma);
+    // initial condition u = uBC(i,x,0) everywhere, then BC! (src/Flow.jl:138-141)
+    double U0[3];
+    Ut(0.0, U0);
+    for (int i = 0; i < D; ++i) fill(u + i * n, n, (T)U0[i]);
+    mg.bc_vec(g, u, U0, d.exit_bc);
+    HIPCK(hipMemcpyAsync(u0, u, n * D * sizeof(T), hipMemcpyDeviceToDevice, st));
+    fill(mu0, n * D, T(1));
+    const double zero[3] = {0, 0, 0};
+    mg.bc_vec(g, mu0, zero, 0);
+    mg.update();
+    HIPCK(hipStreamSynchronize(st));
+  }
+  ~FlowT() override {
+    (void)hipStreamSynchronize(st);
+    dfree(u); dfree(u0); dfree(f); dfree(V); dfree(mu0); dfree(mu1); dfree(p); dfree(sigma); dfree(xw);
+    free_bufs(B);
+    mg.lv[0].ownL = mg.lv[0].ownx = mg.lv[0].ownz = false;
+    (void)hipStreamDestroy(st);
+  }
+  void Ut(double t, double* out) const {  // uBC(i,x,t) uniform in x, evaluated in T
+    for (int i = 0; i < 3; ++i) {
+      T v = (T)desc.uBC[i];
+      if (desc.has_uBC_rate) v = (T)desc.uBC[i] + (T)desc.uBC_rate[i] * (T)t;
+      out[i] = (double)v;
+    }
+  }
+  void fill(T* a, int64_t n, T v) {
+    hipLaunchKernelGGL(k_fill<T>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, a, n, v);
+    KCK();
+  }
+  void sync() override { HIPCK(hipStreamSynchronize(st)); }
+  Prof& profiler() override { return prof; }
+  size_t field_count(int field) override {
+    switch (field) {
+      case WL_P: case WL_SIGMA: return g.n;
+      case WL_MU1: return g.n * D * D;
+      case WL_U: case WL_U0: case WL_F: case WL_V: case WL_MU0: return g.n * D;
+    }
+    throw WlErr(WL_EINVAL, "unknown field id");
+  }
+  void* field_ptr(int field) override {
+    switch (field) {
+      case WL_U: return u; case WL_U0: return u0; case WL_F: return f; case WL_P: return p;
+      case WL_SIGMA: return sigma; case WL_V: return V; case WL_MU0: return mu0; case WL_MU1: return mu1;
+    }
+    throw WlErr(WL_EINVAL, "unknown field id");
+  }
+
+  void measure(const wl_body* b, int nb, double t, double eps) override {
+    (void)t;
+    DBodies bs = to_dbodies(b, nb);
+    const int64_t n = g.n;
+    fill(V, n * D, T(0));
+    fill(mu0, n * D, T(1));
+    fill(mu1, n * D * D, T(0));
+    const T d2 = (T)(2 + eps) * (T)(2 + eps);
+    Box bi = box_inside(g);
+    hipLaunchKernelGGL(k_measure_sdf<T>, grid_of(bi), kBlock, 0, st, g, bi, bs, sigma, d2);
+    KCK();
+    hipLaunchKernelGGL(k_measure_fill<T>, grid_of(bi), kBlock, 0, st, g, bi, bs, (const T*)sigma, V, mu0, mu1, d2,
+                       (T)eps);
+    KCK();
+    const double zero[3] = {0, 0, 0};
+    mg.bc_vec(g, mu0, zero, 0);
+    mg.bc_vec(g, V, zero, desc.exit_bc);
+    sync();
+  }
+  void pois_update() override {
+    mg.update();
+    sync();
+  }
+
+  template <int S>
+  void conv_bdim(bool pred, T dt, double t) {
+    Box ba = box_all(g);
+    Accel acc;
+    acc.on = desc.has_g || desc.has_uBC_rate;
+    for (int i = 0; i < 3; ++i) {
+      // accelerate!: r += g(i,x,t) + ∂uBC/∂t (src/Flow.jl:69-73), uniform in space, formed in T
+      T a;
+      if (desc.has_g && desc.has_uBC_rate) a = ((T)desc.g0[i] + (T)desc.g1[i] * (T)t) + (T)desc.uBC_rate[i];
+      else if (desc.has_g) a = (T)desc.g0[i] + (T)desc.g1[i] * (T)t;
+      else a = (T)desc.uBC_rate[i];
+      acc.add[i] = (double)a;
+    }
+    const T nu = (T)desc.nu;
+    {
+    ProfScope ps(&prof, st, "conv_bdim1", 0, (double)sizeof(T) * 4 * D * g.n);
+    if (pred)
+      hipLaunchKernelGGL((k_conv_bdim1<T, S, true>), grid_of(ba), kBlock, 0, st, g, ba, (const T*)u, u0, f, sigma,
+                         (const T*)V, nu, dt, desc.perdir_mask, acc);
+    else
+      hipLaunchKernelGGL((k_conv_bdim1<T, S, false>), grid_of(ba), kBlock, 0, st, g, ba, (const T*)u, u0, f, sigma,
+                         (const T*)V, nu, dt, desc.perdir_mask, acc);
+    KCK();
+    }
+    Box bi = box_inside(g);
+    ProfScope ps(&prof, st, "bdim2", 0, (double)sizeof(T) * (5 * D + D * D) * ncells_inside(g));
+    if (pred)
+      hipLaunchKernelGGL((k_bdim2<T, true>), grid_of(bi), kBlock, 0, st, g, bi, u, (const T*)f, (const T*)V,
+                         (const T*)mu0, (const T*)mu1);
+    else
+      hipLaunchKernelGGL((k_bdim2<T, false>), grid_of(bi), kBlock, 0, st, g, bi, u, (const T*)f, (const T*)V,
+                         (const T*)mu0, (const T*)mu1);
+    KCK();
+  }
+  void momentum(bool pred, T dt, double t, double t1) {
+    switch (desc.scheme) {
+      case WL_QUICK: conv_bdim<WL_QUICK>(pred, dt, t); break;
+      case WL_VANLEER: conv_bdim<WL_VANLEER>(pred, dt, t); break;
+      default: conv_bdim<WL_CDS>(pred, dt, t); break;
+    }
+    double Ub[3];
+    Ut(t1, Ub);
+    mg.bc_vec(g, u, Ub, desc.exit_bc);
+  }
+
+  // mom_project! (src/Flow.jl:223-232)
+  int project(T w, T dt_end, double t1) {
+    const T dt = w * dt_end;
+    Level<T>& l0 = mg.lv[0];
+    Box bi = box_inside(g);
+    const int64_t nb = nblocks(bi);
+    {
+    ProfScope ps(&prof, st, "residual", 0, (double)sizeof(T) * (2 * D + 4) * ncells_inside(g));
+    hipLaunchKernelGGL((k_residual<T, true>), grid_of(bi), kBlock, 0, st, g, bi, (const T*)mu0, (const T*)p, xw, dt, 1,
+                       sigma, (const T*)u, l0.r, B.partial);
+    KCK();
+    }
+    mg.perbc(g, xw);
+    mg.finish_sum(nb, 0, 1, (double)MG<T>::ninside(g));
+    mg.submean_l2();
+    const int nit = mg.mg_loop(tol, itmx, &r2hist);
+    Box ba = box_all(g);
+    {
+    ProfScope ps(&prof, st, "correct", 0, (double)sizeof(T) * (2 * g.n + 3 * D * ncells_inside(g)));
+    hipLaunchKernelGGL(k_correct<T>, grid_of(ba), kBlock, 0, st, g, ba, u, (const T*)mu0, (const T*)xw, p, dt);
+    KCK();
+    }
+    double Ub[3];
+    Ut(t1, Ub);
+    mg.bc_vec(g, u, Ub, desc.exit_bc);
+    return nit;
+  }
+
+  int step(double dt_d, double t0, double t1, double* dt_next, int16_t* nit) override {
+    if (desc.exit_bc) throw WlErr(WL_EINVAL, "exitBC=true is not supported by the device step");
+    r2hist.clear();
+    const T dt = (T)dt_d;
+    momentum(true, dt, t0, t1);
+    const int n1 = project(T(1), dt, t1);
+    momentum(false, dt, t1, t1);
+    const int n2 = project(T(0.5), dt, t1);
+    // CFL (src/Flow.jl:234-237)
+    Box ba = box_all(g);
+    const int64_t nb = nblocks(ba);
+    T* tpart = (T*)B.partial;
+    {
+    ProfScope ps(&prof, st, "cfl", 0, (double)sizeof(T) * (D + 1) * ncells_inside(g));
+    hipLaunchKernelGGL(k_cfl<T>, grid_of(ba), kBlock, 0, st, g, ba, (const T*)u, sigma, tpart);
+    KCK();
+    }
+    hipLaunchKernelGGL(k_finish_max<T>, dim3(1), dim3(256), 0, st, (const T*)tpart, nb, (T*)B.tsc + 2);
+    KCK();
+    HIPCK(hipMemcpyAsync(B.htsc, (T*)B.tsc + 2, sizeof(T), hipMemcpyDeviceToHost, st));
+    HIPCK(hipStreamSynchronize(st));
+    const T m = ((T*)B.htsc)[0];
+    const T nu = (T)desc.nu;
+    const T inv = T(1) / (m + T(5) * nu);
+    const T dtn = std::isnan(inv) ? inv : std::min(T(10), inv);
+    *dt_next = (double)dtn;
+    if (nit) { nit[0] = (int16_t)n1; nit[1] = (int16_t)n2; }
+    if (!std::isfinite(dtn)) return set_err(WL_ENAN, "non-finite CFL time step (NaN/Inf in u)");
+    return 0;
+  }
+
+  void pressure_force(const wl_body* b, int nb, double t, double* F) override {
+    (void)t;
+    DBodies bs = to_dbodies(b, nb);
+    fill(f, g.n * D, T(0));
+    Box bi = box_inside(g);
+    const int64_t np = nblocks(bi);
+    hipLaunchKernelGGL(k_pressure_force<T>, grid_of(bi), kBlock, 0, st, g, bi, bs, (const T*)p, f, B.partial, np);
+    KCK();
+    for (int i = 0; i < D; ++i) {
+      hipLaunchKernelGGL(k_finish_sum<T>, dim3(1), dim3(256), 0, st, B.partial + i * np, np, B.dsc + 3 + i,
+                         (T*)B.tsc, 0, 1.0);
+      KCK();
+    }
+    HIPCK(hipMemcpyAsync(B.hsc, B.dsc + 3, D * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCK(hipStreamSynchronize(st));
+    for (int i = 0; i < D; ++i) F[i] = B.hsc[i];
+  }
+};
+
+struct wl_flow_s {
+  std::unique_ptr<FlowBase> impl;
+};
+
+// ---------------------------------------------------------------------------------------------
+// Standalone Poisson / MultiLevelPoisson
+// ---------------------------------------------------------------------------------------------
+struct PoisBase {
+  virtual ~PoisBase() {}
+  int D = 3, dtype = WL_F32;
+  virtual int nlevels() = 0;
+  virtual void level_dims(int l, int32_t* Ng) = 0;
+  virtual void io(int level, int field, void* host, size_t nbytes, bool up) = 0;
+  virtual void mult(const void* x, void* z, size_t nbytes) = 0;
+  virtual void update() = 0;
+  virtual int solve(double tol, int itmx) = 0;
+};
+
+template <class T>
+struct PoisT : PoisBase {
+  MG<T> mg;
+  Bufs B;
+  hipStream_t st;
+  PoisT(int D_, const int32_t* Ng, int dt, const void* L, bool multilevel, int perdir) {
+    D = D_;
+    dtype = dt;
+    HIPCK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    mg.st = st;
+    mg.B = &B;
+    mg.perdir = perdir;
+    mg.multilevel = multilevel;
+    int N[3] = {Ng[0], Ng[1], D > 2 ? Ng[2] : 1};
+    Geo g = make_geo(D, N);
+    alloc_bufs(B, nblocks(box_all(g)) + 16, sizeof(T));
+    mg.build(D, N, nullptr, nullptr, nullptr);
+    HIPCK(hipMemcpy(mg.lv[0].L, L, g.n * D * sizeof(T), hipMemcpyHostToDevice));
+    mg.update();
+    HIPCK(hipStreamSynchronize(st));
+  }
+  ~PoisT() override {
+    (void)hipStreamSynchronize(st);
+    free_bufs(B);
+    (void)hipStreamDestroy(st);
+  }
+  int nlevels() override { return (int)mg.lv.size(); }
+  void level_dims(int l, int32_t* Ng) override {
+    if (l < 0 || l >= nlevels()) throw WlErr(WL_EINVAL, "level out of range");
+    for (int d = 0; d < D; ++d) Ng[d] = mg.lv[l].g.N[d];
+  }
+  void io(int level, int field, void* host, size_t nbytes, bool up) override {
+    if (level < 0 || level >= nlevels()) throw WlErr(WL_EINVAL, "level out of range");
+    Level<T>& lv = mg.lv[level];
+    const Geo& g = lv.g;
+    size_t cnt = g.n;
+    T* ptr = nullptr;
+    T* tmp = nullptr;
+    switch (field) {
+      case WL_PX: ptr = lv.x; break;
+      case WL_PL: ptr = lv.L; cnt = g.n * D; break;
+      case WL_PEPS: ptr = lv.e; break;
+      case WL_PR: ptr = lv.r; break;
+      case WL_PZ: ptr = lv.z; break;
+      case WL_PD: case WL_PID: {
+        if (up) throw WlErr(WL_EINVAL, "D and iD are derived from L (set L instead)");
+        tmp = dalloc<T>(g.n);
+        Box bi = box_inside(g);
+        hipLaunchKernelGGL(k_diag<T>, grid_of(bi), kBlock, 0, st, g, bi, (const T*)lv.L, tmp, field == WL_PID);
+        KCK();
+        ptr = tmp;
+      } break;
+      default: throw WlErr(WL_EINVAL, "unknown Poisson field id");
+    }
+    if (nbytes != cnt * sizeof(T)) { dfree(tmp); throw WlErr(WL_EINVAL, "buffer size does not match the field"); }
+    HIPCK(hipStreamSynchronize(st));
+    HIPCK(hipMemcpy(up ? (void*)ptr : host, up ? host : (void*)ptr, nbytes,
+                    up ? hipMemcpyHostToDevice : hipMemcpyDeviceToHost));
+    dfree(tmp);
+  }
+  void mult(const void* x, void* z, size_t nbytes) override {
+    Level<T>& p = mg.lv[0];
+    if (nbytes != p.g.n * sizeof(T)) throw WlErr(WL_EINVAL, "buffer size does not match the field");
+    T* xd = dalloc<T>(p.g.n);
+    HIPCK(hipMemcpy(xd, x, nbytes, hipMemcpyHostToDevice));
+    mg.perbc(p.g, xd);
+    hipLaunchKernelGGL(k_fill<T>, dim3((unsigned)((p.g.n + 255) / 256)), dim3(256), 0, st, p.z, p.g.n, T(0));
+    KCK();
+    Box bi = box_inside(p.g);
+    hipLaunchKernelGGL(k_mult<T>, grid_of(bi), kBlock, 0, st, p.g, bi, (const T*)p.L, (const T*)xd, p.z);
+    KCK();
+    HIPCK(hipStreamSynchronize(st));
+    HIPCK(hipMemcpy(z, p.z, nbytes, hipMemcpyDeviceToHost));
+    dfree(xd);
+  }
+  void update() override {
+    if (mg.multilevel) mg.update();
+    HIPCK(hipStreamSynchronize(st));
+  }
+  int solve(double tol, int itmx) override {
+    mg.residual_generic();
+    const int n = mg.multilevel ? mg.mg_loop(tol, itmx, nullptr) : mg.pcg_loop(tol, itmx);
+    HIPCK(hipStreamSynchronize(st));
+    return n;
+  }
+};
+
+struct wl_pois_s {
+  std::unique_ptr<PoisBase> impl;
+};
+
+// =============================================================================================
+// C ABI
+// =============================================================================================
+extern "C" {
+
+const char* wl_last_error(void) { return g_err.c_str(); }
+
+int wl_device_count(int* n) {
+  API_BEGIN
+  HIPCK(hipGetDeviceCount(n));
+  API_END
+}
+
+int wl_flow_create(wl_flow_t* out, const wl_flow_desc* d) {
+  API_BEGIN
+  if (!out || !d) throw WlErr(WL_EINVAL, "null argument");
+  if (d->D != 2 && d->D != 3) throw WlErr(WL_EINVAL, "D must be 2 or 3");
+  for (int k = 0; k < d->D; ++k)
+    if (d->n[k] < 2) throw WlErr(WL_EINVAL, "each dimension needs at least 2 interior cells");
+  if (d->scheme < WL_QUICK || d->scheme > WL_CDS) throw WlErr(WL_EINVAL, "unknown convective scheme");
+  auto* h = new wl_flow_s;
+  try {
+    if (d->dtype == WL_F64) h->impl.reset(new FlowT<double>(*d));
+    else if (d->dtype == WL_F32) h->impl.reset(new FlowT<float>(*d));
+    else throw WlErr(WL_EINVAL, "dtype must be WL_F32 or WL_F64");
+  } catch (...) {
+    delete h;
+    throw;
+  }
+  *out = h;
+  API_END
+}
+int wl_flow_destroy(wl_flow_t h) {
+  API_BEGIN
+  delete h;
+  API_END
+}
+int wl_flow_field_size(wl_flow_t h, int field, size_t* count) {
+  API_BEGIN
+  *count = h->impl->field_count(field);
+  API_END
+}
+static void flow_io(wl_flow_t h, int field, void* buf, size_t nbytes, bool up) {
+  const size_t es = h->impl->dtype == WL_F64 ? 8 : 4;
+  if (nbytes != h->impl->field_count(field) * es) throw WlErr(WL_EINVAL, "buffer size does not match the field");
+  h->impl->sync();
+  void* dp = h->impl->field_ptr(field);
+  HIPCK(hipMemcpy(up ? dp : buf, up ? buf : dp, nbytes, up ? hipMemcpyHostToDevice : hipMemcpyDeviceToHost));
+}
+int wl_flow_upload(wl_flow_t h, int field, const void* src, size_t nbytes) {
+  API_BEGIN
+  flow_io(h, field, (void*)src, nbytes, true);
+  API_END
+}
+int wl_flow_download(wl_flow_t h, int field, void* dst, size_t nbytes) {
+  API_BEGIN
+  flow_io(h, field, dst, nbytes, false);
+  API_END
+}
+int wl_flow_measure(wl_flow_t h, const wl_body* bodies, int nbodies, double t, double eps) {
+  API_BEGIN
+  h->impl->measure(bodies, nbodies, t, eps);
+  API_END
+}
+int wl_flow_pois_update(wl_flow_t h) {
+  API_BEGIN
+  h->impl->pois_update();
+  API_END
+}
+int wl_flow_set_solver(wl_flow_t h, int mode, double tol, int itmx) {
+  API_BEGIN
+  if (mode != WL_SOLVER_PLAIN) throw WlErr(WL_EINVAL, "solver mode not available");
+  if (itmx < 1) throw WlErr(WL_EINVAL, "itmx must be >= 1");
+  h->impl->mode = mode;
+  h->impl->tol = tol;
+  h->impl->itmx = itmx;
+  API_END
+}
+int wl_mom_step(wl_flow_t h, double dt, double t0, double t1, double* dt_next, int16_t* n_iter) {
+  API_BEGIN
+  return h->impl->step(dt, t0, t1, dt_next, n_iter);
+  API_END
+}
+int wl_pressure_force(wl_flow_t h, const wl_body* bodies, int nbodies, double t, double* F) {
+  API_BEGIN
+  h->impl->pressure_force(bodies, nbodies, t, F);
+  API_END
+}
+int wl_flow_profile(wl_flow_t h, int enable) {
+  API_BEGIN
+  h->impl->sync();
+  Prof& P = h->impl->profiler();
+  P.on = enable != 0;
+  P.recs.clear();
+  P.used = 0;
+  API_END
+}
+int wl_flow_profile_read(wl_flow_t h, wl_kernel_stat* out, int cap, int* count) {
+  API_BEGIN
+  h->impl->sync();
+  Prof& P = h->impl->profiler();
+  std::vector<wl_kernel_stat> agg;
+  for (const auto& r : P.recs) {
+    float ms = 0;
+    HIPCK(hipEventElapsedTime(&ms, r.a, r.b));
+    wl_kernel_stat* s = nullptr;
+    for (auto& a : agg)
+      if (!std::strcmp(a.name, r.key) && a.level == r.level) { s = &a; break; }
+    if (!s) {
+      wl_kernel_stat z;
+      std::memset(&z, 0, sizeof(z));
+      std::snprintf(z.name, sizeof(z.name), "%s", r.key);
+      z.level = r.level;
+      z.bytes_per_launch = r.bytes;
+      agg.push_back(z);
+      s = &agg.back();
+    }
+    s->launches += 1;
+    s->total_ms += ms;
+  }
+  P.recs.clear();
+  P.used = 0;
+  for (int k = 0; k < (int)agg.size() && k < cap; ++k) out[k] = agg[k];
+  *count = (int)agg.size();
+  API_END
+}
+int wl_flow_last_residuals(wl_flow_t h, double* r2, int cap, int* count) {
+  API_BEGIN
+  const auto& v = h->impl->r2hist;
+  for (int k = 0; k < (int)v.size() && k < cap; ++k) r2[k] = v[k];
+  *count = (int)v.size();
+  API_END
+}
+
+int wl_pois_create(wl_pois_t* out, int D, const int32_t* Ng, int dtype, const void* L, int multilevel,
+                   int perdir_mask) {
+  API_BEGIN
+  if (!out || !Ng || !L) throw WlErr(WL_EINVAL, "null argument");
+  if (D != 2 && D != 3) throw WlErr(WL_EINVAL, "D must be 2 or 3");
+  for (int k = 0; k < D; ++k)
+    if (Ng[k] < 3) throw WlErr(WL_EINVAL, "each dimension needs at least 3 cells");
+  auto* h = new wl_pois_s;
+  try {
+    if (dtype == WL_F64) h->impl.reset(new PoisT<double>(D, Ng, dtype, L, multilevel != 0, perdir_mask));
+    else if (dtype == WL_F32) h->impl.reset(new PoisT<float>(D, Ng, dtype, L, multilevel != 0, perdir_mask));
+    else throw WlErr(WL_EINVAL, "dtype must be WL_F32 or WL_F64");
+  } catch (...) {
+    delete h;
+    throw;
+  }
+  *out = h;
+  API_END
+}
+int wl_pois_destroy(wl_pois_t h) {
+  API_BEGIN
+  delete h;
+  API_END
+}
+int wl_pois_nlevels(wl_pois_t h, int* n) {
+  API_BEGIN
+  *n = h->impl->nlevels();
+  API_END
+}
+int wl_pois_level_dims(wl_pois_t h, int level, int32_t* Ng) {
+  API_BEGIN
+  h->impl->level_dims(level, Ng);
+  API_END
+}
+int wl_pois_upload(wl_pois_t h, int level, int pfield, const void* src, size_t nbytes) {
+  API_BEGIN
+  h->impl->io(level, pfield, (void*)src, nbytes, true);
+  API_END
+}
+int wl_pois_download(wl_pois_t h, int level, int pfield, void* dst, size_t nbytes) {
+  API_BEGIN
+  h->impl->io(level, pfield, dst, nbytes, false);
+  API_END
+}
+int wl_pois_mult(wl_pois_t h, const void* x, void* z_out, size_t nbytes) {
+  API_BEGIN
+  h->impl->mult(x, z_out, nbytes);
+  API_END
+}
+int wl_pois_update(wl_pois_t h) {
+  API_BEGIN
+  h->impl->update();
+  API_END
+}
+int wl_pois_solve(wl_pois_t h, double tol, int itmx, int32_t* n_iter) {
+  API_BEGIN
+  if (itmx < 0) throw WlErr(WL_EINVAL, "itmx must be >= 0");
+  *n_iter = h->impl->solve(tol, itmx);
+  API_END
+}
+
+}  // extern "C"
