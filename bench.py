@@ -66,6 +66,7 @@ def roofline(stats, traffic_file):
         with open(traffic_file) as fh:
             tt = json.load(fh)
         traffic = tt.get(dom["kernel"])
+        traffic = None if traffic is None else round(traffic)
     total_b = sum(s["bytes_per_launch"] * s["launches"] for s in stats)
     total_ms = sum(s["total_ms"] for s in stats)
     return dict(bound="hbm", kernel=dom["kernel"], achieved=dom["GBs"], peak=HBM_PEAK_GBS, unit="GB/s",
@@ -104,6 +105,68 @@ def copy_calibration(nbytes=1 << 30, reps=5):
     print(json.dumps({"copy_calibration_bytes": nbytes, "reps": reps}), flush=True)
 
 
+# rocprofv3 kernel names -> the library's profiler keys (level 0 = the largest dispatch of a name)
+PROF_KEYS = {"k_conv_bdim1": "conv_bdim1", "k_bdim2": "bdim2", "k_residual": "residual", "k_submean_l2": "submean_l2",
+             "k_inc_restrict": "inc_restrict", "k_gs_half": "gs_half", "k_increment": "increment",
+             "k_prolong_inc": "prolong_inc", "k_eps_init": "eps_init", "k_correct": "correct", "k_cfl": "cfl"}
+
+
+def _key_of(kname):
+    for k, v in PROF_KEYS.items():
+        if f"wl::{k}<" in kname or f"wl::{k}(" in kname:
+            return v
+    return None
+
+
+def summarize(stats_csv, fetch_csv, write_csv, out_md, out_json, cmd, calib_bytes=1 << 30):
+    """Write the kernel-time summary and the HBM traffic per level-0 launch (FETCH_SIZE/WRITE_SIZE
+    scaled by what they read for a device copy of calib_bytes)."""
+    import collections
+    import csv
+    rows = list(csv.DictReader(open(stats_csv)))
+    lines = [f"# rocprofv3 kernel statistics", "", f"Command: `{cmd}`", "",
+             "| kernel | calls | avg us | total ms | % |", "|---|---|---|---|---|"]
+    for r in rows[:24]:
+        nm = r["Name"].split("(")[0].replace("void ", "")
+        lines.append(f"| {nm} | {r['Calls']} | {float(r['AverageNs']) / 1e3:.2f} | "
+                     f"{float(r['TotalDurationNs']) / 1e6:.3f} | {r['Percentage']} |")
+
+    def counters(fn, cname):
+        per = collections.defaultdict(list)
+        copies = []
+        for r in csv.DictReader(open(fn)):
+            if r["Counter_Name"] != cname:
+                continue
+            v = float(r["Counter_Value"])
+            if "copyBuffer" in r["Kernel_Name"] and v > 1000:
+                copies.append(v)
+            k = _key_of(r["Kernel_Name"])
+            if k:
+                per[(k, r["Kernel_Name"])].append(v)
+        return per, copies
+
+    fetch, fc = counters(fetch_csv, "FETCH_SIZE")
+    write, wc = counters(write_csv, "WRITE_SIZE")
+    f_scale = calib_bytes / (float(np.median(fc)) * 1024) if fc else None
+    w_scale = calib_bytes / (float(np.median(wc)) * 1024) if wc else None
+    traffic = collections.defaultdict(list)
+    lines += ["", f"## HBM traffic per level-0 launch (rocprofv3 --pmc FETCH_SIZE, then WRITE_SIZE)", "",
+              f"Copy calibration: a {calib_bytes >> 20} MiB device copy reads FETCH_SIZE = "
+              f"{np.median(fc) if fc else float('nan'):.0f} KiB and WRITE_SIZE = {np.median(wc) if wc else float('nan'):.0f} KiB, "
+              f"so bytes = FETCH_SIZE x 1024 x {f_scale:.3f} and WRITE_SIZE x 1024 x {w_scale:.3f}.", "",
+              "| kernel | fetch MB | write MB | traffic MB |", "|---|---|---|---|"]
+    for (k, kn), vals in sorted(fetch.items()):
+        fv = max(vals) * 1024 * f_scale
+        wv = max(write.get((k, kn), [0.0])) * 1024 * w_scale
+        traffic[k].append(fv + wv)
+        lines.append(f"| {kn.split('(')[0].replace('void ', '')} | {fv / 1e6:.1f} | {wv / 1e6:.1f} | {(fv + wv) / 1e6:.1f} |")
+    tj = {f"{k}@L0": float(np.mean(v)) for k, v in traffic.items()}
+    with open(out_md, "w") as fh:
+        fh.write("\n".join(lines) + "\n")
+    with open(out_json, "w") as fh:
+        json.dump(dict(tj, _fetch_scale=f_scale, _write_scale=w_scale), fh, indent=1)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -114,7 +177,12 @@ def main():
     ap.add_argument("--c3-steps", type=int, default=20)
     ap.add_argument("--copy-calib", action="store_true")
     ap.add_argument("--quiet-kernels", action="store_true")
+    ap.add_argument("--summarize", nargs=6, metavar=("STATS", "FETCH", "WRITE", "OUT_MD", "OUT_JSON", "CMD"),
+                    help="turn rocprofv3 CSVs into the profile summary and the traffic table")
     a = ap.parse_args()
+    if a.summarize:
+        summarize(*a.summarize)
+        return
     if a.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {a.gpus} is not supported (the flow is stepped on a single "
               f"MI355X; see the sharding design note)", file=sys.stderr)
