@@ -128,6 +128,10 @@ typedef struct {
 } wl_kernel_stat;
 int wl_flow_profile(wl_flow_t h, int enable);
 int wl_flow_profile_read(wl_flow_t h, wl_kernel_stat* out, int cap, int* count);
+/* Device self-tests, no counterpart in the reference.  test 0: the number of finite floats x for
+ * which the kernels' division by 6 differs in any bit from the IEEE quotient (expected 0 apart
+ * from the ranges it delegates to the division itself). */
+int wl_self_test(int test, int64_t* result);
 
 /* ---- Poisson / MultiLevelPoisson: src/Poisson.jl, src/MultiLevelPoisson.jl ---- */
 /* Ng: ghosted sizes; L: Ng x D coefficients (host, layout as above); x, z start at zero. */

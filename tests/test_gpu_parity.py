@@ -229,3 +229,14 @@ def test_mg_random_matches_oracle(wl):
     o.solve()
     say(f"  n gpu {g.n} oracle {o.n}; rel-L2 x {rel(g.x, o.get('x')):.3e}")
     assert g.n == o.n and rel(g.x, o.get("x")) < 1e-5
+
+
+def test_device_div6_is_ieee(wl):
+    """The kernels' division by 6 (reciprocal multiply + FMA correction in the normal range) gives
+    the IEEE quotient for every finite float (exhaustive, on the device)."""
+    import ctypes
+    from waterlily_amd import _abi
+    say("case: exhaustive div6 self-test")
+    res = ctypes.c_int64(-1)
+    _abi.check(_abi.load().wl_self_test(0, ctypes.byref(res)))
+    assert res.value == 0

@@ -59,6 +59,7 @@ SIGNATURES = {
     "wl_flow_last_residuals": (_i, [_vp, _P(_d), _i, _P(_i)]),
     "wl_flow_profile": (_i, [_vp, _i]),
     "wl_flow_profile_read": (_i, [_vp, _P(WlKernelStat), _i, _P(_i)]),
+    "wl_self_test": (_i, [_i, _P(C.c_int64)]),
     "wl_pois_create": (_i, [_P(_vp), _i, _P(C.c_int32), _i, _vp, _i, _i]),
     "wl_pois_destroy": (_i, [_vp]),
     "wl_pois_nlevels": (_i, [_vp, _P(_i)]),

@@ -48,6 +48,34 @@ __device__ __forceinline__ int64_t at(const Geo& g, int i, int j, int k) {
 constexpr int BX = 64, BY = 4;  // one wavefront along x, four along y
 constexpr int RED_THREADS = BX * BY;
 
+// A z-marching tile set: blocks of BX x BY threads, each thread owns one (x,y) column of a tile
+// and walks zc consecutive planes, so the z±1 neighbours of a plane were touched by the same
+// block one iteration earlier.  Tiles are numbered x fastest, then y, then z.
+struct ZT {
+  Box b;         // cells covered
+  int nbx, nby;  // tiles per row / per plane
+  int sx, sy;    // tile strides in x and y
+  int zc;        // planes per tile
+};
+
+// Workgroups are dealt round-robin to the 8 XCDs (each with its own L2).  Re-number them so that
+// XCD k receives one contiguous range of tiles: neighbouring tiles, which read each other's halo
+// rows and planes, then share an L2.  The map is a bijection for any grid size.
+__device__ __forceinline__ int xcd_tile_id() {
+  const int nb = gridDim.x, bid = blockIdx.x;
+  const int q = nb >> 3, rr = nb & 7, x = bid & 7, k = bid >> 3;
+  return x * q + (x < rr ? x : rr) + k;
+}
+__device__ __forceinline__ void zt_cell(const ZT& t, int& i, int& j, int& z0, int& z1) {
+  const int lid = xcd_tile_id();
+  const int tx = lid % t.nbx, r1 = lid / t.nbx;
+  const int ty = r1 % t.nby, tz = r1 / t.nby;
+  i = t.b.lo[0] + tx * t.sx + (int)threadIdx.x;
+  j = t.b.lo[1] + ty * t.sy + (int)threadIdx.y;
+  z0 = t.b.lo[2] + tz * t.zc;
+  z1 = z0 + t.zc - 1 < t.b.hi[2] ? z0 + t.zc - 1 : t.b.hi[2];
+}
+
 // ---------------------------------------------------------------------------------------------
 // wave64 + block reductions (deterministic for a fixed launch shape)
 // ---------------------------------------------------------------------------------------------
@@ -113,10 +141,24 @@ __device__ __forceinline__ T med3(T a, T b, T c) {
   }
   return a;
 }
+// x/6 rounded to nearest.  For float, a reciprocal multiply with one FMA correction (Markstein)
+// equals the IEEE quotient for every |x| in [2^-100, 2^100) (checked for all floats on the CPU
+// and on the device by wl_self_test); outside that range the division is done directly.
+__device__ __forceinline__ float div6(float x) {
+  const float ax = fabsf(x);
+  if (ax >= 0x1p-100f && ax < 0x1p100f) {
+    const float r = 0.16666667f;
+    const float q0 = x * r;
+    return __builtin_fmaf(__builtin_fmaf(-q0, 6.0f, x), r, q0);
+  }
+  return x / 6.0f;
+}
+__device__ __forceinline__ double div6(double x) { return x / 6.0; }
+
 template <class T, int S>
 __device__ __forceinline__ T lam(T u, T c, T d) {
   if constexpr (S == WL_QUICK) {
-    return med3<T>((T(5) * c + T(2) * d - u) / T(6), c, med3<T>(T(10) * c - T(9) * u, c, d));
+    return med3<T>(div6(T(5) * c + T(2) * d - u), c, med3<T>(T(10) * c - T(9) * u, c, d));
   } else if constexpr (S == WL_VANLEER) {
     return (c <= fmin(u, d) || c >= fmax(u, d)) ? c : c + (d - c) * (c - u) / (d - u);
   } else {

@@ -19,99 +19,149 @@ struct Accel {
   double add[3];
 };
 
-// Face flux F_j^i at the lower j-face of cell I (src/Flow.jl:47,56-57,61): variant 0 inner (ϕu),
-// 1 inlet (ϕuL), 2 outlet (ϕuR), 3 periodic (ϕuP with the image index Ip).
+// Face flux F_j^i at the lower j-face of cell I (src/Flow.jl:8-11,47,56-57,61): variant 0 inner
+// (ϕu), 1 inlet (ϕuL), 2 outlet (ϕuR), 3 periodic (ϕuP with the image index Ip).  The three λ
+// arguments are selected first so each flux costs one λ evaluation whatever the sign of u.
 template <class T, int S>
 __device__ __forceinline__ T face_flux(int var, const T* __restrict__ ui, const T* __restrict__ uj, int64_t I,
                                        int64_t si, int64_t sj, int64_t Ip, T nu) {
   const T ua = (uj[I] + uj[I - si]) / T(2);
   const T fI = ui[I], fm = ui[I - sj];
+  const bool cd = (var == 1 && ua > 0) || (var == 2 && ua < 0);
   T phi;
-  if (var == 0) {
-    phi = ua > 0 ? ua * lam<T, S>(ui[I - 2 * sj], fm, fI) : ua * lam<T, S>(ui[I + sj], fI, fm);
-  } else if (var == 1) {
-    phi = ua > 0 ? ua * ((fI + fm) / T(2)) : ua * lam<T, S>(ui[I + sj], fI, fm);
-  } else if (var == 2) {
-    phi = ua < 0 ? ua * ((fI + fm) / T(2)) : ua * lam<T, S>(ui[I - 2 * sj], fm, fI);
+  if (cd) {
+    phi = ua * ((fI + fm) / T(2));
   } else {
-    phi = ua > 0 ? ua * lam<T, S>(ui[Ip], fm, fI) : ua * lam<T, S>(ui[I + sj], fI, fm);
+    const bool pos = var == 2 ? !(ua < 0) : ua > 0;  // upwind side of the face
+    const int64_t ix = pos ? (var == 3 ? Ip : I - 2 * sj) : I + sj;
+    const T fx = ui[ix];
+    phi = ua * lam<T, S>(fx, pos ? fm : fI, pos ? fI : fm);
   }
   return phi - nu * (fI - fm);
 }
 
-// f[I,i] = u⁰ + dt·(Σ_j F_j(I) - F_j(I+e_j) [+ accel]) - V on every cell (ghosts included).
+// Flux through the lower j-face of cell c (all indices >= 1), with the variant its position
+// implies: inlet face (c_j = 1), interior, outlet face (c_j = N_j-1), periodic faces.
+template <class T, int S>
+__device__ __forceinline__ T lower_flux(const Geo& g, const T* __restrict__ u, int i, int j, const int* c, int64_t I,
+                                        int perdir, T nu) {
+  const int Nj = g.N[j];
+  const int64_t sj = g.s[j], si = g.s[i];
+  const T* ui = u + i * g.n;
+  const T* uj = u + j * g.n;
+  const bool per = (perdir >> j) & 1;
+  if (c[j] == 1) return face_flux<T, S>(per ? 3 : 1, ui, uj, I, si, sj, I + (int64_t)(Nj - 4) * sj, nu);
+  if (c[j] == Nj - 1) {
+    if (!per) return face_flux<T, S>(2, ui, uj, I, si, sj, 0, nu);
+    const int64_t K = I - (int64_t)(Nj - 2) * sj;  // the periodic image face (c_j = 1)
+    return face_flux<T, S>(3, ui, uj, K, si, sj, K + (int64_t)(Nj - 4) * sj, nu);
+  }
+  return face_flux<T, S>(0, ui, uj, I, si, sj, 0, nu);
+}
+
+// f[I,i] = u⁰ + dt·(Σ_j F_j(I) - F_j(I+e_j) [+ accel]) - V on every cell (ghosts included),
+// accumulated in the reference's order j = 1..D (src/Flow.jl:41-52 then :178).
+// Each thread owns an (x,y) column and walks z; every face flux is evaluated once: the flux
+// through the upper x face comes from the next lane, the upper y face from the next row (LDS),
+// the upper z face from the next plane (so plane z is finished one iteration later).  Tiles
+// overlap by one column in x and one row in y (the last lane/row only supplies fluxes).
 // PRED: the convecting field is u (== u⁰ of this step) and u⁰ := u is written here.
 // σ receives the conv_diff! scratch Φ on the top ghost layers (the last (i=D, j) flux written
 // there by the reference), which the CFL maximum reads (src/Flow.jl:236).
-template <class T, int S, bool PRED>
-__global__ void __launch_bounds__(256) k_conv_bdim1(Geo g, Box b, const T* __restrict__ u, T* __restrict__ u0,
-                                                    T* __restrict__ f, T* __restrict__ sigma,
-                                                    const T* __restrict__ V, T nu, T dt, int perdir, Accel acc) {
-  int c[3];
-  if (!box_cell(b, c[0], c[1], c[2])) return;
-  const int D = g.D;
-  const int64_t I = at(g, c[0], c[1], c[2]);
+constexpr int CBY = 8;
+template <class T, int S, bool PRED, int D>
+__global__ void __launch_bounds__(64 * CBY) k_conv_bdim1(Geo g, ZT t, const T* __restrict__ u, T* __restrict__ u0,
+                                                         T* __restrict__ f, T* __restrict__ sigma,
+                                                         const T* __restrict__ V, T nu, T dt, int perdir,
+                                                         Accel acc) {
+  __shared__ T sFy[3][CBY][64];
+  int cx, cy, z0, z1;
+  zt_cell(t, cx, cy, z0, z1);
+  const int tx = threadIdx.x, ty = threadIdx.y;
   const int64_t n = g.n;
-  bool allge1 = true, anytop = false;
-  for (int d = 0; d < D; ++d) { allge1 &= c[d] >= 1; anytop |= c[d] == g.N[d] - 1; }
-  for (int i = 0; i < D; ++i) {
-    const T* ui = u + i * n;
-    const int64_t si = g.s[i];
-    T r = 0;
-    if (allge1) {
-      for (int j = 0; j < D; ++j) {
-        const int Nj = g.N[j];
-        if (c[j] > Nj - 2) continue;  // needs c_j in 1..N_j-2 (others are already in 1..N-1)
-        const T* uj = u + j * n;
-        const int64_t sj = g.s[j];
-        const bool per = (perdir >> j) & 1;
-        // + flux through the lower face
-        const int vP = c[j] == 1 ? (per ? 3 : 1) : 0;
-        r += face_flux<T, S>(vP, ui, uj, I, si, sj, I + (int64_t)(Nj - 4) * sj, nu);
-        // - flux through the upper face
-        T M;
-        if (c[j] + 1 == Nj - 1) {
-          if (per) {
-            const int64_t K = I - (int64_t)(Nj - 3) * sj;  // the periodic image face (c_j = 1)
-            M = face_flux<T, S>(3, ui, uj, K, si, sj, K + (int64_t)(Nj - 4) * sj, nu);
-          } else {
-            M = face_flux<T, S>(2, ui, uj, I + sj, si, sj, 0, nu);
-          }
-        } else {
-          M = face_flux<T, S>(0, ui, uj, I + sj, si, sj, 0, nu);
+  const bool ingrid = cx <= g.N[0] - 1 && cy <= g.N[1] - 1;
+  const bool outxy = ingrid && tx < 63 && ty < CBY - 1;
+  const int zend = (D == 3 && z1 + 1 <= g.N[2] - 1) ? z1 + 1 : z1;
+  T rpart[3] = {0, 0, 0}, Fzp[3] = {0, 0, 0};
+
+  auto finish = [&](int cz, const T* rr) {
+    const int64_t I = at(g, cx, cy, cz);
+    for (int i = 0; i < D; ++i) {
+      T r = rr[i];
+      if (acc.on) r += (T)acc.add[i];
+      const T uv = u[I + i * n];
+      const T u0v = PRED ? uv : u0[I + i * n];
+      f[I + i * n] = u0v + dt * r - V[I + i * n];
+      if (PRED) u0[I + i * n] = uv;
+    }
+  };
+
+  for (int cz = z0; cz <= zend; ++cz) {
+    const int c[3] = {cx, cy, cz};
+    const bool allge1 = cx >= 1 && cy >= 1 && (D == 2 || cz >= 1);
+    const bool valid = ingrid && allge1;
+    const int64_t I = valid ? at(g, cx, cy, cz) : 0;
+    T F[3][3];
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) F[i][j] = 0;
+    if (valid)
+      for (int i = 0; i < D; ++i)
+        for (int j = 0; j < D; ++j) F[i][j] = lower_flux<T, S>(g, u, i, j, c, I, perdir, nu);
+    for (int i = 0; i < D; ++i) sFy[i][ty][tx] = F[i][1];
+    __syncthreads();
+    T Fyu[3] = {0, 0, 0};
+    if (ty < CBY - 1)
+      for (int i = 0; i < D; ++i) Fyu[i] = sFy[i][ty + 1][tx];
+    __syncthreads();
+    T Fxu[3];
+    for (int i = 0; i < 3; ++i) Fxu[i] = __shfl_down(F[i][0], 1, 64);
+    T rxy[3] = {0, 0, 0};
+    if (valid)
+      for (int i = 0; i < D; ++i) {
+        T r = 0;
+        if (cx <= g.N[0] - 2) { r += F[i][0]; r -= Fxu[i]; }
+        if (cy <= g.N[1] - 2) { r += F[i][1]; r -= Fyu[i]; }
+        rxy[i] = r;
+      }
+    if (D == 2) {
+      if (outxy) finish(cz, rxy);
+    } else {
+      if (cz > z0 && outxy) {  // plane cz-1 is complete: add its z fluxes
+        const bool zin = cx >= 1 && cy >= 1 && cz - 1 >= 1 && cz - 1 <= g.N[2] - 2;
+        T rr[3];
+        for (int i = 0; i < 3; ++i) {
+          rr[i] = rpart[i];
+          if (zin) { rr[i] += Fzp[i]; rr[i] -= F[i][2]; }
         }
-        r -= M;
+        finish(cz - 1, rr);
+      }
+      for (int i = 0; i < 3; ++i) { rpart[i] = rxy[i]; Fzp[i] = F[i][2]; }
+    }
+    // Φ on the top ghost layers (component D, last direction whose inner range holds the cell)
+    if (valid && outxy && cz <= z1) {
+      bool anytop = false;
+      for (int d = 0; d < D; ++d) anytop |= c[d] == g.N[d] - 1;
+      if (anytop) {
+        int js = -1;
+        for (int j = 0; j < D; ++j)
+          if ((c[j] >= 2 && c[j] <= g.N[j] - 2) || (c[j] == 1 && ((perdir >> j) & 1))) js = j;
+        if (js >= 0) sigma[I] = F[D - 1][js];
       }
     }
-    if (acc.on) r += (T)acc.add[i];
-    const T uv = ui[I];
-    const T u0v = PRED ? uv : u0[I + i * n];
-    f[I + i * n] = u0v + dt * r - V[I + i * n];
-    if (PRED) u0[I + i * n] = uv;
   }
-  // Φ on the top ghost layers (component D, last direction j whose inner range holds the cell)
-  if (allge1 && anytop) {
-    int js = -1, var = 0;
-    for (int j = 0; j < D; ++j) {
-      if (c[j] >= 2 && c[j] <= g.N[j] - 2) { js = j; var = 0; }
-      else if (c[j] == 1 && ((perdir >> j) & 1)) { js = j; var = 3; }
-    }
-    if (js >= 0) {
-      const int i = D - 1;
-      sigma[I] = face_flux<T, S>(var, u + i * n, u + js * n, I, g.s[i], g.s[js],
-                                 I + (int64_t)(g.N[js] - 4) * g.s[js], nu);
-    }
-  }
+  if (D == 3 && zend == z1 && outxy) finish(z1, rpart);  // top ghost plane: no z fluxes
 }
 
 // BDIM! second half + scale_u!: u += μddn(μ₁,f) + V + μ₀·f on inside (src/Flow.jl:179,211-214).
 // PRED: u was scaled by 0 before (mom_step!, :157); CORR: the result is scaled by ½ (:209).
 template <class T, bool PRED>
-__global__ void __launch_bounds__(256) k_bdim2(Geo g, Box b, T* __restrict__ u, const T* __restrict__ f,
+__global__ void __launch_bounds__(256) k_bdim2(Geo g, ZT t, T* __restrict__ u, const T* __restrict__ f,
                                                const T* __restrict__ V, const T* __restrict__ mu0,
                                                const T* __restrict__ mu1) {
-  int c0, c1, c2;
-  if (!box_cell(b, c0, c1, c2)) return;
+  int c0, c1, z0, z1;
+  zt_cell(t, c0, c1, z0, z1);
+  if (c0 > t.b.hi[0] || c1 > t.b.hi[1]) return;
+  for (int c2 = z0; c2 <= z1; ++c2) {
   const int D = g.D;
   const int64_t I = at(g, c0, c1, c2);
   const int64_t n = g.n;
@@ -122,6 +172,7 @@ __global__ void __launch_bounds__(256) k_bdim2(Geo g, Box b, T* __restrict__ u, 
     const T inc = s / T(2) + V[k] + mu0[k] * f[k];
     if (PRED) u[k] = u[k] * T(0) + inc;
     else u[k] = (u[k] + inc) * T(0.5);
+  }
   }
 }
 
@@ -213,138 +264,30 @@ __device__ __forceinline__ T mult_at(const Geo& g, const T* __restrict__ L, int6
   return s;
 }
 
-// ---- residual! (src/Poisson.jl:92-98) fused with mom_project!'s z = div(u) and x .*= dt
-// (src/Flow.jl:225).  xin·scale is written to xout; with FLOW the source z is div(u).
-template <class T, bool FLOW>
-__global__ void __launch_bounds__(256) k_residual(Geo g, Box b, const T* __restrict__ L, const T* __restrict__ xin,
-                                                  T* __restrict__ xout, T scale, int write_x, T* __restrict__ z,
-                                                  const T* __restrict__ u, T* __restrict__ r,
-                                                  double* __restrict__ partial) {
-  int c0, c1, c2;
-  double acc = 0;
-  if (box_cell(b, c0, c1, c2)) {
-    const int64_t I = at(g, c0, c1, c2);
-    T zv;
-    if (FLOW) {
-      T s = 0;
-      for (int i = 0; i < g.D; ++i) s += (u[I + g.s[i] + i * g.n] - u[I + i * g.n]);
-      zv = s;
-      z[I] = zv;
-    } else {
-      zv = z[I];
-    }
-    const T Dg = diag_at(g, L, I);
-    const T iD = inv_diag(Dg);
-    auto xv = [&](int64_t J) { return xin[J] * scale; };
-    const T rv = iD == T(0) ? T(0) : zv - mult_at(g, L, I, Dg, xv);
-    r[I] = rv;
-    if (write_x) xout[I] = xv(I);
-    acc = (double)rv;
-  }
-  const double s = block_sum(acc);
-  if (threadIdx.x == 0 && threadIdx.y == 0) partial[block_linear_id()] = s;
-}
-
-// r -= s (when |s| > 2eps) ; Σr² partials ; ϵ = r·iD (Jacobi!/GS init of the first V-cycle)
+// ---- per-cell bodies shared by the tiled kernels and the single-workgroup coarse tail ----
+// increment! at cell I: r -= ω·Aϵ; x += ω·ϵ (src/Poisson.jl:100-104); returns the new r
 template <class T>
-__global__ void __launch_bounds__(256) k_submean_l2(Geo g, Box b, const T* __restrict__ L, T* __restrict__ r,
-                                                    T* __restrict__ e, const T* __restrict__ sdev,
-                                                    double* __restrict__ partial) {
-  int c0, c1, c2;
-  double acc = 0;
-  if (box_cell(b, c0, c1, c2)) {
-    const int64_t I = at(g, c0, c1, c2);
-    T rv = r[I];
-    const T s = sdev[0];
-    if (sdev[1] != T(0)) { rv = rv - s; r[I] = rv; }
-    e[I] = rv * inv_diag(diag_at(g, L, I));
-    acc = (double)rv * (double)rv;
-  }
-  const double t = block_sum(acc);
-  if (threadIdx.x == 0 && threadIdx.y == 0) partial[block_linear_id()] = t;
+__device__ __forceinline__ T increment_cell(const Geo& g, const T* __restrict__ L, T* __restrict__ r,
+                                            T* __restrict__ x, const T* __restrict__ e, T w, int64_t I) {
+  const T Dg = diag_at(g, L, I);
+  auto xv = [&](int64_t J) { return e[J]; };
+  const T rn = r[I] - w * mult_at(g, L, I, Dg, xv);
+  r[I] = rn;
+  x[I] = x[I] + w * e[I];
+  return rn;
 }
-
-// ϵ = r·iD on inside (Jacobi!/GS-RB init, src/Poisson.jl:112,142)
+// one GS-RB update at half-range cell c (src/Poisson.jl:116-128): the last index is doubled and
+// shifted so colour `it` updates cells whose 1-based index sum is even for odd it
 template <class T>
-__global__ void __launch_bounds__(256) k_eps_init(Geo g, Box b, const T* __restrict__ L, const T* __restrict__ r,
-                                                  T* __restrict__ e) {
-  int c0, c1, c2;
-  if (!box_cell(b, c0, c1, c2)) return;
-  const int64_t I = at(g, c0, c1, c2);
-  e[I] = r[I] * inv_diag(diag_at(g, L, I));
-}
-
-// increment! (src/Poisson.jl:100-104): r -= ω·Aϵ; x += ω·ϵ.  Optionally the Σr² partials (L₂ of
-// the finest level after a V-cycle, src/MultiLevelPoisson.jl:115).
-template <class T, bool L2>
-__global__ void __launch_bounds__(256) k_increment(Geo g, Box b, const T* __restrict__ L, T* __restrict__ r,
-                                                   T* __restrict__ x, const T* __restrict__ e, T w,
-                                                   double* __restrict__ partial) {
-  int c0, c1, c2;
-  double acc = 0;
-  if (box_cell(b, c0, c1, c2)) {
-    const int64_t I = at(g, c0, c1, c2);
-    const T Dg = diag_at(g, L, I);
-    auto xv = [&](int64_t J) { return e[J]; };
-    const T rn = r[I] - w * mult_at(g, L, I, Dg, xv);
-    r[I] = rn;
-    x[I] = x[I] + w * e[I];
-    acc = (double)rn * (double)rn;
-  }
-  if (L2) {
-    const double t = block_sum(acc);
-    if (threadIdx.x == 0 && threadIdx.y == 0) partial[block_linear_id()] = t;
-  }
-}
-
-// Jacobi!'s increment! (ω=1) on the fine level fused with restrict! and fill!(coarse.x,0)
-// (src/MultiLevelPoisson.jl:92-94,13-19,49): one thread per coarse inside cell walks its
-// children in x-fastest order, updates them, and sums their new residuals.  ϵ of the coarse
-// level is initialised (r·iD) for its own Jacobi! / GS-RB.
-template <class T>
-__global__ void __launch_bounds__(256) k_inc_restrict(Geo gf, Geo gc, Box bc, int cm, const T* __restrict__ Lf,
-                                                      T* __restrict__ rf, T* __restrict__ xf,
-                                                      const T* __restrict__ ef, const T* __restrict__ Lc,
-                                                      T* __restrict__ rc, T* __restrict__ xc, T* __restrict__ ec) {
-  int c[3];
-  if (!box_cell(bc, c[0], c[1], c[2])) return;
-  int lo[3], hi[3];
-  for (int d = 0; d < 3; ++d) {
-    if (d < gc.D && ((cm >> d) & 1)) { lo[d] = 2 * c[d] - 1; hi[d] = 2 * c[d]; }  // 0-based up(I,c)
-    else { lo[d] = hi[d] = c[d]; }
-  }
-  const T w = T(1);
-  auto xv = [&](int64_t J) { return ef[J]; };
-  T s = 0;
-  for (int k = lo[2]; k <= hi[2]; ++k)
-    for (int j = lo[1]; j <= hi[1]; ++j)
-      for (int i = lo[0]; i <= hi[0]; ++i) {
-        const int64_t J = at(gf, i, j, k);
-        const T Dg = diag_at(gf, Lf, J);
-        const T rn = rf[J] - w * mult_at(gf, Lf, J, Dg, xv);
-        rf[J] = rn;
-        xf[J] = xf[J] + w * ef[J];
-        s += rn;
-      }
-  const int64_t Ic = at(gc, c[0], c[1], c[2]);
-  rc[Ic] = s;
-  xc[Ic] = 0;
-  ec[Ic] = s * inv_diag(diag_at(gc, Lc, Ic));
-}
-
-// GS-RB half sweep (src/Poisson.jl:124-128,144-146) over half_rangek: the last index is doubled
-// and shifted so colour `it` updates cells whose 1-based index sum is even for odd it.
-template <class T>
-__global__ void __launch_bounds__(256) k_gs_half(Geo g, Box b, const T* __restrict__ L, const T* __restrict__ r,
-                                                 T* __restrict__ e, int it) {
-  int c[3];
-  if (!box_cell(b, c[0], c[1], c[2])) return;
+__device__ __forceinline__ void gs_cell(const Geo& g, const T* __restrict__ L, const T* __restrict__ r,
+                                        T* __restrict__ e, int it, int c0, int c1, int c2) {
+  int q[3] = {c0, c1, c2};
   const int Ld = g.D - 1;
   int front = 0;
-  for (int d = 0; d < Ld; ++d) front += c[d] + 1;          // 1-based front indices
-  const int kv = c[Ld] + 1;                                 // 1-based half index
-  c[Ld] = 2 * kv - 1 - (front + it) % 2 - 1;                // back to 0-based
-  const int64_t I = at(g, c[0], c[1], c[2]);
+  for (int d = 0; d < Ld; ++d) front += q[d] + 1;  // 1-based front indices
+  const int kv = q[Ld] + 1;                        // 1-based half index
+  q[Ld] = 2 * kv - 1 - (front + it) % 2 - 1;       // back to 0-based
+  const int64_t I = at(g, q[0], q[1], q[2]);
   const T Dg = diag_at(g, L, I);
   T s = r[I];
   for (int i = 0; i < g.D; ++i) {
@@ -353,86 +296,443 @@ __global__ void __launch_bounds__(256) k_gs_half(Geo g, Box b, const T* __restri
   }
   e[I] = s * inv_diag(Dg);
 }
-
-// prolongate! + increment! on the fine level (src/MultiLevelPoisson.jl:98-100,50): ϵ_fine is
-// the injection x_coarse[down(I)] (0 in the ghosts, or the periodic image), synthesized on the
-// fly for the 7-point product; then ϵ := r·iD initialises the fine level's GS-RB (smooth!).
+// Jacobi!'s increment (ω=1) of the 2^D children of coarse cell c, in x-fastest order, their new
+// residuals summed into the coarse r (restrict!), coarse x = 0 (fill!) and coarse ϵ = r·iD
 template <class T>
-__global__ void __launch_bounds__(256) k_prolong_inc(Geo gf, Geo gc, Box bf, int cm, int perdir,
-                                                     const T* __restrict__ Lf, T* __restrict__ rf,
-                                                     T* __restrict__ xf, T* __restrict__ ef,
-                                                     const T* __restrict__ xc, T w) {
-  int c[3];
-  if (!box_cell(bf, c[0], c[1], c[2])) return;
-  auto eps = [&](int i, int j, int k) -> T {
-    int q[3] = {i, j, k};
-    for (int d = 0; d < gf.D; ++d) {
-      if (q[d] == 0 || q[d] == gf.N[d] - 1) {
-        if (!((perdir >> d) & 1)) return T(0);
-        q[d] = q[d] == 0 ? gf.N[d] - 2 : 1;
-      }
-    }
-    for (int d = 0; d < gf.D; ++d)
-      if ((cm >> d) & 1) q[d] = (q[d] + 3) / 2 - 1;  // down(I) = (I+2)÷2 in 1-based
-    return xc[at(gc, q[0], q[1], q[2])];
+__device__ __forceinline__ void inc_restrict_cell(const Geo& gf, const Geo& gc, int cm, const T* __restrict__ Lf,
+                                                  T* __restrict__ rf, T* __restrict__ xf, const T* __restrict__ ef,
+                                                  const T* __restrict__ Lc, T* __restrict__ rc, T* __restrict__ xc,
+                                                  T* __restrict__ ec, const int* c) {
+  int lo[3], hi[3];
+  for (int d = 0; d < 3; ++d) {
+    if (d < gc.D && ((cm >> d) & 1)) { lo[d] = 2 * c[d] - 1; hi[d] = 2 * c[d]; }  // 0-based up(I,c)
+    else { lo[d] = hi[d] = c[d]; }
+  }
+  T s = 0;
+  for (int k = lo[2]; k <= hi[2]; ++k)
+    for (int j = lo[1]; j <= hi[1]; ++j)
+      for (int i = lo[0]; i <= hi[0]; ++i) s += increment_cell(gf, Lf, rf, xf, ef, T(1), at(gf, i, j, k));
+  const int64_t Ic = at(gc, c[0], c[1], c[2]);
+  rc[Ic] = s;
+  xc[Ic] = 0;
+  ec[Ic] = s * inv_diag(diag_at(gc, Lc, Ic));
+}
+// prolongate! + increment! at fine cell c, then ϵ := r·iD (src/MultiLevelPoisson.jl:98-100,50).
+// The coarse index of a fine index is (c+1)>>1 along a coarsened direction (down(I)=(I+2)÷2 in
+// 1-based indices); a ghost index maps to its periodic image, or to "no value" (ϵ = 0).
+__device__ __forceinline__ int pro_map(int c, int N, bool per, bool coarse) {
+  if (c == 0 || c == N - 1) {
+    if (!per) return -1;
+    c = c == 0 ? N - 2 : 1;
+  }
+  return coarse ? (c + 1) >> 1 : c;
+}
+template <class T>
+__device__ __forceinline__ void prolong_cell(const Geo& gf, const Geo& gc, int cm, int perdir, const T* __restrict__ Lf,
+                                             T* __restrict__ rf, T* __restrict__ xf, T* __restrict__ ef,
+                                             const T* __restrict__ xc, T w, const int* c) {
+  int q[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};  // q[d][o]: coarse index of c_d + o - 1
+  for (int d = 0; d < gf.D; ++d)
+    for (int o = 0; o < 3; ++o) q[d][o] = pro_map(c[d] + o - 1, gf.N[d], (perdir >> d) & 1, (cm >> d) & 1);
+  auto xcv = [&](int a, int b, int z) -> T {
+    return (a < 0 || b < 0 || z < 0) ? T(0) : xc[(int64_t)a + gc.s[1] * b + gc.s[2] * z];
   };
   const int64_t I = at(gf, c[0], c[1], c[2]);
   const T Dg = diag_at(gf, Lf, I);
-  const T e0 = eps(c[0], c[1], c[2]);
+  const T e0 = xcv(q[0][1], q[1][1], q[2][1]);
   T s = e0 * Dg;
-  for (int i = 0; i < gf.D; ++i) {
-    int lo[3] = {c[0], c[1], c[2]}, up[3] = {c[0], c[1], c[2]};
-    lo[i] -= 1;
-    up[i] += 1;
-    const int64_t si = gf.s[i];
-    s += (eps(lo[0], lo[1], lo[2]) * Lf[I + i * gf.n] + eps(up[0], up[1], up[2]) * Lf[I + si + i * gf.n]);
-  }
+  s += (xcv(q[0][0], q[1][1], q[2][1]) * Lf[I] + xcv(q[0][2], q[1][1], q[2][1]) * Lf[I + 1]);
+  s += (xcv(q[0][1], q[1][0], q[2][1]) * Lf[I + gf.n] + xcv(q[0][1], q[1][2], q[2][1]) * Lf[I + gf.s[1] + gf.n]);
+  if (gf.D == 3)
+    s += (xcv(q[0][1], q[1][1], q[2][0]) * Lf[I + 2 * gf.n] +
+          xcv(q[0][1], q[1][1], q[2][2]) * Lf[I + gf.s[2] + 2 * gf.n]);
   const T rn = rf[I] - w * s;
   rf[I] = rn;
   xf[I] = xf[I] + w * e0;
   ef[I] = rn * inv_diag(Dg);
 }
 
+// ---- residual! (src/Poisson.jl:92-98) fused with mom_project!'s z = div(u) and x .*= dt
+// (src/Flow.jl:225).  xin·scale is written to xout; with FLOW the source z is div(u).
+template <class T, bool FLOW>
+__global__ void __launch_bounds__(256) k_residual(Geo g, ZT t, const T* __restrict__ L, const T* __restrict__ xin,
+                                                  T* __restrict__ xout, T scale, int write_x, T* __restrict__ z,
+                                                  const T* __restrict__ u, T* __restrict__ r,
+                                                  double* __restrict__ partial) {
+  int c0, c1, z0, z1;
+  zt_cell(t, c0, c1, z0, z1);
+  double acc = 0;
+  if (c0 <= t.b.hi[0] && c1 <= t.b.hi[1]) {
+    for (int c2 = z0; c2 <= z1; ++c2) {
+      const int64_t I = at(g, c0, c1, c2);
+      T zv;
+      if (FLOW) {
+        T s = 0;
+        for (int i = 0; i < g.D; ++i) s += (u[I + g.s[i] + i * g.n] - u[I + i * g.n]);
+        zv = s;
+        z[I] = zv;
+      } else {
+        zv = z[I];
+      }
+      const T Dg = diag_at(g, L, I);
+      const T iD = inv_diag(Dg);
+      auto xv = [&](int64_t J) { return xin[J] * scale; };
+      const T rv = iD == T(0) ? T(0) : zv - mult_at(g, L, I, Dg, xv);
+      r[I] = rv;
+      if (write_x) xout[I] = xv(I);
+      acc += (double)rv;
+    }
+  }
+  const double s = block_sum(acc);
+  if (threadIdx.x == 0 && threadIdx.y == 0) partial[blockIdx.x] = s;
+}
+
+// r -= s (when |s| > 2eps) ; Σr² partials ; ϵ = r·iD (Jacobi!/GS init of the first V-cycle)
+template <class T>
+__global__ void __launch_bounds__(256) k_submean_l2(Geo g, ZT t, const T* __restrict__ L, T* __restrict__ r,
+                                                    T* __restrict__ e, const T* __restrict__ sdev,
+                                                    double* __restrict__ partial) {
+  int c0, c1, z0, z1;
+  zt_cell(t, c0, c1, z0, z1);
+  double acc = 0;
+  if (c0 <= t.b.hi[0] && c1 <= t.b.hi[1]) {
+    const T s = sdev[0];
+    const bool sub = sdev[1] != T(0);
+    for (int c2 = z0; c2 <= z1; ++c2) {
+      const int64_t I = at(g, c0, c1, c2);
+      T rv = r[I];
+      if (sub) { rv = rv - s; r[I] = rv; }
+      e[I] = rv * inv_diag(diag_at(g, L, I));
+      acc += (double)rv * (double)rv;
+    }
+  }
+  const double tt = block_sum(acc);
+  if (threadIdx.x == 0 && threadIdx.y == 0) partial[blockIdx.x] = tt;
+}
+
+// ϵ = r·iD on inside (Jacobi!/GS-RB init, src/Poisson.jl:112,142)
+template <class T>
+__global__ void __launch_bounds__(256) k_eps_init(Geo g, ZT t, const T* __restrict__ L, const T* __restrict__ r,
+                                                  T* __restrict__ e) {
+  int c0, c1, z0, z1;
+  zt_cell(t, c0, c1, z0, z1);
+  if (c0 > t.b.hi[0] || c1 > t.b.hi[1]) return;
+  for (int c2 = z0; c2 <= z1; ++c2) {
+    const int64_t I = at(g, c0, c1, c2);
+    e[I] = r[I] * inv_diag(diag_at(g, L, I));
+  }
+}
+
+// increment! (src/Poisson.jl:100-104): r -= ω·Aϵ; x += ω·ϵ.  Optionally the Σr² partials (L₂ of
+// the finest level after a V-cycle, src/MultiLevelPoisson.jl:115).
+template <class T, bool L2>
+__global__ void __launch_bounds__(256) k_increment(Geo g, ZT t, const T* __restrict__ L, T* __restrict__ r,
+                                                   T* __restrict__ x, const T* __restrict__ e, T w,
+                                                   double* __restrict__ partial) {
+  int c0, c1, z0, z1;
+  zt_cell(t, c0, c1, z0, z1);
+  double acc = 0;
+  if (c0 <= t.b.hi[0] && c1 <= t.b.hi[1]) {
+    for (int c2 = z0; c2 <= z1; ++c2) {
+      const T rn = increment_cell(g, L, r, x, e, w, at(g, c0, c1, c2));
+      acc += (double)rn * (double)rn;
+    }
+  }
+  if (L2) {
+    const double tt = block_sum(acc);
+    if (threadIdx.x == 0 && threadIdx.y == 0) partial[blockIdx.x] = tt;
+  }
+}
+
+// Jacobi!'s increment! (ω=1) on the fine level fused with restrict! and fill!(coarse.x,0)
+// (src/MultiLevelPoisson.jl:92-94,13-19,49): one thread per coarse inside cell (any coarsening mask).
+template <class T>
+__global__ void __launch_bounds__(256) k_inc_restrict(Geo gf, Geo gc, ZT t, int cm, const T* __restrict__ Lf,
+                                                      T* __restrict__ rf, T* __restrict__ xf,
+                                                      const T* __restrict__ ef, const T* __restrict__ Lc,
+                                                      T* __restrict__ rc, T* __restrict__ xc, T* __restrict__ ec) {
+  int c[3], z0, z1;
+  zt_cell(t, c[0], c[1], z0, z1);
+  if (c[0] > t.b.hi[0] || c[1] > t.b.hi[1]) return;
+  for (c[2] = z0; c[2] <= z1; ++c[2]) inc_restrict_cell(gf, gc, cm, Lf, rf, xf, ef, Lc, rc, xc, ec, c);
+}
+
+// The same fused Jacobi! increment + restrict! + fill! when every direction is coarsened: one
+// thread per fine (x,y) column (unit-stride loads), walking the fine plane pairs of its coarse
+// planes; each coarse cell's owner (even lane, even row) then sums its 2^D children from LDS in
+// the reference's x-fastest order (src/MultiLevelPoisson.jl:13-19).  The tile's x and y ranges
+// are fine-cell ranges, its z range is in coarse planes.
+template <class T, int D>
+__global__ void __launch_bounds__(256) k_inc_restrict_full(Geo gf, Geo gc, ZT t, const T* __restrict__ Lf,
+                                                           T* __restrict__ rf, T* __restrict__ xf,
+                                                           const T* __restrict__ ef, const T* __restrict__ Lc,
+                                                           T* __restrict__ rc, T* __restrict__ xc,
+                                                           T* __restrict__ ec) {
+  __shared__ T sR[2][BY][64];
+  int cx, cy, z0, z1;
+  zt_cell(t, cx, cy, z0, z1);
+  const int tx = threadIdx.x, ty = threadIdx.y;
+  const bool in = cx <= t.b.hi[0] && cy <= t.b.hi[1];
+  auto xv = [&](int64_t J) { return ef[J]; };
+  constexpr int NZ = D == 3 ? 2 : 1;
+  for (int kc = z0; kc <= z1; ++kc) {
+    for (int q = 0; q < NZ; ++q) {
+      const int cz = D == 3 ? 2 * kc - 1 + q : 0;
+      T rn = 0;
+      if (in) {
+        const int64_t J = at(gf, cx, cy, cz);
+        const T Dg = diag_at(gf, Lf, J);
+        rn = rf[J] - T(1) * mult_at(gf, Lf, J, Dg, xv);
+        rf[J] = rn;
+        xf[J] = xf[J] + T(1) * ef[J];
+      }
+      sR[q][ty][tx] = rn;
+    }
+    __syncthreads();
+    if (in && !(tx & 1) && !(ty & 1)) {
+      T s = 0;
+      for (int q = 0; q < NZ; ++q)
+        for (int yy = 0; yy < 2; ++yy)
+          for (int xx = 0; xx < 2; ++xx) s += sR[q][ty + yy][tx + xx];
+      const int64_t Ic = at(gc, (cx + 1) >> 1, (cy + 1) >> 1, D == 3 ? kc : 0);
+      rc[Ic] = s;
+      xc[Ic] = 0;
+      ec[Ic] = s * inv_diag(diag_at(gc, Lc, Ic));
+    }
+    __syncthreads();
+  }
+}
+
+// GS-RB half sweep (src/Poisson.jl:124-128,144-146) over half_rangek
+template <class T>
+__global__ void __launch_bounds__(256) k_gs_half(Geo g, ZT t, const T* __restrict__ L, const T* __restrict__ r,
+                                                 T* __restrict__ e, int it) {
+  int c0, c1, z0, z1;
+  zt_cell(t, c0, c1, z0, z1);
+  if (c0 > t.b.hi[0] || c1 > t.b.hi[1]) return;
+  for (int kz = z0; kz <= z1; ++kz) gs_cell(g, L, r, e, it, c0, c1, kz);
+}
+
+// prolongate! + increment! on the fine level, then ϵ := r·iD (the fine level's GS-RB init)
+template <class T>
+__global__ void __launch_bounds__(256) k_prolong_inc(Geo gf, Geo gc, ZT t, int cm, int perdir,
+                                                     const T* __restrict__ Lf, T* __restrict__ rf,
+                                                     T* __restrict__ xf, T* __restrict__ ef,
+                                                     const T* __restrict__ xc, T w) {
+  int c[3], z0, z1;
+  zt_cell(t, c[0], c[1], z0, z1);
+  if (c[0] > t.b.hi[0] || c[1] > t.b.hi[1]) return;
+  for (c[2] = z0; c[2] <= z1; ++c[2]) prolong_cell(gf, gc, cm, perdir, Lf, rf, xf, ef, xc, w, c);
+}
+
+// ---- the coarse end of the V-cycle in one workgroup ----
+// Levels below a few thousand cells are latency-bound: their passes are run back to back by one
+// 1024-thread workgroup with a barrier between passes, from the restriction out of the first tail
+// level down to the coarsest and back up through that level's smooth! (src/MultiLevelPoisson.jl:
+// 88-101, 106; GaussSeidelRB! src/Poisson.jl:141-148), with perBC! where the reference has it.
+constexpr int TAIL_MAX_LEVELS = 11;
+constexpr int TAIL_LDS_BYTES = 65536;
+template <class T>
+struct TailLevel {
+  Geo g;
+  T* L;
+  T* x;
+  T* r;
+  T* e;
+  int cmask;
+  int64_t off;  // element offset of this level's arrays in the LDS image (L, then x, r, e)
+};
+template <class T>
+struct TailDesc {
+  int n;
+  int perdir;
+  TailLevel<T> lv[TAIL_MAX_LEVELS];
+};
+__device__ __forceinline__ void box_decode(const Box& b, int idx, int* c) {
+  const int nx = b.hi[0] - b.lo[0] + 1, ny = b.hi[1] - b.lo[1] + 1;
+  c[0] = b.lo[0] + idx % nx;
+  const int r1 = idx / nx;
+  c[1] = b.lo[1] + r1 % ny;
+  c[2] = b.lo[2] + r1 / ny;
+}
+__device__ __forceinline__ int box_count(const Box& b) {
+  return (b.hi[0] - b.lo[0] + 1) * (b.hi[1] - b.lo[1] + 1) * (b.hi[2] - b.lo[2] + 1);
+}
+__device__ __forceinline__ Box dbox_inside(const Geo& g) {
+  Box b;
+  for (int d = 0; d < 3; ++d) {
+    if (d < g.D) { b.lo[d] = 1; b.hi[d] = g.N[d] - 2; } else { b.lo[d] = b.hi[d] = 0; }
+  }
+  return b;
+}
+// perBC! of a scalar field over every cell of a small level (periodic ghosts -> images)
+template <class T>
+__device__ void tail_perbc(const Geo& g, T* a, int perdir) {
+  if (!perdir) return;
+  const int cnt = (int)g.n;
+  for (int idx = threadIdx.x; idx < cnt; idx += blockDim.x) {
+    int c[3];
+    Box b;
+    for (int d = 0; d < 3; ++d) { b.lo[d] = 0; b.hi[d] = g.N[d] - 1; }
+    box_decode(b, idx, c);
+    int m[3] = {c[0], c[1], c[2]};
+    bool ghost = false;
+    for (int q = 0; q < g.D; ++q) {
+      if (!((perdir >> q) & 1)) continue;
+      if (c[q] == 0) { m[q] = g.N[q] - 2; ghost = true; }
+      else if (c[q] == g.N[q] - 1) { m[q] = 1; ghost = true; }
+    }
+    if (ghost) a[at(g, c[0], c[1], c[2])] = a[at(g, m[0], m[1], m[2])];
+  }
+  __syncthreads();
+}
+template <class T>
+struct TLev {
+  Geo g;
+  T *L, *x, *r, *e;
+  int cm;
+};
+// LDS: every tail level's L, x, r, ϵ are copied into LDS first and x, r, ϵ copied back at the
+// end, so each pass runs at LDS latency; otherwise the passes work on the global arrays.
+template <class T, bool LDS>
+__global__ void __launch_bounds__(1024) k_mg_tail(TailDesc<T> d, T w) {
+  extern __shared__ __align__(16) unsigned char tail_smem[];
+  T* sb = reinterpret_cast<T*>(tail_smem);
+  auto lev = [&](int l) -> TLev<T> {
+    TLev<T> q;
+    q.g = d.lv[l].g;
+    q.cm = d.lv[l].cmask;
+    if (LDS) {
+      q.L = sb + d.lv[l].off;
+      q.x = q.L + q.g.D * q.g.n;
+      q.r = q.x + q.g.n;
+      q.e = q.r + q.g.n;
+    } else {
+      q.L = d.lv[l].L; q.x = d.lv[l].x; q.r = d.lv[l].r; q.e = d.lv[l].e;
+    }
+    return q;
+  };
+  if (LDS) {
+    for (int l = 0; l < d.n; ++l) {
+      const TLev<T> q = lev(l);
+      const int n = (int)q.g.n, nL = q.g.D * n;
+      for (int k = threadIdx.x; k < nL; k += blockDim.x) q.L[k] = d.lv[l].L[k];
+      for (int k = threadIdx.x; k < n; k += blockDim.x) {
+        q.x[k] = d.lv[l].x[k];
+        q.r[k] = d.lv[l].r[k];
+        q.e[k] = d.lv[l].e[k];
+      }
+    }
+    __syncthreads();
+  }
+  for (int l = 0; l + 1 < d.n; ++l) {  // down-leg: Jacobi! + restrict! + fill!
+    const TLev<T> f = lev(l), c = lev(l + 1);
+    tail_perbc(f.g, f.e, d.perdir);
+    const Box bc = dbox_inside(c.g);
+    const int cnt = box_count(bc);
+    for (int idx = threadIdx.x; idx < cnt; idx += blockDim.x) {
+      int q[3];
+      box_decode(bc, idx, q);
+      inc_restrict_cell(f.g, c.g, f.cm, (const T*)f.L, f.r, f.x, (const T*)f.e, (const T*)c.L, c.r, c.x, c.e, q);
+    }
+    __syncthreads();
+  }
+  for (int l = d.n - 1; l >= 0; --l) {  // up-leg: smooth!(l), then prolongate! + increment!(l-1)
+    const TLev<T> p = lev(l);
+    tail_perbc(p.g, p.e, d.perdir);
+    Box bh = dbox_inside(p.g);
+    bh.hi[p.g.D - 1] = p.g.N[p.g.D - 1] / 2 - 1;
+    const int nh = box_count(bh);
+    for (int it = 1; it <= 4; ++it) {
+      for (int idx = threadIdx.x; idx < nh; idx += blockDim.x) {
+        int q[3];
+        box_decode(bh, idx, q);
+        gs_cell(p.g, (const T*)p.L, (const T*)p.r, p.e, it, q[0], q[1], q[2]);
+      }
+      __syncthreads();
+    }
+    tail_perbc(p.g, p.e, d.perdir);
+    const Box bi = dbox_inside(p.g);
+    const int ni = box_count(bi);
+    for (int idx = threadIdx.x; idx < ni; idx += blockDim.x) {
+      int q[3];
+      box_decode(bi, idx, q);
+      increment_cell(p.g, (const T*)p.L, p.r, p.x, (const T*)p.e, w, at(p.g, q[0], q[1], q[2]));
+    }
+    __syncthreads();
+    if (l > 0) {
+      const TLev<T> f = lev(l - 1);
+      const Box bf = dbox_inside(f.g);
+      const int nf = box_count(bf);
+      for (int idx = threadIdx.x; idx < nf; idx += blockDim.x) {
+        int q[3];
+        box_decode(bf, idx, q);
+        prolong_cell(f.g, p.g, f.cm, d.perdir, (const T*)f.L, f.r, f.x, f.e, (const T*)p.x, w, q);
+      }
+      __syncthreads();
+    }
+  }
+  if (LDS) {
+    for (int l = 0; l < d.n; ++l) {
+      const TLev<T> q = lev(l);
+      const int n = (int)q.g.n;
+      for (int k = threadIdx.x; k < n; k += blockDim.x) {
+        d.lv[l].x[k] = q.x[k];
+        d.lv[l].r[k] = q.r[k];
+        d.lv[l].e[k] = q.e[k];
+      }
+    }
+  }
+}
+
 // mom_project! correction (src/Flow.jl:227-230): u[I,i] -= L[I,i]·(x[I]-x[I-e_i]) on inside,
 // p = x/dt on every cell.
 template <class T>
-__global__ void __launch_bounds__(256) k_correct(Geo g, Box b, T* __restrict__ u, const T* __restrict__ L,
+__global__ void __launch_bounds__(256) k_correct(Geo g, ZT t, T* __restrict__ u, const T* __restrict__ L,
                                                  const T* __restrict__ x, T* __restrict__ p, T dt) {
-  int c[3];
-  if (!box_cell(b, c[0], c[1], c[2])) return;
-  const int64_t I = at(g, c[0], c[1], c[2]);
-  bool in = true;
-  for (int d = 0; d < g.D; ++d) in &= c[d] >= 1 && c[d] <= g.N[d] - 2;
-  const T xI = x[I];
-  if (in)
-    for (int i = 0; i < g.D; ++i) u[I + i * g.n] -= L[I + i * g.n] * (xI - x[I - g.s[i]]);
-  p[I] = xI / dt;
+  int c[3], z0, z1;
+  zt_cell(t, c[0], c[1], z0, z1);
+  if (c[0] > t.b.hi[0] || c[1] > t.b.hi[1]) return;
+  for (c[2] = z0; c[2] <= z1; ++c[2]) {
+    const int64_t I = at(g, c[0], c[1], c[2]);
+    bool in = true;
+    for (int d = 0; d < g.D; ++d) in &= c[d] >= 1 && c[d] <= g.N[d] - 2;
+    const T xI = x[I];
+    if (in)
+      for (int i = 0; i < g.D; ++i) u[I + i * g.n] -= L[I + i * g.n] * (xI - x[I - g.s[i]]);
+    p[I] = xI / dt;
+  }
 }
 
 // CFL (src/Flow.jl:234-244): σ = flux_out(u) on inside; block maxima of σ over every cell.
 template <class T>
-__global__ void __launch_bounds__(256) k_cfl(Geo g, Box b, const T* __restrict__ u, T* __restrict__ sigma,
+__global__ void __launch_bounds__(256) k_cfl(Geo g, ZT t, const T* __restrict__ u, T* __restrict__ sigma,
                                              T* __restrict__ partial) {
-  int c[3];
+  int c[3], z0, z1;
+  zt_cell(t, c[0], c[1], z0, z1);
   T m = -T(INFINITY);
-  if (box_cell(b, c[0], c[1], c[2])) {
-    const int64_t I = at(g, c[0], c[1], c[2]);
-    bool in = true;
-    for (int d = 0; d < g.D; ++d) in &= c[d] >= 1 && c[d] <= g.N[d] - 2;
-    if (in) {
-      T s = 0;
-      for (int i = 0; i < g.D; ++i) {
-        const T a = u[I + g.s[i] + i * g.n], bq = -u[I + i * g.n];
-        s += ((a > T(0) ? a : T(0)) + (bq > T(0) ? bq : T(0)));
+  if (c[0] <= t.b.hi[0] && c[1] <= t.b.hi[1]) {
+    for (c[2] = z0; c[2] <= z1; ++c[2]) {
+      const int64_t I = at(g, c[0], c[1], c[2]);
+      bool in = true;
+      for (int d = 0; d < g.D; ++d) in &= c[d] >= 1 && c[d] <= g.N[d] - 2;
+      T v;
+      if (in) {
+        T s = 0;
+        for (int i = 0; i < g.D; ++i) {
+          const T a = u[I + g.s[i] + i * g.n], bq = -u[I + i * g.n];
+          s += ((a > T(0) ? a : T(0)) + (bq > T(0) ? bq : T(0)));
+        }
+        sigma[I] = s;
+        v = s;
+      } else {
+        v = sigma[I];
       }
-      sigma[I] = s;
-      m = s;
-    } else {
-      m = sigma[I];
+      m = v > m ? v : m;
     }
   }
   m = block_max<T>(m, -T(INFINITY));
-  if (threadIdx.x == 0 && threadIdx.y == 0) partial[block_linear_id()] = m;
+  if (threadIdx.x == 0 && threadIdx.y == 0) partial[blockIdx.x] = m;
 }
 
 // =============================================================================================
@@ -625,6 +925,21 @@ __global__ void __launch_bounds__(256) k_pressure_force(Geo g, Box b, DBodies bs
     const double s = block_sum(acc[i]);
     if (threadIdx.x == 0 && threadIdx.y == 0) partial[block_linear_id() + i * pstride] = s;
   }
+}
+
+// Device self-test: counts floats x (all 2^32 bit patterns, finite ones) for which div6(x)
+// differs in any bit from the IEEE quotient x/6.
+__global__ void __launch_bounds__(256) k_selftest_div6(unsigned long long* bad) {
+  unsigned long long cnt = 0;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t b = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; b < (1ull << 32); b += stride) {
+    const float x = __uint_as_float((unsigned)b);
+    if (!isfinite(x)) continue;
+    const float want = x / 6.0f;
+    const float got = div6(x);
+    if (__float_as_uint(want) != __float_as_uint(got)) ++cnt;
+  }
+  if (cnt) atomicAdd(bad, cnt);
 }
 
 }  // namespace wl

@@ -11,6 +11,7 @@
 #include <vector>
 #include <memory>
 #include <algorithm>
+#include <cstdlib>
 #include "wl_kernels.h"
 
 using namespace wl;
@@ -77,6 +78,35 @@ static int64_t nblocks(const Box& b) {
   return (int64_t)gd.x * gd.y * gd.z;
 }
 static const dim3 kBlock(BX, BY, 1);
+
+// Blocks-in-flight target of the z-marching tiles (WL_ZT_TARGET overrides it, for tuning).
+static int zt_target_default() {
+  static int v = [] {
+    const char* e = std::getenv("WL_ZT_TARGET");
+    return e ? std::max(1, std::atoi(e)) : 4096;
+  }();
+  return v;
+}
+// z-marching tiles over a box: enough planes per tile to keep ~target blocks in flight
+static ZT make_zt(const Box& b, int target = 0) {
+  if (target <= 0) target = zt_target_default();
+  ZT t;
+  t.b = b;
+  t.sx = BX;
+  t.sy = BY;
+  const int nx = b.hi[0] - b.lo[0] + 1, ny = b.hi[1] - b.lo[1] + 1, nz = b.hi[2] - b.lo[2] + 1;
+  t.nbx = (nx + BX - 1) / BX;
+  t.nby = (ny + BY - 1) / BY;
+  const int64_t per = (int64_t)t.nbx * t.nby;
+  int64_t zc = (int64_t)nz * per / target;
+  t.zc = (int)std::min<int64_t>(32, std::max<int64_t>(1, zc));
+  return t;
+}
+static int64_t zt_blocks(const ZT& t) {
+  const int nz = t.b.hi[2] - t.b.lo[2] + 1;
+  return (int64_t)t.nbx * t.nby * ((nz + t.zc - 1) / t.zc);
+}
+static dim3 zt_grid(const ZT& t) { return dim3((unsigned)zt_blocks(t)); }
 
 template <class T>
 static T* dalloc(int64_t n) {
@@ -285,9 +315,9 @@ struct MG {
   void residual_generic() {
     Level<T>& p = lv[0];
     perbc(p.g, p.x);
-    Box b = box_inside(p.g);
-    const int64_t nb = nblocks(b);
-    hipLaunchKernelGGL((k_residual<T, false>), grid_of(b), kBlock, 0, st, p.g, b, p.L, p.x, p.x, T(1), 0, p.z,
+    ZT b = make_zt(box_inside(p.g));
+    const int64_t nb = zt_blocks(b);
+    hipLaunchKernelGGL((k_residual<T, false>), zt_grid(b), kBlock, 0, st, p.g, b, p.L, p.x, p.x, T(1), 0, p.z,
                        (const T*)nullptr, p.r, B->partial);
     KCK();
     finish_sum(nb, 0, 1, (double)ninside(p.g));
@@ -295,10 +325,10 @@ struct MG {
   }
   void submean_l2() {
     Level<T>& p = lv[0];
-    Box b = box_inside(p.g);
-    const int64_t nb = nblocks(b);
+    ZT b = make_zt(box_inside(p.g));
+    const int64_t nb = zt_blocks(b);
     ProfScope ps(P, st, "submean_l2", 0, (double)sizeof(T) * (p.g.D + 3) * ncells_inside(p.g));
-    hipLaunchKernelGGL(k_submean_l2<T>, grid_of(b), kBlock, 0, st, p.g, b, p.L, p.r, p.e, (const T*)B->tsc,
+    hipLaunchKernelGGL(k_submean_l2<T>, zt_grid(b), kBlock, 0, st, p.g, b, p.L, p.r, p.e, (const T*)B->tsc,
                        B->partial);
     KCK();
     finish_sum(nb, 1);
@@ -311,58 +341,128 @@ struct MG {
 
   void eps_init(int l) {
     Level<T>& p = lv[l];
-    Box b = box_inside(p.g);
+    ZT b = make_zt(box_inside(p.g));
     ProfScope ps(P, st, "eps_init", l, (double)sizeof(T) * (p.g.D + 2) * ncells_inside(p.g));
-    hipLaunchKernelGGL(k_eps_init<T>, grid_of(b), kBlock, 0, st, p.g, b, p.L, p.r, p.e);
+    hipLaunchKernelGGL(k_eps_init<T>, zt_grid(b), kBlock, 0, st, p.g, b, p.L, p.r, p.e);
     KCK();
   }
   void gs_sweeps(int l) {
     Level<T>& p = lv[l];
     perbc(p.g, p.e);
-    Box b = box_half(p.g);
-    if (box_empty(b)) return;
+    Box bh = box_half(p.g);
+    if (box_empty(bh)) return;
+    ZT b = make_zt(bh);
     for (int it = 1; it <= 4; ++it) {
       ProfScope ps(P, st, "gs_half", l, (double)sizeof(T) * (p.g.D + 1.5) * ncells_inside(p.g));
-      hipLaunchKernelGGL(k_gs_half<T>, grid_of(b), kBlock, 0, st, p.g, b, p.L, p.r, p.e, it);
+      hipLaunchKernelGGL(k_gs_half<T>, zt_grid(b), kBlock, 0, st, p.g, b, p.L, p.r, p.e, it);
       KCK();
     }
   }
   void increment(int l, T w, bool l2) {
     Level<T>& p = lv[l];
     perbc(p.g, p.e);
-    Box b = box_inside(p.g);
+    ZT b = make_zt(box_inside(p.g));
     ProfScope ps(P, st, "increment", l, (double)sizeof(T) * (p.g.D + 5) * ncells_inside(p.g));
     if (l2)
-      hipLaunchKernelGGL((k_increment<T, true>), grid_of(b), kBlock, 0, st, p.g, b, p.L, p.r, p.x, p.e, w, B->partial);
+      hipLaunchKernelGGL((k_increment<T, true>), zt_grid(b), kBlock, 0, st, p.g, b, p.L, p.r, p.x, p.e, w, B->partial);
     else
-      hipLaunchKernelGGL((k_increment<T, false>), grid_of(b), kBlock, 0, st, p.g, b, p.L, p.r, p.x, p.e, w,
+      hipLaunchKernelGGL((k_increment<T, false>), zt_grid(b), kBlock, 0, st, p.g, b, p.L, p.r, p.x, p.e, w,
                          B->partial);
     KCK();
-    if (l2) finish_sum(nblocks(b), 1);
+    if (l2) finish_sum(zt_blocks(b), 1);
+  }
+
+  // LDS bytes the levels l..end need in the coarse tail (L, x, r, ϵ of each)
+  int64_t tail_bytes(int l) const {
+    int64_t b = 0;
+    for (size_t k = l; k < lv.size(); ++k) b += (int64_t)(lv[k].g.D + 3) * lv[k].g.n * (int64_t)sizeof(T);
+    return b;
+  }
+  // First level run by the single-workgroup coarse tail: the first level (>= 1) from which every
+  // coarser level fits the tail's LDS image; WL_TAIL_CELLS (cells of the first tail level) makes
+  // it a global-memory tail from that size instead.  -1: no tail.
+  int tail_start(bool* in_lds) const {
+    static const int64_t cells = [] {
+      const char* e = std::getenv("WL_TAIL_CELLS");
+      return e ? (int64_t)std::atoll(e) : (int64_t)-1;
+    }();
+    *in_lds = false;
+    if (cells == 0) return -1;
+    if (cells > 0) {
+      for (size_t l = 1; l < lv.size(); ++l)
+        if (lv[l].g.n <= cells) return (int)l;
+      return -1;
+    }
+    for (size_t l = 1; l < lv.size(); ++l)
+      if (tail_bytes((int)l) <= TAIL_LDS_BYTES) { *in_lds = true; return (int)l; }
+    return -1;
+  }
+  void launch_tail(int Lt, bool in_lds, T w) {
+    TailDesc<T> d;
+    d.n = (int)lv.size() - Lt;
+    d.perdir = perdir;
+    int64_t off = 0;
+    for (int k = 0; k < d.n; ++k) {
+      const Level<T>& q = lv[Lt + k];
+      d.lv[k].g = q.g;
+      d.lv[k].L = q.L;
+      d.lv[k].x = q.x;
+      d.lv[k].r = q.r;
+      d.lv[k].e = q.e;
+      d.lv[k].cmask = q.cmask;
+      d.lv[k].off = off;
+      off += (int64_t)(q.g.D + 3) * q.g.n;
+    }
+    ProfScope ps(P, st, "mg_tail", Lt, 0.0);
+    if (in_lds)
+      hipLaunchKernelGGL((k_mg_tail<T, true>), dim3(1), dim3(1024), (size_t)(off * sizeof(T)), st, d, w);
+    else
+      hipLaunchKernelGGL((k_mg_tail<T, false>), dim3(1), dim3(1024), 0, st, d, w);
+    KCK();
   }
 
   // Vcycle! (src/MultiLevelPoisson.jl:88-101) from the state "ϵ(0) = r·iD"
   void vcycle(T w) {
     const int Lm = (int)lv.size() - 1;
-    for (int l = 0; l < Lm; ++l) {  // down-leg: Jacobi! + restrict! + fill!
+    bool in_lds = false;
+    const int Lt = tail_start(&in_lds);
+    const int down_end = Lt > 0 ? Lt : Lm;
+    for (int l = 0; l < down_end; ++l) {  // down-leg: Jacobi! + restrict! + fill!
       Level<T>& f = lv[l];
       Level<T>& c = lv[l + 1];
       perbc(f.g, f.e);
-      Box bc = box_inside(c.g);
       ProfScope ps(P, st, "inc_restrict", l,
                    (double)sizeof(T) * ((f.g.D + 5) * ncells_inside(f.g) + 3 * ncells_inside(c.g)));
-      hipLaunchKernelGGL(k_inc_restrict<T>, grid_of(bc), kBlock, 0, st, f.g, c.g, bc, f.cmask, f.L, f.r, f.x, f.e,
-                         c.L, c.r, c.x, c.e);
+      const int full = (1 << f.g.D) - 1;
+      if (f.cmask == full) {
+        // fine x,y tile ranges, coarse z planes (the fine inside extents are even here)
+        Box b = box_inside(f.g);
+        if (f.g.D == 3) { b.lo[2] = 1; b.hi[2] = c.g.N[2] - 2; }
+        ZT bc = make_zt(b);
+        if (f.g.D == 3)
+          hipLaunchKernelGGL((k_inc_restrict_full<T, 3>), zt_grid(bc), kBlock, 0, st, f.g, c.g, bc, f.L, f.r, f.x,
+                             f.e, c.L, c.r, c.x, c.e);
+        else
+          hipLaunchKernelGGL((k_inc_restrict_full<T, 2>), zt_grid(bc), kBlock, 0, st, f.g, c.g, bc, f.L, f.r, f.x,
+                             f.e, c.L, c.r, c.x, c.e);
+      } else {
+        ZT bc = make_zt(box_inside(c.g));
+        hipLaunchKernelGGL(k_inc_restrict<T>, zt_grid(bc), kBlock, 0, st, f.g, c.g, bc, f.cmask, f.L, f.r, f.x, f.e,
+                           c.L, c.r, c.x, c.e);
+      }
       KCK();
     }
-    for (int c = Lm; c >= 1; --c) {  // up-leg: smooth!(coarse) then prolongate! + increment!(fine)
-      gs_sweeps(c);
-      increment(c, w, false);
+    if (Lt > 0) launch_tail(Lt, in_lds, w);  // levels Lt..Lm, through smooth!(Lt)
+    for (int c = Lt > 0 ? Lt : Lm; c >= 1; --c) {  // up-leg: smooth!(coarse), prolongate! + increment!(fine)
+      if (!(Lt > 0 && c == Lt)) {
+        gs_sweeps(c);
+        increment(c, w, false);
+      }
       Level<T>& f = lv[c - 1];
-      Box bf = box_inside(f.g);
+      ZT bf = make_zt(box_inside(f.g));
       ProfScope ps(P, st, "prolong_inc", c - 1,
                    (double)sizeof(T) * ((f.g.D + 5) * ncells_inside(f.g) + ncells_inside(lv[c].g)));
-      hipLaunchKernelGGL(k_prolong_inc<T>, grid_of(bf), kBlock, 0, st, f.g, lv[c].g, bf, f.cmask, perdir, f.L, f.r,
+      hipLaunchKernelGGL(k_prolong_inc<T>, zt_grid(bf), kBlock, 0, st, f.g, lv[c].g, bf, f.cmask, perdir, f.L, f.r,
                          f.x, f.e, (const T*)lv[c].x, w);
       KCK();
     }
@@ -586,9 +686,18 @@ struct FlowT : FlowBase {
     sync();
   }
 
-  template <int S>
+  template <int S, int DD>
   void conv_bdim(bool pred, T dt, double t) {
-    Box ba = box_all(g);
+    ZT ba = make_zt(box_all(g));
+    // overlapping tiles: 63 output columns and CBY-1 output rows per block
+    ba.sx = 63;
+    ba.sy = CBY - 1;
+    ba.nbx = (g.N[0] + 62) / 63;
+    ba.nby = (g.N[1] + CBY - 2) / (CBY - 1);
+    {
+      const int64_t per = (int64_t)ba.nbx * ba.nby;
+      ba.zc = (int)std::min<int64_t>(32, std::max<int64_t>(1, (int64_t)g.N[2] * per / (zt_target_default() / 2)));
+    }
     Accel acc;
     acc.on = desc.has_g || desc.has_uBC_rate;
     for (int i = 0; i < 3; ++i) {
@@ -600,31 +709,40 @@ struct FlowT : FlowBase {
       acc.add[i] = (double)a;
     }
     const T nu = (T)desc.nu;
+    const dim3 cblk(64, CBY, 1);
     {
-    ProfScope ps(&prof, st, "conv_bdim1", 0, (double)sizeof(T) * 4 * D * g.n);
-    if (pred)
-      hipLaunchKernelGGL((k_conv_bdim1<T, S, true>), grid_of(ba), kBlock, 0, st, g, ba, (const T*)u, u0, f, sigma,
-                         (const T*)V, nu, dt, desc.perdir_mask, acc);
-    else
-      hipLaunchKernelGGL((k_conv_bdim1<T, S, false>), grid_of(ba), kBlock, 0, st, g, ba, (const T*)u, u0, f, sigma,
-                         (const T*)V, nu, dt, desc.perdir_mask, acc);
-    KCK();
+      ProfScope ps(&prof, st, "conv_bdim1", 0, (double)sizeof(T) * 4 * D * g.n);
+      if (pred)
+        hipLaunchKernelGGL((k_conv_bdim1<T, S, true, DD>), zt_grid(ba), cblk, 0, st, g, ba, (const T*)u, u0, f, sigma,
+                           (const T*)V, nu, dt, desc.perdir_mask, acc);
+      else
+        hipLaunchKernelGGL((k_conv_bdim1<T, S, false, DD>), zt_grid(ba), cblk, 0, st, g, ba, (const T*)u, u0, f,
+                           sigma, (const T*)V, nu, dt, desc.perdir_mask, acc);
+      KCK();
     }
-    Box bi = box_inside(g);
+    ZT bi = make_zt(box_inside(g));
     ProfScope ps(&prof, st, "bdim2", 0, (double)sizeof(T) * (5 * D + D * D) * ncells_inside(g));
     if (pred)
-      hipLaunchKernelGGL((k_bdim2<T, true>), grid_of(bi), kBlock, 0, st, g, bi, u, (const T*)f, (const T*)V,
+      hipLaunchKernelGGL((k_bdim2<T, true>), zt_grid(bi), kBlock, 0, st, g, bi, u, (const T*)f, (const T*)V,
                          (const T*)mu0, (const T*)mu1);
     else
-      hipLaunchKernelGGL((k_bdim2<T, false>), grid_of(bi), kBlock, 0, st, g, bi, u, (const T*)f, (const T*)V,
+      hipLaunchKernelGGL((k_bdim2<T, false>), zt_grid(bi), kBlock, 0, st, g, bi, u, (const T*)f, (const T*)V,
                          (const T*)mu0, (const T*)mu1);
     KCK();
   }
   void momentum(bool pred, T dt, double t, double t1) {
-    switch (desc.scheme) {
-      case WL_QUICK: conv_bdim<WL_QUICK>(pred, dt, t); break;
-      case WL_VANLEER: conv_bdim<WL_VANLEER>(pred, dt, t); break;
-      default: conv_bdim<WL_CDS>(pred, dt, t); break;
+    if (D == 3) {
+      switch (desc.scheme) {
+        case WL_QUICK: conv_bdim<WL_QUICK, 3>(pred, dt, t); break;
+        case WL_VANLEER: conv_bdim<WL_VANLEER, 3>(pred, dt, t); break;
+        default: conv_bdim<WL_CDS, 3>(pred, dt, t); break;
+      }
+    } else {
+      switch (desc.scheme) {
+        case WL_QUICK: conv_bdim<WL_QUICK, 2>(pred, dt, t); break;
+        case WL_VANLEER: conv_bdim<WL_VANLEER, 2>(pred, dt, t); break;
+        default: conv_bdim<WL_CDS, 2>(pred, dt, t); break;
+      }
     }
     double Ub[3];
     Ut(t1, Ub);
@@ -635,11 +753,11 @@ struct FlowT : FlowBase {
   int project(T w, T dt_end, double t1) {
     const T dt = w * dt_end;
     Level<T>& l0 = mg.lv[0];
-    Box bi = box_inside(g);
-    const int64_t nb = nblocks(bi);
+    ZT bi = make_zt(box_inside(g));
+    const int64_t nb = zt_blocks(bi);
     {
     ProfScope ps(&prof, st, "residual", 0, (double)sizeof(T) * (2 * D + 4) * ncells_inside(g));
-    hipLaunchKernelGGL((k_residual<T, true>), grid_of(bi), kBlock, 0, st, g, bi, (const T*)mu0, (const T*)p, xw, dt, 1,
+    hipLaunchKernelGGL((k_residual<T, true>), zt_grid(bi), kBlock, 0, st, g, bi, (const T*)mu0, (const T*)p, xw, dt, 1,
                        sigma, (const T*)u, l0.r, B.partial);
     KCK();
     }
@@ -647,10 +765,10 @@ struct FlowT : FlowBase {
     mg.finish_sum(nb, 0, 1, (double)MG<T>::ninside(g));
     mg.submean_l2();
     const int nit = mg.mg_loop(tol, itmx, &r2hist);
-    Box ba = box_all(g);
+    ZT ba = make_zt(box_all(g));
     {
     ProfScope ps(&prof, st, "correct", 0, (double)sizeof(T) * (2 * g.n + 3 * D * ncells_inside(g)));
-    hipLaunchKernelGGL(k_correct<T>, grid_of(ba), kBlock, 0, st, g, ba, u, (const T*)mu0, (const T*)xw, p, dt);
+    hipLaunchKernelGGL(k_correct<T>, zt_grid(ba), kBlock, 0, st, g, ba, u, (const T*)mu0, (const T*)xw, p, dt);
     KCK();
     }
     double Ub[3];
@@ -668,12 +786,12 @@ struct FlowT : FlowBase {
     momentum(false, dt, t1, t1);
     const int n2 = project(T(0.5), dt, t1);
     // CFL (src/Flow.jl:234-237)
-    Box ba = box_all(g);
-    const int64_t nb = nblocks(ba);
+    ZT ba = make_zt(box_all(g));
+    const int64_t nb = zt_blocks(ba);
     T* tpart = (T*)B.partial;
     {
     ProfScope ps(&prof, st, "cfl", 0, (double)sizeof(T) * (D + 1) * ncells_inside(g));
-    hipLaunchKernelGGL(k_cfl<T>, grid_of(ba), kBlock, 0, st, g, ba, (const T*)u, sigma, tpart);
+    hipLaunchKernelGGL(k_cfl<T>, zt_grid(ba), kBlock, 0, st, g, ba, (const T*)u, sigma, tpart);
     KCK();
     }
     hipLaunchKernelGGL(k_finish_max<T>, dim3(1), dim3(256), 0, st, (const T*)tpart, nb, (T*)B.tsc + 2);
@@ -949,6 +1067,21 @@ int wl_flow_last_residuals(wl_flow_t h, double* r2, int cap, int* count) {
   const auto& v = h->impl->r2hist;
   for (int k = 0; k < (int)v.size() && k < cap; ++k) r2[k] = v[k];
   *count = (int)v.size();
+  API_END
+}
+
+int wl_self_test(int test, int64_t* result) {
+  API_BEGIN
+  if (test != 0) throw WlErr(WL_EINVAL, "unknown self test");
+  unsigned long long* d = nullptr;
+  HIPCK(hipMalloc((void**)&d, sizeof(unsigned long long)));
+  HIPCK(hipMemset(d, 0, sizeof(unsigned long long)));
+  hipLaunchKernelGGL(k_selftest_div6, dim3(4096), dim3(256), 0, 0, d);
+  KCK();
+  unsigned long long h = 0;
+  HIPCK(hipMemcpy(&h, d, sizeof(h), hipMemcpyDeviceToHost));
+  HIPCK(hipFree(d));
+  *result = (int64_t)h;
   API_END
 }
 
