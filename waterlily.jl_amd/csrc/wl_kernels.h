@@ -531,7 +531,7 @@ __global__ void __launch_bounds__(256) k_prolong_inc(Geo gf, Geo gc, ZT t, int c
 // level down to the coarsest and back up through that level's smooth! (src/MultiLevelPoisson.jl:
 // 88-101, 106; GaussSeidelRB! src/Poisson.jl:141-148), with perBC! where the reference has it.
 constexpr int TAIL_MAX_LEVELS = 11;
-constexpr int TAIL_LDS_BYTES = 65536;
+constexpr int TAIL_LDS_BYTES = 160 * 1024 - 8192;
 template <class T>
 struct TailLevel {
   Geo g;
@@ -681,6 +681,182 @@ __global__ void __launch_bounds__(1024) k_mg_tail(TailDesc<T> d, T w) {
         d.lv[l].r[k] = q.r[k];
         d.lv[l].e[k] = q.e[k];
       }
+    }
+  }
+}
+
+// LDS image of a tail level: L (D arrays), x, r, ϵ, D, iD, indexed with 32-bit offsets.
+template <class T>
+struct SLev {
+  int D, N0, N1, N2, s1, s2, n, cm;
+  T *L, *x, *r, *e, *Dg, *iD;
+};
+// Loop over a 0-based inclusive box with the workgroup viewed as bx x by x bz threads (powers of
+// two, no divisions): each thread visits the cells congruent to its own coordinates.
+template <class F>
+__device__ __forceinline__ void tile_loop(const int* lo, const int* hi, int bx, int by, int bz, F&& f) {
+  const int t = threadIdx.x;
+  const int tx = t % bx, ty = (t / bx) % by, tz = t / (bx * by);
+  for (int k = lo[2] + tz; k <= hi[2]; k += bz)
+    for (int j = lo[1] + ty; j <= hi[1]; j += by)
+      for (int i = lo[0] + tx; i <= hi[0]; i += bx) f(i, j, k);
+}
+template <class T>
+__device__ __forceinline__ T smult(const SLev<T>& q, const T* __restrict__ v, int I) {
+  T s = v[I] * q.Dg[I];
+  s += (v[I - 1] * q.L[I] + v[I + 1] * q.L[I + 1]);
+  s += (v[I - q.s1] * q.L[q.n + I] + v[I + q.s1] * q.L[q.n + I + q.s1]);
+  if (q.D == 3) s += (v[I - q.s2] * q.L[2 * q.n + I] + v[I + q.s2] * q.L[2 * q.n + I + q.s2]);
+  return s;
+}
+// The coarse tail with every level resident in LDS (L, x, r, ϵ copied in, D and iD formed once,
+// x, r, ϵ copied back at the end): same passes and per-cell arithmetic as k_mg_tail.
+template <class T>
+__global__ void __launch_bounds__(1024) k_mg_tail_lds(TailDesc<T> d, T w, int bx, int by, int bz) {
+  extern __shared__ __align__(16) unsigned char tail_smem[];
+  T* sb = reinterpret_cast<T*>(tail_smem);
+  auto lev = [&](int l) -> SLev<T> {
+    SLev<T> q;
+    const Geo& g = d.lv[l].g;
+    q.D = g.D; q.N0 = g.N[0]; q.N1 = g.N[1]; q.N2 = g.N[2];
+    q.s1 = g.N[0]; q.s2 = g.N[0] * g.N[1]; q.n = (int)g.n; q.cm = d.lv[l].cmask;
+    q.L = sb + d.lv[l].off;
+    q.x = q.L + q.D * q.n; q.r = q.x + q.n; q.e = q.r + q.n; q.Dg = q.e + q.n; q.iD = q.Dg + q.n;
+    return q;
+  };
+  auto inside = [](const SLev<T>& q, int* lo, int* hi) {
+    lo[0] = lo[1] = 1; hi[0] = q.N0 - 2; hi[1] = q.N1 - 2;
+    if (q.D == 3) { lo[2] = 1; hi[2] = q.N2 - 2; } else { lo[2] = hi[2] = 0; }
+  };
+  for (int l = 0; l < d.n; ++l) {  // copy in
+    const SLev<T> q = lev(l);
+    const int nL = q.D * q.n;
+    for (int k = threadIdx.x; k < nL; k += blockDim.x) q.L[k] = d.lv[l].L[k];
+    for (int k = threadIdx.x; k < q.n; k += blockDim.x) {
+      q.x[k] = d.lv[l].x[k]; q.r[k] = d.lv[l].r[k]; q.e[k] = d.lv[l].e[k];
+      q.Dg[k] = T(0); q.iD[k] = T(0);
+    }
+  }
+  __syncthreads();
+  for (int l = 0; l < d.n; ++l) {  // D, iD on inside (src/Poisson.jl:43-46)
+    const SLev<T> q = lev(l);
+    int lo[3], hi[3];
+    inside(q, lo, hi);
+    tile_loop(lo, hi, bx, by, bz, [&](int i, int j, int k) {
+      const int I = i + q.s1 * j + q.s2 * k;
+      T sd = 0;
+      sd -= (q.L[I] + q.L[I + 1]);
+      sd -= (q.L[q.n + I] + q.L[q.n + I + q.s1]);
+      if (q.D == 3) sd -= (q.L[2 * q.n + I] + q.L[2 * q.n + I + q.s2]);
+      q.Dg[I] = sd;
+      q.iD[I] = sd == T(0) ? sd : T(1) / sd;
+    });
+  }
+  __syncthreads();
+  auto perbc = [&](const SLev<T>& q, T* a) {
+    if (!d.perdir) return;
+    const int lo[3] = {0, 0, 0}, hi[3] = {q.N0 - 1, q.N1 - 1, q.N2 - 1};
+    const int N[3] = {q.N0, q.N1, q.N2};
+    tile_loop(lo, hi, bx, by, bz, [&](int i, int j, int k) {
+      int c[3] = {i, j, k}, m[3] = {i, j, k};
+      bool ghost = false;
+      for (int r = 0; r < q.D; ++r) {
+        if (!((d.perdir >> r) & 1)) continue;
+        if (c[r] == 0) { m[r] = N[r] - 2; ghost = true; }
+        else if (c[r] == N[r] - 1) { m[r] = 1; ghost = true; }
+      }
+      if (ghost) a[i + q.s1 * j + q.s2 * k] = a[m[0] + q.s1 * m[1] + q.s2 * m[2]];
+    });
+    __syncthreads();
+  };
+  for (int l = 0; l + 1 < d.n; ++l) {  // down-leg: Jacobi! + restrict! + fill!
+    const SLev<T> f = lev(l), c = lev(l + 1);
+    perbc(f, f.e);
+    int lo[3], hi[3];
+    inside(c, lo, hi);
+    tile_loop(lo, hi, bx, by, bz, [&](int i, int j, int k) {
+      const int cc[3] = {i, j, k};
+      int a0[3], a1[3];
+      for (int q = 0; q < 3; ++q) {
+        if (q < f.D && ((f.cm >> q) & 1)) { a0[q] = 2 * cc[q] - 1; a1[q] = 2 * cc[q]; } else { a0[q] = a1[q] = cc[q]; }
+      }
+      T s = 0;
+      for (int kk = a0[2]; kk <= a1[2]; ++kk)
+        for (int jj = a0[1]; jj <= a1[1]; ++jj)
+          for (int ii = a0[0]; ii <= a1[0]; ++ii) {
+            const int J = ii + f.s1 * jj + f.s2 * kk;
+            const T rn = f.r[J] - T(1) * smult(f, f.e, J);
+            f.r[J] = rn;
+            f.x[J] = f.x[J] + T(1) * f.e[J];
+            s += rn;
+          }
+      const int Ic = i + c.s1 * j + c.s2 * k;
+      c.r[Ic] = s;
+      c.x[Ic] = 0;
+      c.e[Ic] = s * c.iD[Ic];
+    });
+    __syncthreads();
+  }
+  for (int l = d.n - 1; l >= 0; --l) {  // up-leg: smooth!(l), then prolongate! + increment!(l-1)
+    const SLev<T> p = lev(l);
+    perbc(p, p.e);
+    int lo[3], hi[3];
+    inside(p, lo, hi);
+    int hh[3] = {hi[0], hi[1], hi[2]};
+    const int Ld = p.D - 1;
+    hh[Ld] = (Ld == 2 ? p.N2 : p.N1) / 2 - 1;  // half_rangek
+    for (int it = 1; it <= 4; ++it) {
+      tile_loop(lo, hh, bx, by, bz, [&](int i, int j, int k) {
+        int q[3] = {i, j, k};
+        int front = 0;
+        for (int dd = 0; dd < Ld; ++dd) front += q[dd] + 1;
+        q[Ld] = 2 * (q[Ld] + 1) - 1 - (front + it) % 2 - 1;
+        const int I = q[0] + p.s1 * q[1] + p.s2 * q[2];
+        T s = p.r[I];
+        s -= (p.e[I - 1] * p.L[I] + p.e[I + 1] * p.L[I + 1]);
+        s -= (p.e[I - p.s1] * p.L[p.n + I] + p.e[I + p.s1] * p.L[p.n + I + p.s1]);
+        if (p.D == 3) s -= (p.e[I - p.s2] * p.L[2 * p.n + I] + p.e[I + p.s2] * p.L[2 * p.n + I + p.s2]);
+        p.e[I] = s * p.iD[I];
+      });
+      __syncthreads();
+    }
+    perbc(p, p.e);
+    tile_loop(lo, hi, bx, by, bz, [&](int i, int j, int k) {
+      const int I = i + p.s1 * j + p.s2 * k;
+      p.r[I] = p.r[I] - w * smult(p, p.e, I);
+      p.x[I] = p.x[I] + w * p.e[I];
+    });
+    __syncthreads();
+    if (l > 0) {
+      const SLev<T> f = lev(l - 1);
+      int flo[3], fhi[3];
+      inside(f, flo, fhi);
+      const int N[3] = {f.N0, f.N1, f.N2};
+      tile_loop(flo, fhi, bx, by, bz, [&](int i, int j, int k) {
+        const int c[3] = {i, j, k};
+        int q[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+        for (int dd = 0; dd < f.D; ++dd)
+          for (int o = 0; o < 3; ++o) q[dd][o] = pro_map(c[dd] + o - 1, N[dd], (d.perdir >> dd) & 1, (f.cm >> dd) & 1);
+        auto xcv = [&](int a, int b, int z) -> T { return (a < 0 || b < 0 || z < 0) ? T(0) : p.x[a + p.s1 * b + p.s2 * z]; };
+        const int I = i + f.s1 * j + f.s2 * k;
+        const T e0 = xcv(q[0][1], q[1][1], q[2][1]);
+        T s = e0 * f.Dg[I];
+        s += (xcv(q[0][0], q[1][1], q[2][1]) * f.L[I] + xcv(q[0][2], q[1][1], q[2][1]) * f.L[I + 1]);
+        s += (xcv(q[0][1], q[1][0], q[2][1]) * f.L[f.n + I] + xcv(q[0][1], q[1][2], q[2][1]) * f.L[f.n + I + f.s1]);
+        if (f.D == 3)
+          s += (xcv(q[0][1], q[1][1], q[2][0]) * f.L[2 * f.n + I] + xcv(q[0][1], q[1][1], q[2][2]) * f.L[2 * f.n + I + f.s2]);
+        const T rn = f.r[I] - w * s;
+        f.r[I] = rn;
+        f.x[I] = f.x[I] + w * e0;
+        f.e[I] = rn * f.iD[I];
+      });
+      __syncthreads();
+    }
+  }
+  for (int l = 0; l < d.n; ++l) {  // copy back
+    const SLev<T> q = lev(l);
+    for (int k = threadIdx.x; k < q.n; k += blockDim.x) {
+      d.lv[l].x[k] = q.x[k]; d.lv[l].r[k] = q.r[k]; d.lv[l].e[k] = q.e[k];
     }
   }
 }

@@ -375,7 +375,7 @@ struct MG {
   // LDS bytes the levels l..end need in the coarse tail (L, x, r, ϵ of each)
   int64_t tail_bytes(int l) const {
     int64_t b = 0;
-    for (size_t k = l; k < lv.size(); ++k) b += (int64_t)(lv[k].g.D + 3) * lv[k].g.n * (int64_t)sizeof(T);
+    for (size_t k = l; k < lv.size(); ++k) b += (int64_t)(lv[k].g.D + 5) * lv[k].g.n * (int64_t)sizeof(T);
     return b;
   }
   // First level run by the single-workgroup coarse tail: the first level (>= 1) from which every
@@ -411,12 +411,19 @@ struct MG {
       d.lv[k].e = q.e;
       d.lv[k].cmask = q.cmask;
       d.lv[k].off = off;
-      off += (int64_t)(q.g.D + 3) * q.g.n;
+      off += (int64_t)(q.g.D + 5) * q.g.n;
     }
     ProfScope ps(P, st, "mg_tail", Lt, 0.0);
-    if (in_lds)
-      hipLaunchKernelGGL((k_mg_tail<T, true>), dim3(1), dim3(1024), (size_t)(off * sizeof(T)), st, d, w);
-    else
+    if (in_lds) {
+      static bool attr_set = false;
+      if (!attr_set) {
+        HIPCK(hipFuncSetAttribute((const void*)k_mg_tail_lds<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  TAIL_LDS_BYTES));
+        attr_set = true;
+      }
+      const int bx = lv[Lt].g.D == 3 ? 16 : 32, by = lv[Lt].g.D == 3 ? 8 : 32, bz = lv[Lt].g.D == 3 ? 8 : 1;
+      hipLaunchKernelGGL(k_mg_tail_lds<T>, dim3(1), dim3(1024), (size_t)(off * sizeof(T)), st, d, w, bx, by, bz);
+    } else
       hipLaunchKernelGGL((k_mg_tail<T, false>), dim3(1), dim3(1024), 0, st, d, w);
     KCK();
   }
