@@ -132,14 +132,10 @@ __device__ __forceinline__ int block_linear_id() {
 // ---------------------------------------------------------------------------------------------
 template <class T>
 __device__ __forceinline__ T med3(T a, T b, T c) {
-  if (a > b) {
-    if (b >= c) return b;
-    if (a > c) return c;
-  } else {
-    if (b <= c) return b;
-    if (a < c) return c;
-  }
-  return a;
+  // the branch structure of median(a,b,c) (src/Flow.jl:27-36) as selects
+  const T hi = b >= c ? b : (a > c ? c : a);  // a > b
+  const T lo = b <= c ? b : (a < c ? c : a);  // a <= b (or unordered)
+  return a > b ? hi : lo;
 }
 // x/6 rounded to nearest.  For float, a reciprocal multiply with one FMA correction (Markstein)
 // equals the IEEE quotient for every |x| in [2^-100, 2^100) (checked for all floats on the CPU

@@ -104,9 +104,40 @@ __global__ void __launch_bounds__(64 * CBY) k_conv_bdim1(Geo g, ZT t, const T* _
     T F[3][3];
     for (int i = 0; i < 3; ++i)
       for (int j = 0; j < 3; ++j) F[i][j] = 0;
-    if (valid)
-      for (int i = 0; i < D; ++i)
-        for (int j = 0; j < D; ++j) F[i][j] = lower_flux<T, S>(g, u, i, j, c, I, perdir, nu);
+    if (valid) {
+      // every distinct velocity the 9 face fluxes of this cell read: u_i[I] and u_i[I-e_j]
+      T U0[3], UM[3][3];
+      for (int i = 0; i < D; ++i) {
+        U0[i] = u[I + i * n];
+        for (int j = 0; j < D; ++j) UM[i][j] = u[I - g.s[j] + i * n];
+      }
+      for (int j = 0; j < D; ++j) {
+        const int Nj = g.N[j];
+        const int64_t sj = g.s[j];
+        const bool per = (perdir >> j) & 1;
+        if (per && c[j] == Nj - 1) {  // the periodic image of face 1 (rare lanes)
+          const int64_t K = I - (int64_t)(Nj - 2) * sj;
+          for (int i = 0; i < D; ++i)
+            F[i][j] = face_flux<T, S>(3, u + i * n, u + j * n, K, g.s[i], sj, K + (int64_t)(Nj - 4) * sj, nu);
+          continue;
+        }
+        // variant of the lower j-face: 1 inlet, 2 outlet, 3 periodic low, 0 interior
+        const int var = c[j] == 1 ? (per ? 3 : 1) : (c[j] == Nj - 1 ? 2 : 0);
+        const int64_t Ip = I + (int64_t)(Nj - 4) * sj;
+        for (int i = 0; i < D; ++i) {
+          const T fI = U0[i], fm = UM[i][j];
+          const T ua = (U0[j] + UM[j][i]) / T(2);
+          const bool cd = (var == 1 && ua > 0) || (var == 2 && ua < 0);
+          const bool pos = var == 2 ? !(ua < 0) : ua > 0;
+          int64_t ix = pos ? (var == 3 ? Ip : I - 2 * sj) : I + sj;
+          ix = cd ? I : ix;
+          const T fx = u[ix + i * n];
+          const T lv = lam<T, S>(fx, pos ? fm : fI, pos ? fI : fm);
+          const T phi = cd ? ua * ((fI + fm) / T(2)) : ua * lv;
+          F[i][j] = phi - nu * (fI - fm);
+        }
+      }
+    }
     for (int i = 0; i < D; ++i) sFy[i][ty][tx] = F[i][1];
     __syncthreads();
     T Fyu[3] = {0, 0, 0};
