@@ -137,6 +137,13 @@ __device__ __forceinline__ T med3(T a, T b, T c) {
   const T lo = b <= c ? b : (a < c ? c : a);  // a <= b (or unordered)
   return a > b ? hi : lo;
 }
+// Float: the hardware median of three (v_med3_f32).  For ordered inputs the median value is
+// unique, so this equals the branch form except possibly in the sign of an exact zero when both
+// +0 and -0 are among the inputs.
+template <>
+__device__ __forceinline__ float med3<float>(float a, float b, float c) {
+  return __builtin_amdgcn_fmed3f(a, b, c);
+}
 // x/6 rounded to nearest.  For float, a reciprocal multiply with one FMA correction (Markstein)
 // equals the IEEE quotient for every |x| in [2^-100, 2^100) (checked for all floats on the CPU
 // and on the device by wl_self_test); outside that range the division is done directly.

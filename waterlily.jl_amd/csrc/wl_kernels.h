@@ -183,6 +183,183 @@ __global__ void __launch_bounds__(64 * CBY) k_conv_bdim1(Geo g, ZT t, const T* _
   if (D == 3 && zend == z1 && outxy) finish(z1, rpart);  // top ghost plane: no z fluxes
 }
 
+// The same fused conv_diff! + accelerate! + BDIM! first loop for non-periodic flows, with every
+// stencil read from LDS or registers: each thread keeps a 4-plane window (z-2..z+1) of the three
+// velocity components at its own (x,y); plane z of the tile (+2/-1 halo in x and y) is staged in
+// LDS, so the 9 face fluxes of a cell cost no global address arithmetic.
+template <class T, int S, bool PRED, int D, bool PF>
+__global__ void __launch_bounds__(64 * CBY) k_conv_lds(Geo g, ZT t, const T* __restrict__ u, T* __restrict__ u0,
+                                                       T* __restrict__ f, T* __restrict__ sigma,
+                                                       const T* __restrict__ V, T nu, T dt, Accel acc) {
+  constexpr int RX = 64 + 3, RY = CBY + 3;  // staged x range x0-2..x0+64, y range y0-2..y0+CBY
+  constexpr int NH = 3 * RY + 3 * 64;       // halo cells per component and plane
+  constexpr int NHT = (D * NH + 64 * CBY - 1) / (64 * CBY);  // halo loads per thread
+  __shared__ T sU[3][RY][RX];
+  __shared__ T sFy[3][CBY][64];
+  int cx, cy, z0, z1;
+  zt_cell(t, cx, cy, z0, z1);
+  const int tx = threadIdx.x, ty = threadIdx.y, tid = tx + 64 * ty;
+  const int x0 = cx - tx, y0 = cy - ty;
+  const int64_t n = g.n;
+  const int N0 = g.N[0], N1 = g.N[1], N2 = g.N[2];
+  const bool ingrid = cx <= N0 - 1 && cy <= N1 - 1;
+  const bool outxy = ingrid && tx < 63 && ty < CBY - 1;
+  const int zend = (D == 3 && z1 + 1 <= N2 - 1) ? z1 + 1 : z1;
+  const int rx = tx + 2, ry = ty + 2;
+  auto ldc = [&](int i, int z) -> T {
+    return (ingrid && z >= 0 && z <= N2 - 1) ? u[at(g, cx, cy, z) + i * n] : T(0);
+  };
+  // halo cell h of a plane: (component, staged x, staged y)
+  auto halo_pos = [&](int h, int& i, int& hx, int& hy) {
+    i = h / NH;
+    const int q = h % NH;
+    const bool col = q < 3 * RY;  // one of the x-halo columns 0, 1, RX-1, else a y-halo row
+    const int q2 = q - 3 * RY;
+    const int a = col ? q / RY : q2 / 64;  // 0, 1 or 2 (the far side)
+    const int b = col ? q % RY : 2 + q2 % 64;
+    hx = col ? (a == 2 ? RX - 1 : a) : b;
+    hy = col ? b : (a == 2 ? RY - 1 : a);
+  };
+  auto ld_halo = [&](int k, int z) -> T {
+    const int h = tid + k * 64 * CBY;
+    if (h >= D * NH) return T(0);
+    int i, hx, hy;
+    halo_pos(h, i, hx, hy);
+    const int gx = x0 - 2 + hx, gy = y0 - 2 + hy;
+    return (gx >= 0 && gx <= N0 - 1 && gy >= 0 && gy <= N1 - 1) ? u[at(g, gx, gy, z) + i * n] : T(0);
+  };
+  // register window W[i][k]: component i at planes z-2+k (k = 0..3) of this column
+  T W[3][4];
+  for (int i = 0; i < 3; ++i)
+    for (int k = 0; k < 4; ++k) W[i][k] = (i < D) ? ldc(i, z0 - 2 + k) : T(0);
+  // prefetched values for the next plane: window plane z+2, halo ring of plane z+1, V and u⁰ of plane z
+  T Wn[3] = {0, 0, 0}, Hn[NHT], Hc[NHT], Vn[3] = {0, 0, 0}, Un[3] = {0, 0, 0};
+  for (int k = 0; k < NHT; ++k) Hc[k] = ld_halo(k, z0);
+  T rpart[3] = {0, 0, 0}, Fzp[3] = {0, 0, 0}, Vp[3] = {0, 0, 0}, Up[3] = {0, 0, 0};
+
+  struct T3 { T v[3]; };
+  auto finish = [&](int cz, T3 rr, T3 uvals, T3 Vv, T3 U0v) {
+    const int64_t I = at(g, cx, cy, cz);
+#pragma unroll
+    for (int i = 0; i < D; ++i) {
+      T r = rr.v[i];
+      if (acc.on) r += (T)acc.add[i];
+      const T uv = uvals.v[i];
+      const T u0v = PRED ? uv : U0v.v[i];
+      f[I + i * n] = u0v + dt * r - Vv.v[i];
+      if (PRED) u0[I + i * n] = uv;
+    }
+  };
+  auto ld_cell = [&](const T* a, int i, int z) -> T { return (outxy && z <= N2 - 1) ? a[at(g, cx, cy, z) + i * n] : T(0); };
+
+  for (int cz = z0; cz <= zend; ++cz) {
+    if (!PF && cz > z0) {  // without prefetch: advance the window and load the halo now
+      for (int i = 0; i < D; ++i) { W[i][0] = W[i][1]; W[i][1] = W[i][2]; W[i][2] = W[i][3]; W[i][3] = D == 3 ? ldc(i, cz + 1) : T(0); }
+      for (int k = 0; k < NHT; ++k) Hc[k] = ld_halo(k, cz);
+    }
+    // stage plane cz: own values from the window, the halo ring
+    for (int i = 0; i < D; ++i) sU[i][ry][rx] = W[i][2];
+    for (int k = 0; k < NHT; ++k) {
+      const int h = tid + k * 64 * CBY;
+      if (h < D * NH) {
+        int i, hx, hy;
+        halo_pos(h, i, hx, hy);
+        sU[i][hy][hx] = Hc[k];
+      }
+    }
+    // prefetch for the next plane (consumed after the barriers below)
+    if (PF && cz < zend) {
+      for (int i = 0; i < D; ++i) Wn[i] = D == 3 ? ldc(i, cz + 2) : T(0);
+      for (int k = 0; k < NHT; ++k) Hn[k] = ld_halo(k, cz + 1);
+    }
+    for (int i = 0; i < D; ++i) {
+      Vn[i] = ld_cell(V, i, cz);
+      if (!PRED) Un[i] = ld_cell(u0, i, cz);
+    }
+    __syncthreads();
+    const int c[3] = {cx, cy, cz};
+    const bool valid = ingrid && cx >= 1 && cy >= 1 && (D == 2 || cz >= 1);
+    T F[3][3];
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) F[i][j] = 0;
+    if (valid) {
+      for (int j = 0; j < D; ++j) {
+        const int Nj = g.N[j];
+        const int var = c[j] == 1 ? 1 : (c[j] == Nj - 1 ? 2 : 0);
+        for (int i = 0; i < D; ++i) {
+          // values of component i along direction j around the face: I-2, I-1, I, I+1
+          T vm2, vm1, v0, vp1;
+          if (j == 0) { vm2 = sU[i][ry][rx - 2]; vm1 = sU[i][ry][rx - 1]; v0 = sU[i][ry][rx]; vp1 = sU[i][ry][rx + 1]; }
+          else if (j == 1) { vm2 = sU[i][ry - 2][rx]; vm1 = sU[i][ry - 1][rx]; v0 = sU[i][ry][rx]; vp1 = sU[i][ry + 1][rx]; }
+          else { vm2 = W[i][0]; vm1 = W[i][1]; v0 = W[i][2]; vp1 = W[i][3]; }
+          // u_j at I and at I - e_i (the advecting velocity on the i-face)
+          const T ujI = (j == 2) ? W[2][2] : sU[j][ry][rx];
+          const T ujm = (i == 0) ? sU[j][ry][rx - 1] : (i == 1) ? sU[j][ry - 1][rx] : W[j][1];
+          const T ua = (ujI + ujm) / T(2);
+          const T fI = v0, fm = vm1;
+          const bool cd = (var == 1 && ua > 0) || (var == 2 && ua < 0);
+          const bool pos = var == 2 ? !(ua < 0) : ua > 0;
+          const T fx = pos ? vm2 : vp1;
+          const T lv = lam<T, S>(fx, pos ? fm : fI, pos ? fI : fm);
+          const T phi = cd ? ua * ((fI + fm) / T(2)) : ua * lv;
+          F[i][j] = phi - nu * (fI - fm);
+        }
+      }
+    }
+    for (int i = 0; i < D; ++i) sFy[i][ty][tx] = F[i][1];
+    __syncthreads();
+    T Fyu[3] = {0, 0, 0};
+    if (ty < CBY - 1)
+      for (int i = 0; i < D; ++i) Fyu[i] = sFy[i][ty + 1][tx];
+    T Fxu[3];
+    for (int i = 0; i < 3; ++i) Fxu[i] = __shfl_down(F[i][0], 1, 64);
+    T rxy[3] = {0, 0, 0};
+    if (valid)
+      for (int i = 0; i < D; ++i) {
+        T r = 0;
+        if (cx <= N0 - 2) { r += F[i][0]; r -= Fxu[i]; }
+        if (cy <= N1 - 2) { r += F[i][1]; r -= Fyu[i]; }
+        rxy[i] = r;
+      }
+    if (D == 2) {
+      if (outxy) finish(cz, T3{{rxy[0], rxy[1], rxy[2]}}, T3{{W[0][2], W[1][2], W[2][2]}}, T3{{Vn[0], Vn[1], Vn[2]}},
+                        T3{{Un[0], Un[1], Un[2]}});
+    } else {
+      if (cz > z0 && outxy) {  // plane cz-1 is complete: add its z fluxes
+        const bool zin = cx >= 1 && cy >= 1 && cz - 1 >= 1 && cz - 1 <= N2 - 2;
+        T rr[3];
+        for (int i = 0; i < 3; ++i) {
+          rr[i] = rpart[i];
+          if (zin) { rr[i] += Fzp[i]; rr[i] -= F[i][2]; }
+        }
+        finish(cz - 1, T3{{rr[0], rr[1], rr[2]}}, T3{{W[0][1], W[1][1], W[2][1]}}, T3{{Vp[0], Vp[1], Vp[2]}},
+               T3{{Up[0], Up[1], Up[2]}});
+      }
+      for (int i = 0; i < 3; ++i) { rpart[i] = rxy[i]; Fzp[i] = F[i][2]; Vp[i] = Vn[i]; Up[i] = Un[i]; }
+    }
+    // Φ on the top ghost layers (component D, last direction whose inner range holds the cell)
+    if (valid && outxy && cz <= z1) {
+      bool anytop = false;
+      for (int d = 0; d < D; ++d) anytop |= c[d] == g.N[d] - 1;
+      if (anytop) {
+        int js = -1;
+        for (int j = 0; j < D; ++j)
+          if (c[j] >= 2 && c[j] <= g.N[j] - 2) js = j;
+        if (js >= 0) sigma[at(g, cx, cy, cz)] = F[D - 1][js];
+      }
+    }
+    // advance the window to planes cz-1..cz+2 and take the prefetched halo
+    if (PF && cz < zend) {
+      for (int i = 0; i < D; ++i) { W[i][0] = W[i][1]; W[i][1] = W[i][2]; W[i][2] = W[i][3]; W[i][3] = Wn[i]; }
+      for (int k = 0; k < NHT; ++k) Hc[k] = Hn[k];
+    }
+  }
+  if (D == 3 && zend == z1 && outxy) {  // top ghost plane: no z fluxes
+    finish(z1, T3{{rpart[0], rpart[1], rpart[2]}}, T3{{W[0][2], W[1][2], W[2][2]}}, T3{{Vp[0], Vp[1], Vp[2]}},
+           T3{{Up[0], Up[1], Up[2]}});
+  }
+}
+
 // BDIM! second half + scale_u!: u += μddn(μ₁,f) + V + μ₀·f on inside (src/Flow.jl:179,211-214).
 // PRED: u was scaled by 0 before (mom_step!, :157); CORR: the result is scaled by ½ (:209).
 template <class T, bool PRED>

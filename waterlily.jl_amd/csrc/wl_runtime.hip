@@ -102,6 +102,14 @@ static ZT make_zt(const Box& b, int target = 0) {
   t.zc = (int)std::min<int64_t>(32, std::max<int64_t>(1, zc));
   return t;
 }
+// conv kernel variant: prefetch the next plane one iteration ahead (WL_CONV_PREFETCH=0 disables)
+static bool conv_prefetch() {
+  static bool v = [] {
+    const char* e = std::getenv("WL_CONV_PREFETCH");
+    return !e || std::atoi(e) != 0;
+  }();
+  return v;
+}
 static int64_t zt_blocks(const ZT& t) {
   const int nz = t.b.hi[2] - t.b.lo[2] + 1;
   return (int64_t)t.nbx * t.nby * ((nz + t.zc - 1) / t.zc);
@@ -719,12 +727,28 @@ struct FlowT : FlowBase {
     const dim3 cblk(64, CBY, 1);
     {
       ProfScope ps(&prof, st, "conv_bdim1", 0, (double)sizeof(T) * 4 * D * g.n);
-      if (pred)
-        hipLaunchKernelGGL((k_conv_bdim1<T, S, true, DD>), zt_grid(ba), cblk, 0, st, g, ba, (const T*)u, u0, f, sigma,
-                           (const T*)V, nu, dt, desc.perdir_mask, acc);
-      else
-        hipLaunchKernelGGL((k_conv_bdim1<T, S, false, DD>), zt_grid(ba), cblk, 0, st, g, ba, (const T*)u, u0, f,
-                           sigma, (const T*)V, nu, dt, desc.perdir_mask, acc);
+      if (desc.perdir_mask) {
+        if (pred)
+          hipLaunchKernelGGL((k_conv_bdim1<T, S, true, DD>), zt_grid(ba), cblk, 0, st, g, ba, (const T*)u, u0, f,
+                             sigma, (const T*)V, nu, dt, desc.perdir_mask, acc);
+        else
+          hipLaunchKernelGGL((k_conv_bdim1<T, S, false, DD>), zt_grid(ba), cblk, 0, st, g, ba, (const T*)u, u0, f,
+                             sigma, (const T*)V, nu, dt, desc.perdir_mask, acc);
+      } else if (conv_prefetch()) {
+        if (pred)
+          hipLaunchKernelGGL((k_conv_lds<T, S, true, DD, true>), zt_grid(ba), cblk, 0, st, g, ba, (const T*)u, u0, f,
+                             sigma, (const T*)V, nu, dt, acc);
+        else
+          hipLaunchKernelGGL((k_conv_lds<T, S, false, DD, true>), zt_grid(ba), cblk, 0, st, g, ba, (const T*)u, u0, f,
+                             sigma, (const T*)V, nu, dt, acc);
+      } else {
+        if (pred)
+          hipLaunchKernelGGL((k_conv_lds<T, S, true, DD, false>), zt_grid(ba), cblk, 0, st, g, ba, (const T*)u, u0, f,
+                             sigma, (const T*)V, nu, dt, acc);
+        else
+          hipLaunchKernelGGL((k_conv_lds<T, S, false, DD, false>), zt_grid(ba), cblk, 0, st, g, ba, (const T*)u, u0, f,
+                             sigma, (const T*)V, nu, dt, acc);
+      }
       KCK();
     }
     ZT bi = make_zt(box_inside(g));
