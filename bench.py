@@ -28,25 +28,33 @@ def sphere_sim(wl, n, R, center, Re, L):
     return wl.Simulation(n, (1, 0, 0), L, ν=L / Re, body=wl.Sphere(center, R), T=np.float32)
 
 
-def run_gpu(wl, n, R, center, Re, L, steps, warmup, profile=True):
+def run_gpu(wl, n, R, center, Re, L, steps, warmup, profile_steps=None):
+    """Warm up, time `steps` sim_step!s, then run `profile_steps` more with per-kernel HIP events
+    recorded on the library's stream (the events add launch gaps, so they stay out of the timed steps)."""
     sim = sphere_sim(wl, n, R, center, Re, L)
     for _ in range(warmup):
         wl.sim_step(sim, remeasure=False)
     n0 = len(sim.pois.n)
-    if profile:
-        sim.flow.profile(True)
     t0 = time.perf_counter()
     for _ in range(steps):
         wl.sim_step(sim, remeasure=False)
     t1 = time.perf_counter()  # every sim_step returns after its CFL reduction reached the host
-    stats = sim.flow.profile_read() if profile else []
-    if profile:
+    nv = sim.pois.n[n0:]
+    stats, psec = [], None
+    ps = steps if profile_steps is None else profile_steps
+    if ps > 0:
+        sim.flow.profile(True)
+        p0 = time.perf_counter()
+        for _ in range(ps):
+            wl.sim_step(sim, remeasure=False)
+        psec = (time.perf_counter() - p0) / ps
+        stats = sim.flow.profile_read()
         sim.flow.profile(False)
     cells = int(np.prod(n))
     sec = t1 - t0
-    nv = sim.pois.n[n0:]
     return dict(sec=sec, ms_per_step=1e3 * sec / steps, mcells=cells * steps / sec / 1e6, steps_per_s=steps / sec,
-                vcycles_per_step=sum(nv) / steps, stats=stats, sim=sim)
+                vcycles_per_step=sum(nv) / steps, stats=stats, sim=sim,
+                profiled_ms_per_step=None if psec is None else 1e3 * psec)
 
 
 def roofline(stats, traffic_file):
@@ -201,7 +209,9 @@ def main():
                                     "Float32, tol Σr²<1e-4", grid=list(n), sphere_radius=R, Re=Re,
                            method="plain", remeasure=False),
                sim_steps_per_s=round(r["steps_per_s"], 3), vcycles_per_step=round(r["vcycles_per_step"], 3),
-               roofline=roof)
+               roofline=roof, profiled_ms_per_step=round(r["profiled_ms_per_step"], 4),
+               timing="value from the timed steps with no per-kernel events; per-kernel HIP-event times "
+                      "from a second pass of the same number of steps")
     if not a.quiet_kernels:
         out["kernels"] = rows[:14]
     if not a.no_c3:

@@ -635,8 +635,9 @@ __global__ void __launch_bounds__(256) k_eps_init(Geo g, ZT t, const T* __restri
 // the finest level after a V-cycle, src/MultiLevelPoisson.jl:115).
 template <class T, bool L2>
 __global__ void __launch_bounds__(256) k_increment(Geo g, ZT t, const T* __restrict__ L, T* __restrict__ r,
-                                                   T* __restrict__ x, const T* __restrict__ e, T w,
+                                                   T* __restrict__ x, const T* __restrict__ e, const T* __restrict__ wp,
                                                    double* __restrict__ partial) {
+  const T w = *wp;
   int c0, c1, z0, z1;
   zt_cell(t, c0, c1, z0, z1);
   double acc = 0;
@@ -726,7 +727,8 @@ template <class T>
 __global__ void __launch_bounds__(256) k_prolong_inc(Geo gf, Geo gc, ZT t, int cm, int perdir,
                                                      const T* __restrict__ Lf, T* __restrict__ rf,
                                                      T* __restrict__ xf, T* __restrict__ ef,
-                                                     const T* __restrict__ xc, T w) {
+                                                     const T* __restrict__ xc, const T* __restrict__ wp) {
+  const T w = *wp;
   int c[3], z0, z1;
   zt_cell(t, c[0], c[1], z0, z1);
   if (c[0] > t.b.hi[0] || c[1] > t.b.hi[1]) return;
@@ -803,7 +805,8 @@ struct TLev {
 // LDS: every tail level's L, x, r, ϵ are copied into LDS first and x, r, ϵ copied back at the
 // end, so each pass runs at LDS latency; otherwise the passes work on the global arrays.
 template <class T, bool LDS>
-__global__ void __launch_bounds__(1024) k_mg_tail(TailDesc<T> d, T w) {
+__global__ void __launch_bounds__(1024) k_mg_tail(TailDesc<T> d, const T* __restrict__ wp) {
+  const T w = *wp;
   extern __shared__ __align__(16) unsigned char tail_smem[];
   T* sb = reinterpret_cast<T*>(tail_smem);
   auto lev = [&](int l) -> TLev<T> {
@@ -920,7 +923,9 @@ __device__ __forceinline__ T smult(const SLev<T>& q, const T* __restrict__ v, in
 // The coarse tail with every level resident in LDS (L, x, r, ϵ copied in, D and iD formed once,
 // x, r, ϵ copied back at the end): same passes and per-cell arithmetic as k_mg_tail.
 template <class T>
-__global__ void __launch_bounds__(1024) k_mg_tail_lds(TailDesc<T> d, T w, int bx, int by, int bz) {
+__global__ void __launch_bounds__(1024) k_mg_tail_lds(TailDesc<T> d, const T* __restrict__ wp, int bx, int by,
+                                                      int bz) {
+  const T w = *wp;
   extern __shared__ __align__(16) unsigned char tail_smem[];
   T* sb = reinterpret_cast<T*>(tail_smem);
   auto lev = [&](int l) -> SLev<T> {
@@ -1309,6 +1314,26 @@ __global__ void __launch_bounds__(256) k_pressure_force(Geo g, Box b, DBodies bs
     const double s = block_sum(acc[i]);
     if (threadIdx.x == 0 && threadIdx.y == 0) partial[block_linear_id() + i * pstride] = s;
   }
+}
+
+// ω of solver!(ml) kept on the device (src/MultiLevelPoisson.jl:110,117-121): st[3] = ω,
+// st[4] = r₂ (both in T).  k_mg_init sets ω = 1 and r₂ = Σr² after residual!; k_mg_control
+// takes the new Σr² (dsc[1], rounded to T), adapts ω, stores r₂ and keeps the old r₂ in dsc[6].
+template <class T>
+__global__ void k_mg_init(const double* __restrict__ dsc, T* __restrict__ st) {
+  if (threadIdx.x == 0) { st[3] = T(1); st[4] = (T)dsc[1]; }
+}
+template <class T>
+__global__ void k_mg_control(double* __restrict__ dsc, T* __restrict__ st) {
+  if (threadIdx.x != 0) return;
+  const T rnew = (T)dsc[1];
+  T w = st[3];
+  const T r2 = st[4];
+  if (rnew >= r2) w = (T)fmax(0.2, 0.9 * (double)w);
+  else if (rnew < r2) w = (T)fmin(1.0, 1.02 * (double)w);
+  st[3] = w;
+  st[4] = rnew;
+  dsc[6] = (double)r2;
 }
 
 // Device self-test: counts floats x (all 2^32 bit patterns, finite ones) for which div6(x)
