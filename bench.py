@@ -114,12 +114,16 @@ def copy_calibration(nbytes=1 << 30, reps=5):
 
 
 # rocprofv3 kernel names -> the library's profiler keys (level 0 = the largest dispatch of a name)
-PROF_KEYS = {"k_inc_restrict_full": "inc_restrict", "k_mg_tail": "mg_tail", "k_conv_bdim1": "conv_bdim1", "k_bdim2": "bdim2", "k_residual": "residual", "k_submean_l2": "submean_l2",
+PROF_KEYS = {"k_inc_restrict_full": "inc_restrict", "k_mg_tail": "mg_tail", "k_mg_tail_lds": "mg_tail", "k_bdim2": "bdim2", "k_residual": "residual", "k_submean_l2": "submean_l2",
              "k_inc_restrict": "inc_restrict", "k_gs_half": "gs_half", "k_increment": "increment",
              "k_prolong_inc": "prolong_inc", "k_eps_init": "eps_init", "k_correct": "correct", "k_cfl": "cfl"}
 
 
 def _key_of(kname):
+    for k in ("k_conv_lds", "k_conv_bdim1"):  # template arguments <T, scheme, PRED, ...>
+        if f"wl::{k}<" in kname:
+            args = kname.split(f"wl::{k}<")[1].split(">")[0].split(",")
+            return "conv_pred" if args[2].strip() == "true" else "conv_corr"
     for k, v in PROF_KEYS.items():
         if f"wl::{k}<" in kname or f"wl::{k}(" in kname:
             return v

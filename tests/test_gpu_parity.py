@@ -129,6 +129,30 @@ def test_graph_replay_matches_direct(wl, monkeypatch):
     say("  n =", g.pois.n)
 
 
+@pytest.mark.parametrize("D", [2, 3])
+def test_exit_bc(wl, D):
+    """exitBC=true (src/core.jl:226-233; src/Flow.jl:141,195): convective exit with the mean-flux
+    correction, against the Float32 oracle (2D: 20 steps of a circle; 3D: 3 steps of a sphere)."""
+    n, c, R, steps = ((96, 64), (31, 31), 8, 20) if D == 2 else ((64, 32, 32), (20, 16, 16), 6, 3)
+    say(f"case: exitBC D={D} {n}, {steps} steps")
+    uBC = (1,) + (0,) * (D - 1)
+    sim = wl.Simulation(n, uBC, 2 * R, ν=2 * R / 250, body=wl.Sphere(c, R), T=np.float32, exitBC=True)
+    osim = O.OSimulation(n, uBC, 2 * R, nu=2 * R / 250, dtype=O.WL_F32, exit_bc=True,
+                         bodies=[{"kind": 1, "center": tuple(c), "radius": R}])
+    for _ in range(steps):
+        wl.sim_step(sim, remeasure=False)
+        osim.step(remeasure=False)
+    u, uo = sim.flow.u, osim.flow.get("u")
+    eu, ep = rel(u, uo), rel(sim.flow.p, osim.flow.get("p"))
+    say(f"  rel-L2 u {eu:.3e} p {ep:.3e}; n {sim.pois.n[-4:]}")
+    assert eu < 1e-5 and ep < 1e-5
+    assert sim.pois.n == osim.n
+    # the exit plane carries the inflow's mass flux (the correction's purpose)
+    ex = u[-1, 1:-1, 0] if D == 2 else u[-1, 1:-1, 1:-1, 0]
+    inl = u[1, 1:-1, 0] if D == 2 else u[1, 1:-1, 1:-1, 0]
+    assert abs(float(np.mean(ex, dtype=np.float64)) - float(np.mean(inl, dtype=np.float64))) < 1e-5
+
+
 def test_impulsive_box(wl):
     say("case: impulsive box")
     U = (2 / 3, -1 / 3)

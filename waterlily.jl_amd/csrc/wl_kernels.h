@@ -65,12 +65,13 @@ __device__ __forceinline__ T lower_flux(const Geo& g, const T* __restrict__ u, i
 // through the upper x face comes from the next lane, the upper y face from the next row (LDS),
 // the upper z face from the next plane (so plane z is finished one iteration later).  Tiles
 // overlap by one column in x and one row in y (the last lane/row only supplies fluxes).
-// PRED: the convecting field is u (== u⁰ of this step) and u⁰ := u is written here.
+// PRED: the convecting field is u (== u⁰ of this step); u⁰ is not read, and the caller makes the
+// buffer of u the step's u⁰ afterwards instead of copying it.
 // σ receives the conv_diff! scratch Φ on the top ghost layers (the last (i=D, j) flux written
 // there by the reference), which the CFL maximum reads (src/Flow.jl:236).
 constexpr int CBY = 8;
 template <class T, int S, bool PRED, int D>
-__global__ void __launch_bounds__(64 * CBY) k_conv_bdim1(Geo g, ZT t, const T* __restrict__ u, T* __restrict__ u0,
+__global__ void __launch_bounds__(64 * CBY) k_conv_bdim1(Geo g, ZT t, const T* __restrict__ u, const T* __restrict__ u0,
                                                          T* __restrict__ f, T* __restrict__ sigma,
                                                          const T* __restrict__ V, T nu, T dt, int perdir,
                                                          Accel acc) {
@@ -92,7 +93,6 @@ __global__ void __launch_bounds__(64 * CBY) k_conv_bdim1(Geo g, ZT t, const T* _
       const T uv = u[I + i * n];
       const T u0v = PRED ? uv : u0[I + i * n];
       f[I + i * n] = u0v + dt * r - V[I + i * n];
-      if (PRED) u0[I + i * n] = uv;
     }
   };
 
@@ -188,7 +188,7 @@ __global__ void __launch_bounds__(64 * CBY) k_conv_bdim1(Geo g, ZT t, const T* _
 // velocity components at its own (x,y); plane z of the tile (+2/-1 halo in x and y) is staged in
 // LDS, so the 9 face fluxes of a cell cost no global address arithmetic.
 template <class T, int S, bool PRED, int D, bool PF>
-__global__ void __launch_bounds__(64 * CBY) k_conv_lds(Geo g, ZT t, const T* __restrict__ u, T* __restrict__ u0,
+__global__ void __launch_bounds__(64 * CBY, sizeof(T) == 4 ? 4 : 2) k_conv_lds(Geo g, ZT t, const T* __restrict__ u, const T* __restrict__ u0,
                                                        T* __restrict__ f, T* __restrict__ sigma,
                                                        const T* __restrict__ V, T nu, T dt, Accel acc) {
   constexpr int RX = 64 + 3, RY = CBY + 3;  // staged x range x0-2..x0+64, y range y0-2..y0+CBY
@@ -233,21 +233,22 @@ __global__ void __launch_bounds__(64 * CBY) k_conv_lds(Geo g, ZT t, const T* __r
   for (int i = 0; i < 3; ++i)
     for (int k = 0; k < 4; ++k) W[i][k] = (i < D) ? ldc(i, z0 - 2 + k) : T(0);
   // prefetched values for the next plane: window plane z+2, halo ring of plane z+1, V and u⁰ of plane z
-  T Wn[3] = {0, 0, 0}, Hn[NHT], Hc[NHT], Vn[3] = {0, 0, 0}, Un[3] = {0, 0, 0};
+  T Wn[3] = {0, 0, 0}, Hn[NHT], Hc[NHT], Vn[3] = {0, 0, 0};
   for (int k = 0; k < NHT; ++k) Hc[k] = ld_halo(k, z0);
-  T rpart[3] = {0, 0, 0}, Fzp[3] = {0, 0, 0}, Vp[3] = {0, 0, 0}, Up[3] = {0, 0, 0};
+  T rpart[3] = {0, 0, 0}, Fzp[3] = {0, 0, 0}, Vp[3] = {0, 0, 0};
 
   struct T3 { T v[3]; };
-  auto finish = [&](int cz, T3 rr, T3 uvals, T3 Vv, T3 U0v) {
+  // u⁰ of the corrector is read here rather than prefetched: the registers it would hold keep the
+  // kernel within 128 VGPRs (two workgroups per CU)
+  auto finish = [&](int cz, T3 rr, T3 uvals, T3 Vv) {
     const int64_t I = at(g, cx, cy, cz);
 #pragma unroll
     for (int i = 0; i < D; ++i) {
       T r = rr.v[i];
       if (acc.on) r += (T)acc.add[i];
       const T uv = uvals.v[i];
-      const T u0v = PRED ? uv : U0v.v[i];
+      const T u0v = PRED ? uv : u0[I + i * n];
       f[I + i * n] = u0v + dt * r - Vv.v[i];
-      if (PRED) u0[I + i * n] = uv;
     }
   };
   auto ld_cell = [&](const T* a, int i, int z) -> T { return (outxy && z <= N2 - 1) ? a[at(g, cx, cy, z) + i * n] : T(0); };
@@ -272,10 +273,7 @@ __global__ void __launch_bounds__(64 * CBY) k_conv_lds(Geo g, ZT t, const T* __r
       for (int i = 0; i < D; ++i) Wn[i] = D == 3 ? ldc(i, cz + 2) : T(0);
       for (int k = 0; k < NHT; ++k) Hn[k] = ld_halo(k, cz + 1);
     }
-    for (int i = 0; i < D; ++i) {
-      Vn[i] = ld_cell(V, i, cz);
-      if (!PRED) Un[i] = ld_cell(u0, i, cz);
-    }
+    for (int i = 0; i < D; ++i) Vn[i] = ld_cell(V, i, cz);
     __syncthreads();
     const int c[3] = {cx, cy, cz};
     const bool valid = ingrid && cx >= 1 && cy >= 1 && (D == 2 || cz >= 1);
@@ -322,8 +320,7 @@ __global__ void __launch_bounds__(64 * CBY) k_conv_lds(Geo g, ZT t, const T* __r
         rxy[i] = r;
       }
     if (D == 2) {
-      if (outxy) finish(cz, T3{{rxy[0], rxy[1], rxy[2]}}, T3{{W[0][2], W[1][2], W[2][2]}}, T3{{Vn[0], Vn[1], Vn[2]}},
-                        T3{{Un[0], Un[1], Un[2]}});
+      if (outxy) finish(cz, T3{{rxy[0], rxy[1], rxy[2]}}, T3{{W[0][2], W[1][2], W[2][2]}}, T3{{Vn[0], Vn[1], Vn[2]}});
     } else {
       if (cz > z0 && outxy) {  // plane cz-1 is complete: add its z fluxes
         const bool zin = cx >= 1 && cy >= 1 && cz - 1 >= 1 && cz - 1 <= N2 - 2;
@@ -332,10 +329,9 @@ __global__ void __launch_bounds__(64 * CBY) k_conv_lds(Geo g, ZT t, const T* __r
           rr[i] = rpart[i];
           if (zin) { rr[i] += Fzp[i]; rr[i] -= F[i][2]; }
         }
-        finish(cz - 1, T3{{rr[0], rr[1], rr[2]}}, T3{{W[0][1], W[1][1], W[2][1]}}, T3{{Vp[0], Vp[1], Vp[2]}},
-               T3{{Up[0], Up[1], Up[2]}});
+        finish(cz - 1, T3{{rr[0], rr[1], rr[2]}}, T3{{W[0][1], W[1][1], W[2][1]}}, T3{{Vp[0], Vp[1], Vp[2]}});
       }
-      for (int i = 0; i < 3; ++i) { rpart[i] = rxy[i]; Fzp[i] = F[i][2]; Vp[i] = Vn[i]; Up[i] = Un[i]; }
+      for (int i = 0; i < 3; ++i) { rpart[i] = rxy[i]; Fzp[i] = F[i][2]; Vp[i] = Vn[i]; }
     }
     // Φ on the top ghost layers (component D, last direction whose inner range holds the cell)
     if (valid && outxy && cz <= z1) {
@@ -355,17 +351,17 @@ __global__ void __launch_bounds__(64 * CBY) k_conv_lds(Geo g, ZT t, const T* __r
     }
   }
   if (D == 3 && zend == z1 && outxy) {  // top ghost plane: no z fluxes
-    finish(z1, T3{{rpart[0], rpart[1], rpart[2]}}, T3{{W[0][2], W[1][2], W[2][2]}}, T3{{Vp[0], Vp[1], Vp[2]}},
-           T3{{Up[0], Up[1], Up[2]}});
+    finish(z1, T3{{rpart[0], rpart[1], rpart[2]}}, T3{{W[0][2], W[1][2], W[2][2]}}, T3{{Vp[0], Vp[1], Vp[2]}});
   }
 }
 
 // BDIM! second half + scale_u!: u += μddn(μ₁,f) + V + μ₀·f on inside (src/Flow.jl:179,211-214).
-// PRED: u was scaled by 0 before (mom_step!, :157); CORR: the result is scaled by ½ (:209).
+// PRED: u is u⁰·0 (mom_step!, :157) with u⁰ read from uold, a distinct buffer; CORR: u is updated
+// in place and the result is scaled by ½ (:209).
 template <class T, bool PRED>
-__global__ void __launch_bounds__(256) k_bdim2(Geo g, ZT t, T* __restrict__ u, const T* __restrict__ f,
-                                               const T* __restrict__ V, const T* __restrict__ mu0,
-                                               const T* __restrict__ mu1) {
+__global__ void __launch_bounds__(256) k_bdim2(Geo g, ZT t, T* __restrict__ u, const T* __restrict__ uold,
+                                               const T* __restrict__ f, const T* __restrict__ V,
+                                               const T* __restrict__ mu0, const T* __restrict__ mu1) {
   int c0, c1, z0, z1;
   zt_cell(t, c0, c1, z0, z1);
   if (c0 > t.b.hi[0] || c1 > t.b.hi[1]) return;
@@ -378,7 +374,7 @@ __global__ void __launch_bounds__(256) k_bdim2(Geo g, ZT t, T* __restrict__ u, c
     T s = 0;
     for (int j = 0; j < D; ++j) s += mu1[I + (i + (int64_t)D * j) * n] * (f[k + g.s[j]] - f[k - g.s[j]]);
     const T inc = s / T(2) + V[k] + mu0[k] * f[k];
-    if (PRED) u[k] = u[k] * T(0) + inc;
+    if (PRED) u[k] = uold[k] * T(0) + inc;
     else u[k] = (u[k] + inc) * T(0.5);
   }
   }
@@ -436,6 +432,46 @@ __global__ void __launch_bounds__(256) k_bc(Geo g, T* __restrict__ a, int ncomp_
   }
   if (m[0] == c[0] && m[1] == c[1] && m[2] == c[2]) return;
   ai[at(g, c[0], c[1], c[2])] = ai[at(g, m[0], m[1], m[2])];
+}
+
+// exitBC! (src/core.jl:226-233) on the x-component of u: U = mean of u over the inflow plane
+// x = 2 (1-based, y and z interior); the exit ghost plane x = N gets the 1D upwind convection
+// u = u⁰ − U·Δt·(u⁰[I] − u⁰[I−eₓ]); then the exit plane's mean flux imbalance ∮u = mean − U is
+// subtracted. One workgroup; both plane sums accumulate in double; each thread corrects the cells
+// it wrote, so the block barrier inside the second sum orders every write before its correction.
+template <class T>
+__global__ void __launch_bounds__(1024) k_exit_bc(Geo g, T* u, const T* u0, T dt) {
+  __shared__ T bc_sh;
+  const int ny = g.N[1] - 2, nz = g.D > 2 ? g.N[2] - 2 : 1;
+  const int64_t cnt = (int64_t)ny * nz;
+  const int xe = g.N[0] - 1;
+  double s = 0;
+  for (int64_t t = threadIdx.x; t < cnt; t += blockDim.x) {
+    const int j = 1 + (int)(t % ny), k = g.D > 2 ? 1 + (int)(t / ny) : 0;
+    s += (double)u[at(g, 1, j, k)];
+  }
+  s = block_sum(s);
+  if (threadIdx.x == 0) bc_sh = (T)s / (T)cnt;
+  __syncthreads();
+  const T U = bc_sh;
+  const T c = U * dt;
+  double s2 = 0;
+  for (int64_t t = threadIdx.x; t < cnt; t += blockDim.x) {
+    const int j = 1 + (int)(t % ny), k = g.D > 2 ? 1 + (int)(t / ny) : 0;
+    const int64_t I = at(g, xe, j, k);
+    const T v = u0[I] - c * (u0[I] - u0[I - 1]);
+    u[I] = v;
+    s2 += (double)v;
+  }
+  s2 = block_sum(s2);
+  if (threadIdx.x == 0) bc_sh = (T)s2 / (T)cnt - U;
+  __syncthreads();
+  const T flux = bc_sh;
+  for (int64_t t = threadIdx.x; t < cnt; t += blockDim.x) {
+    const int j = 1 + (int)(t % ny), k = g.D > 2 ? 1 + (int)(t / ny) : 0;
+    const int64_t I = at(g, xe, j, k);
+    u[I] = u[I] - flux;
+  }
 }
 
 // =============================================================================================

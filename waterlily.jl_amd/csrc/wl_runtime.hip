@@ -684,6 +684,7 @@ struct FlowT : FlowBase {
     Ut(0.0, U0);
     for (int i = 0; i < D; ++i) fill(u + i * n, n, (T)U0[i]);
     mg.bc_vec(g, u, U0, d.exit_bc);
+    exit_bc(u, T(0));  // exitBC!(u,u,0): src/Flow.jl:141 calls it whatever the exitBC flag
     HIPCK(hipMemcpyAsync(u0, u, n * D * sizeof(T), hipMemcpyDeviceToDevice, st));
     fill(mu0, n * D, T(1));
     const double zero[3] = {0, 0, 0};
@@ -776,39 +777,42 @@ struct FlowT : FlowBase {
     const T nu = (T)desc.nu;
     const dim3 cblk(64, CBY, 1);
     {
-      ProfScope ps(&prof, st, "conv_bdim1", 0, (double)sizeof(T) * 4 * D * g.n);
+      ProfScope ps(&prof, st, pred ? "conv_pred" : "conv_corr", 0, (double)sizeof(T) * (pred ? 3 : 4) * D * g.n);
       if (desc.perdir_mask) {
         if (pred)
-          hipLaunchKernelGGL((k_conv_bdim1<T, S, true, DD>), zt_grid(ba), cblk, 0, st, g, ba, (const T*)u, u0, f,
+          hipLaunchKernelGGL((k_conv_bdim1<T, S, true, DD>), zt_grid(ba), cblk, 0, st, g, ba, (const T*)u, (const T*)u0, f,
                              sigma, (const T*)V, nu, dt, desc.perdir_mask, acc);
         else
-          hipLaunchKernelGGL((k_conv_bdim1<T, S, false, DD>), zt_grid(ba), cblk, 0, st, g, ba, (const T*)u, u0, f,
+          hipLaunchKernelGGL((k_conv_bdim1<T, S, false, DD>), zt_grid(ba), cblk, 0, st, g, ba, (const T*)u, (const T*)u0, f,
                              sigma, (const T*)V, nu, dt, desc.perdir_mask, acc);
       } else if (conv_prefetch()) {
         if (pred)
-          hipLaunchKernelGGL((k_conv_lds<T, S, true, DD, true>), zt_grid(ba), cblk, 0, st, g, ba, (const T*)u, u0, f,
+          hipLaunchKernelGGL((k_conv_lds<T, S, true, DD, true>), zt_grid(ba), cblk, 0, st, g, ba, (const T*)u, (const T*)u0, f,
                              sigma, (const T*)V, nu, dt, acc);
         else
-          hipLaunchKernelGGL((k_conv_lds<T, S, false, DD, true>), zt_grid(ba), cblk, 0, st, g, ba, (const T*)u, u0, f,
+          hipLaunchKernelGGL((k_conv_lds<T, S, false, DD, true>), zt_grid(ba), cblk, 0, st, g, ba, (const T*)u, (const T*)u0, f,
                              sigma, (const T*)V, nu, dt, acc);
       } else {
         if (pred)
-          hipLaunchKernelGGL((k_conv_lds<T, S, true, DD, false>), zt_grid(ba), cblk, 0, st, g, ba, (const T*)u, u0, f,
+          hipLaunchKernelGGL((k_conv_lds<T, S, true, DD, false>), zt_grid(ba), cblk, 0, st, g, ba, (const T*)u, (const T*)u0, f,
                              sigma, (const T*)V, nu, dt, acc);
         else
-          hipLaunchKernelGGL((k_conv_lds<T, S, false, DD, false>), zt_grid(ba), cblk, 0, st, g, ba, (const T*)u, u0, f,
+          hipLaunchKernelGGL((k_conv_lds<T, S, false, DD, false>), zt_grid(ba), cblk, 0, st, g, ba, (const T*)u, (const T*)u0, f,
                              sigma, (const T*)V, nu, dt, acc);
       }
       KCK();
     }
+    // u⁰ .= u (src/Flow.jl:157) as a buffer swap: the old u becomes u⁰; the new u is written on
+    // inside by the predictor's BDIM!, and on every ghost by the BC! (and exitBC!) that follow.
+    if (pred) std::swap(u, u0);
     ZT bi = make_zt(box_inside(g));
     ProfScope ps(&prof, st, "bdim2", 0, (double)sizeof(T) * (5 * D + D * D) * ncells_inside(g));
     if (pred)
-      hipLaunchKernelGGL((k_bdim2<T, true>), zt_grid(bi), kBlock, 0, st, g, bi, u, (const T*)f, (const T*)V,
-                         (const T*)mu0, (const T*)mu1);
+      hipLaunchKernelGGL((k_bdim2<T, true>), zt_grid(bi), kBlock, 0, st, g, bi, u, (const T*)u0, (const T*)f,
+                         (const T*)V, (const T*)mu0, (const T*)mu1);
     else
-      hipLaunchKernelGGL((k_bdim2<T, false>), zt_grid(bi), kBlock, 0, st, g, bi, u, (const T*)f, (const T*)V,
-                         (const T*)mu0, (const T*)mu1);
+      hipLaunchKernelGGL((k_bdim2<T, false>), zt_grid(bi), kBlock, 0, st, g, bi, u, (const T*)u0, (const T*)f,
+                         (const T*)V, (const T*)mu0, (const T*)mu1);
     KCK();
   }
   void momentum(bool pred, T dt, double t, double t1) {
@@ -828,6 +832,13 @@ struct FlowT : FlowBase {
     double Ub[3];
     Ut(t1, Ub);
     mg.bc_vec(g, u, Ub, desc.exit_bc);
+    if (pred && desc.exit_bc) exit_bc(u0, dt);  // src/Flow.jl:195, predictor only
+  }
+  // exitBC!(u, src, Δt) (src/core.jl:226-233)
+  void exit_bc(const T* src, T dt) {
+    ProfScope ps(&prof, st, "exit_bc", 0, 0.0);
+    hipLaunchKernelGGL(k_exit_bc<T>, dim3(1), dim3(1024), 0, st, g, u, src, dt);
+    KCK();
   }
 
   // mom_project! (src/Flow.jl:223-232)
@@ -859,7 +870,6 @@ struct FlowT : FlowBase {
   }
 
   int step(double dt_d, double t0, double t1, double* dt_next, int16_t* nit) override {
-    if (desc.exit_bc) throw WlErr(WL_EINVAL, "exitBC=true is not supported by the device step");
     r2hist.clear();
     const T dt = (T)dt_d;
     momentum(true, dt, t0, t1);
