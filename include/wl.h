@@ -111,6 +111,27 @@ int wl_flow_set_solver(wl_flow_t h, int mode, double tol, int itmx);
 int wl_mom_step(wl_flow_t h, double dt, double t0, double t1, double* dt_next, int16_t* n_iter);
 /* F = Σ p·nds(body) with the product in T and the sum in Float64; F has D entries. */
 int wl_pressure_force(wl_flow_t h, const wl_body* bodies, int nbodies, double t, double* F);
+/* pressure_force, viscous_force, pressure_moment and viscous_moment (src/Metrics.jl:126-197) of the
+ * flow's p and u about x0 (D entries): each cell's density is formed in T as the reference forms it
+ * (moments: loc − x₀ and the cross product in Float64, rounded to T) and the densities are summed
+ * in Float64.  Fp, Fv: D entries; Mp, Mv: 1 entry in 2D (the 2-vector cross product), 3 in 3D;
+ * any output may be NULL.  Replaces the Julia functions of the same names (Metrics.jl:146-197). */
+int wl_flow_metrics(wl_flow_t h, const wl_body* bodies, int nbodies, double t, const double* x0, double* Fp,
+                    double* Fv, double* Mp, double* Mv);
+
+/* MeanFlow (src/Metrics.jl:199-261): running averages P, U (and UU = u⊗u with uu_stats) on the
+ * flow's device.  The time vector t stays with the caller, which forms each update's weight
+ * ε = dt/(dt + time(meanflow) + eps(T)) (1 on the first update) as update!(meanflow, flow) does
+ * (:229-241).  Fields download in Julia layout: P (Ng), U (Ng×D), UU and τ = uu (Ng×D×D). */
+typedef struct wl_meanflow_s* wl_meanflow_t;
+enum wl_mfield { WL_MF_P = 0, WL_MF_U = 1, WL_MF_UU = 2, WL_MF_TAU = 3 };
+int wl_meanflow_create(wl_meanflow_t* out, wl_flow_t flow, int uu_stats); /* MeanFlow(flow; uu_stats), :208-216 */
+int wl_meanflow_destroy(wl_meanflow_t m);
+int wl_meanflow_update(wl_meanflow_t m, wl_flow_t flow, double eps);     /* update!(meanflow, flow), :229-241 */
+int wl_meanflow_reset(wl_meanflow_t m);                                  /* reset!(meanflow), :222-227 (fields) */
+int wl_meanflow_download(wl_meanflow_t m, int field, void* dst, size_t nbytes); /* P, U, UU; τ = uu(m), :243-252 */
+int wl_meanflow_copy_to_flow(wl_meanflow_t m, wl_flow_t flow);           /* copy!(flow, meanflow), :254-257 */
+
 /* Σr² residual history of the last mom_step! (both projections); returns the count written. */
 int wl_flow_last_residuals(wl_flow_t h, double* r2, int cap, int* count);
 

@@ -694,6 +694,96 @@ void pressure_force(Flow<T>& a, const wl_body* bs, int nb, T t, double* F) {
   }
 }
 
+// S(I,u) (src/Metrics.jl:37-44,140): rate of strain at the centre of cell I, in T; ∂(i,j,I,u)
+// is the face difference on the diagonal and the four-face average of central differences off it.
+template <class T>
+void strain(const Grid& g, const T* u, int64_t I, T S[3][3]) {
+  const int D = g.D;
+  const int64_t n = g.n;
+  T J[3][3] = {};
+  for (int i = 0; i < D; ++i)
+    for (int j = 0; j < D; ++j) {
+      const int64_t si = g.st(i), sj = g.st(j);
+      const T* ui = u + i * n;
+      J[i][j] = i == j ? ui[I + si] - ui[I]
+                       : (((ui[I + sj] + ui[I + sj + si]) - ui[I - sj]) - ui[I - sj + si]) / T(4);
+    }
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) S[i][j] = i < D && j < D ? (J[i][j] + J[j][i]) / T(2) : T(0);
+}
+
+// pressure_force, viscous_force, pressure_moment, viscous_moment (src/Metrics.jl:126-197) on the
+// fields p and u (either may be null): each cell's df value is formed as the reference forms it
+// (force products in T; moments with loc − x₀ and the cross product in Float64, stored to T) and
+// the df values are summed in Float64 in loop order.  Moments have 1 entry in 2D (the 2-vector
+// cross product), 3 in 3D.
+template <class T>
+void metrics(const Grid& g, const T* p, const T* u, T nu, const wl_body* bs, int nb, const double* x0,
+             double* Fp, double* Fv, double* Mp, double* Mv) {
+  const int D = g.D, nM = D == 2 ? 1 : 3;
+  const int64_t n = g.n;
+  // df arrays in T (zero outside inside), as the reference stores them before its Float64 sums
+  std::vector<T> dfp(p ? n * D : 0, T(0)), dfv(u ? n * D : 0, T(0)), dmp(p ? n * nM : 0, T(0)),
+      dmv(u ? n * nM : 0, T(0));
+  const T c = T(-2) * nu;
+  auto cross = [&](const double* a, const double* b, double* r) {
+    if (D == 2) r[0] = a[0] * b[1] - a[1] * b[0];
+    else { r[0] = a[1] * b[2] - a[2] * b[1]; r[1] = a[2] * b[0] - a[0] * b[2]; r[2] = a[0] * b[1] - a[1] * b[0]; }
+  };
+  loop(g, r_inside(g), [&](int64_t I, int i1, int i2, int i3) {
+    T x[3] = {T(i1) - T(1.5), T(i2) - T(1.5), T(i3) - T(1.5)};
+    T d, nv[3];
+    bodies_measure<T>(bs, nb, D, x, T(1), d, nv);
+    const T k = kern(std::clamp(d, T(-1), T(1)));
+    T nds[3] = {0, 0, 0};
+    double a[3] = {0, 0, 0}, nd[3] = {0, 0, 0}, cr[3];
+    for (int i = 0; i < D; ++i) { nds[i] = nv[i] * k; a[i] = (double)x[i] - x0[i]; nd[i] = (double)nds[i]; }
+    if (p) {
+      for (int i = 0; i < D; ++i) dfp[I + i * n] = p[I] * nds[i];
+      cross(a, nd, cr);
+      for (int q = 0; q < nM; ++q) dmp[I + q * n] = (T)((double)p[I] * cr[q]);
+    }
+    if (u) {
+      T S[3][3];
+      strain(g, u, I, S);
+      double b[3] = {0, 0, 0};
+      for (int i = 0; i < D; ++i) {
+        T w = (c * S[i][0]) * nds[0];
+        T sn = S[i][0] * nds[0];
+        for (int j = 1; j < D; ++j) { w = w + (c * S[i][j]) * nds[j]; sn = sn + S[i][j] * nds[j]; }
+        dfv[I + i * n] = w;
+        b[i] = (double)sn;
+      }
+      cross(a, b, cr);
+      for (int q = 0; q < nM; ++q) dmv[I + q * n] = (T)((double)c * cr[q]);
+    }
+  });
+  auto sum = [&](const std::vector<T>& v, int m, double* out) {
+    for (int q = 0; q < 3; ++q) out[q] = 0;
+    for (int q = 0; q < m && !v.empty(); ++q)
+      for (int64_t I = 0; I < n; ++I) out[q] += (double)v[I + q * n];
+  };
+  sum(dfp, D, Fp);
+  sum(dfv, D, Fv);
+  sum(dmp, nM, Mp);
+  sum(dmv, nM, Mv);
+}
+
+// update!(meanflow, flow) blend (src/Metrics.jl:229-241) with the weight ε formed by the caller
+template <class T>
+void meanflow_update(int64_t n, int D, T e, T* P, const T* p, T* U, const T* u, T* UU) {
+  const T om = T(1) - e;
+  for (int64_t I = 0; I < n; ++I) P[I] = e * p[I] + om * P[I];
+  for (int64_t I = 0; I < n * D; ++I) U[I] = e * u[I] + om * U[I];
+  if (UU)
+    for (int i = 0; i < D; ++i)
+      for (int j = 0; j < D; ++j)
+        for (int64_t I = 0; I < n; ++I) {
+          T& v = UU[I + (i + (int64_t)D * j) * n];
+          v = e * (u[I + i * n] * u[I + j * n]) + om * v;
+        }
+}
+
 }  // namespace wlo
 
 // =============================================================================================
@@ -867,6 +957,40 @@ int wlo_mom_step(wlo_flow* h, double dt, double t0, double t1, double* dt_next, 
 int wlo_pressure_force(wlo_flow* h, const wl_body* b, int nb, double t, double* F) {
   if (h->dtype == WL_F64) pressure_force<double>(*(Flow<double>*)h->impl, b, nb, t, F);
   else pressure_force<float>(*(Flow<float>*)h->impl, b, nb, (float)t, F);
+  return 0;
+}
+// metrics on raw column-major fields (p: Ng, u: Ng×D); null p or u skips those integrals
+int wlo_metrics(int dtype, int D, const int32_t* Ng, const void* p, const void* u, double nu, const wl_body* b, int nb,
+                const double* x0, double* Fp, double* Fv, double* Mp, double* Mv) {
+  Grid g;
+  int N[3] = {Ng[0], Ng[1], D > 2 ? Ng[2] : 1};
+  g.set(D, N);
+  if (dtype == WL_F64) metrics<double>(g, (const double*)p, (const double*)u, nu, b, nb, x0, Fp, Fv, Mp, Mv);
+  else metrics<float>(g, (const float*)p, (const float*)u, (float)nu, b, nb, x0, Fp, Fv, Mp, Mv);
+  return 0;
+}
+// S(I,u) at the 1-based cell I; S row-major D×D
+int wlo_strain(int dtype, int D, const int32_t* Ng, const void* u, const int32_t* I1, double* S) {
+  Grid g;
+  int N[3] = {Ng[0], Ng[1], D > 2 ? Ng[2] : 1};
+  g.set(D, N);
+  const int64_t I = g.at(I1[0], I1[1], D > 2 ? I1[2] : 1);
+  auto out = [&](auto* uu) {
+    using TT = std::remove_const_t<std::remove_pointer_t<decltype(uu)>>;
+    TT Sm[3][3];
+    strain<TT>(g, uu, I, Sm);
+    for (int i = 0; i < D; ++i)
+      for (int j = 0; j < D; ++j) S[i * D + j] = (double)Sm[i][j];
+  };
+  if (dtype == WL_F64) out((const double*)u); else out((const float*)u);
+  return 0;
+}
+int wlo_meanflow_update(int dtype, int64_t n, int D, double e, void* P, const void* p, void* U, const void* u,
+                        void* UU) {
+  if (dtype == WL_F64)
+    meanflow_update<double>(n, D, e, (double*)P, (const double*)p, (double*)U, (const double*)u, (double*)UU);
+  else
+    meanflow_update<float>(n, D, (float)e, (float*)P, (const float*)p, (float*)U, (const float*)u, (float*)UU);
   return 0;
 }
 int wlo_flow_last_residuals(wlo_flow* h, double* r2, int cap, int* count) {

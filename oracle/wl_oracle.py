@@ -81,6 +81,9 @@ def lib():
             "wlo_pois_op": (i, [vp, i, i, d]),
             "wlo_pois_L2": (d, [vp, i]),
             "wlo_pois_vcycle": (i, [vp, d]),
+            "wlo_metrics": (i, [i, i, P(C.c_int32), vp, vp, d, P(WlBody), i, P(d), P(d), P(d), P(d), P(d)]),
+            "wlo_strain": (i, [i, i, P(C.c_int32), vp, P(C.c_int32), P(d)]),
+            "wlo_meanflow_update": (i, [i, C.c_int64, i, d, vp, vp, vp, vp, vp]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -314,6 +317,44 @@ def exitbc(u, u0, dt):
     Ng = (C.c_int32 * 3)(*(list(u.shape[:-1]) + [1] * (3 - D)))
     lib().wlo_exitbc(dtype, D, Ng, u.ctypes.data, u0.ctypes.data, dt)
     return u
+
+
+def metrics(p, u, nu, bodies, x0=(0.0, 0.0, 0.0)):
+    """(pressure_force, viscous_force, pressure_moment, viscous_moment) of src/Metrics.jl:126-197
+    on numpy fields p (Ng) and u (Ng×D); either may be None.  Moments: 1 entry in 2D, 3 in 3D."""
+    f = p if p is not None else u
+    D = f.ndim if p is not None else f.ndim - 1
+    Ngl = list(f.shape[:D])
+    dtype = WL_F64 if f.dtype == np.float64 else WL_F32
+    Ng = (C.c_int32 * 3)(*(Ngl + [1] * (3 - D)))
+    outs = [(C.c_double * 3)() for _ in range(4)]
+    x = (C.c_double * 3)(*(list(x0) + [0.0] * (3 - len(x0))))
+    pp = None if p is None else np.asfortranarray(p)
+    uu = None if u is None else np.asfortranarray(u)
+    _ck(lib().wlo_metrics(dtype, D, Ng, None if pp is None else pp.ctypes.data, None if uu is None else uu.ctypes.data,
+                          nu, make_bodies(bodies), len(bodies), x, *outs), "metrics")
+    nM = 1 if D == 2 else 3
+    return (np.array(outs[0][:D]), np.array(outs[1][:D]), np.array(outs[2][:nM]), np.array(outs[3][:nM]))
+
+
+def strain(u, I):
+    """S(I,u) (src/Metrics.jl:140) at the 1-based cell I of a numpy velocity field."""
+    D = u.ndim - 1
+    uu = np.asfortranarray(u)
+    dtype = WL_F64 if u.dtype == np.float64 else WL_F32
+    Ng = (C.c_int32 * 3)(*(list(u.shape[:-1]) + [1] * (3 - D)))
+    Ic = (C.c_int32 * 3)(*(list(I) + [1] * (3 - D)))
+    S = (C.c_double * 9)()
+    _ck(lib().wlo_strain(dtype, D, Ng, uu.ctypes.data, Ic, S), "strain")
+    return np.array(S[:D * D]).reshape(D, D)
+
+
+def meanflow_update(e, P, p, U, u, UU=None):
+    """The update!(meanflow, flow) blend (src/Metrics.jl:234-240) in place, weight e."""
+    D = U.size // P.size
+    dtype = WL_F64 if p.dtype == np.float64 else WL_F32
+    _ck(lib().wlo_meanflow_update(dtype, p.size, D, e, P.ctypes.data, p.ctypes.data, U.ctypes.data, u.ctypes.data,
+                                  None if UU is None else UU.ctypes.data), "meanflow_update")
 
 
 class OSimulation:

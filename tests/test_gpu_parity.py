@@ -153,6 +153,75 @@ def test_exit_bc(wl, D):
     assert abs(float(np.mean(ex, dtype=np.float64)) - float(np.mean(inl, dtype=np.float64))) < 1e-5
 
 
+def test_uniform_rows_match_loaded(wl, monkeypatch):
+    """The uniform-coefficient rows (L formed, not loaded; V and μ₁ skipped where they vanish)
+    give the same bits as loading every coefficient (WL_LROWS=0 at build), over several steps."""
+    say("case: uniform rows vs loaded coefficients, 3D 64x32x32 sphere, 4 steps")
+    n, c = (64, 32, 32), (24, 16, 16)
+    monkeypatch.setenv("WL_LROWS", "1")
+    a = wl.Simulation(n, (1, 0, 0), 12, ν=12 / 400, body=wl.Sphere(c, 6), T=np.float32)
+    monkeypatch.setenv("WL_LROWS", "0")
+    b = wl.Simulation(n, (1, 0, 0), 12, ν=12 / 400, body=wl.Sphere(c, 6), T=np.float32)
+    for k in range(4):
+        wl.sim_step(a, remeasure=False)
+        wl.sim_step(b, remeasure=False)
+        for name in ("u", "p", "f"):
+            assert np.array_equal(a.flow.get(name), b.flow.get(name)), (k, name)
+    assert a.flow.Δt == b.flow.Δt and a.pois.n == b.pois.n
+
+
+def test_nonzero_V_rows(wl):
+    """A body velocity V set in a few x-rows (uploaded on both sides): one step against the
+    oracle, so the rows where V is not +0 take the loaded path."""
+    say("case: one step 3D 48x32x32 with V uploaded in a band")
+    sim, osim = oracle_pair(wl, (48, 32, 32), (1, 0, 0), 6, (16, 16, 16), 12 / 500)
+    V = osim.flow.get("V").copy()
+    rng = np.random.default_rng(3)
+    V[10:30, 12:20, 14:18, :] = rng.uniform(-0.2, 0.2, size=V[10:30, 12:20, 14:18, :].shape)
+    for name in ("mu0", "mu1"):
+        sim.flow.set(name, osim.flow.get(name))
+    sim.flow.set("V", V)
+    osim.flow.set("V", V)
+    sim.pois.update()
+    wl.mom_step(sim.flow)
+    osim.step(remeasure=False)
+    for name in ("u", "p"):
+        a, b = sim.flow.get(name), osim.flow.get(name)
+        say(f"  {name}: rel-L2 {rel(a, b):.3e}  bit-equal {np.mean(a == b):.6f}")
+        assert rel(a, b) < 1e-6, name
+    assert sim.pois.n == osim.n
+
+
+def test_mixed_precision_solver(wl):
+    """WL_SOLVER_MIXED: FP32 V-cycles inside an FP64 defect-correction loop.  A 3D sphere
+    (64x32x32, Re 400) for 20 steps against the Float64 oracle: every projection ends below the
+    tolerance on the FP64 residual, u and p stay within 1e-4 (rel-L2) of the Float64 oracle and the
+    pressure force within 1e-3, and the plain Float32 path is reported beside it."""
+    say("case: mixed-precision solver, 3D 64x32x32 sphere, 20 steps vs Float64 oracle")
+    n, c, R, nu = (64, 32, 32), (20, 16, 16), 6, 12 / 400
+    body = [{"kind": 1, "center": c, "radius": R}]
+    mix = wl.Simulation(n, (1, 0, 0), 2 * R, ν=nu, body=wl.Sphere(c, R), T=np.float32)
+    mix.pois.set_solver(mode=wl.WL_SOLVER_MIXED)
+    pl = wl.Simulation(n, (1, 0, 0), 2 * R, ν=nu, body=wl.Sphere(c, R), T=np.float32)
+    o64 = O.OSimulation(n, (1, 0, 0), 2 * R, nu=nu, dtype=O.WL_F64, bodies=body)
+    worst_m = worst_p = 0.0
+    for k in range(20):
+        wl.sim_step(mix, remeasure=False)
+        wl.sim_step(pl, remeasure=False)
+        o64.step(remeasure=False)
+        r2 = mix.flow.residuals()
+        assert r2[-1] < 1e-4 or mix.pois.n[-1] == 32, (k, r2)
+        Fo = o64.pressure_force()
+        sc = max(np.max(np.abs(Fo)), 1e-12)
+        worst_m = max(worst_m, float(np.max(np.abs(wl.pressure_force(mix) - Fo)) / sc))
+        worst_p = max(worst_p, float(np.max(np.abs(wl.pressure_force(pl) - Fo)) / sc))
+    um, pm = rel(mix.flow.u, o64.flow.get("u")), rel(mix.flow.p, o64.flow.get("p"))
+    up, pp = rel(pl.flow.u, o64.flow.get("u")), rel(pl.flow.p, o64.flow.get("p"))
+    say(f"  vs F64 oracle: mixed u {um:.3e} p {pm:.3e} force {worst_m:.3e} n {mix.pois.n[-4:]}; "
+        f"plain u {up:.3e} p {pp:.3e} force {worst_p:.3e} n {pl.pois.n[-4:]}")
+    assert um < 1e-4 and pm < 1e-4 and worst_m < 1e-3
+
+
 def test_impulsive_box(wl):
     say("case: impulsive box")
     U = (2 / 3, -1 / 3)

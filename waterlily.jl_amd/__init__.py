@@ -4,9 +4,10 @@ host-side mirror of the reference's Simulation / Flow / MultiLevelPoisson API.
 The package directory is named `waterlily.jl_amd`; load it with `load_package()` from
 `__graft_entry__` (it registers the module as `waterlily_amd`).
 """
-from ._abi import LIB_PATH, WaterLilyError, load as load_library  # noqa: F401
+from ._abi import LIB_PATH, WL_SOLVER_MIXED, WL_SOLVER_PLAIN, WaterLilyError, load as load_library  # noqa: F401
 from .api import (  # noqa: F401
-    AbstractBody, BodyUnion, Circle, Ellipsoid, Flow, FlowPoisson, L2, MultiLevelPoisson, NoBody, Poisson,
-    Simulation, Sphere, cds, coarsen_mask, measure, measure_flow, mom_step, mult, pressure_force, quick, sim_step,
-    sim_time, solver, time, update, vanLeer,
+    AbstractBody, BodyUnion, Circle, Ellipsoid, Flow, FlowPoisson, L2, MeanFlow, MultiLevelPoisson, NoBody, Poisson,
+    Simulation, Sphere, cds, coarsen_mask, copy, measure, measure_flow, meanflow_update, mom_step, mult,
+    pressure_force, pressure_moment, quick, reset, sim_step, sim_time, solver, time, total_force, total_moment,
+    update, uu, vanLeer, viscous_force, viscous_moment,
 )

@@ -16,6 +16,7 @@ PFIELDS = {"x": 0, "L": 1, "D": 2, "iD": 3, "eps": 4, "r": 5, "z": 6}
 WL_SOLVER_PLAIN, WL_SOLVER_MIXED = 0, 1
 WL_BODY_SPHERE, WL_BODY_ELLIPSOID = 1, 2
 WL_ENAN = -3
+MFIELDS = {"P": 0, "U": 1, "UU": 2, "tau": 3}
 
 
 class WlBody(C.Structure):
@@ -56,6 +57,13 @@ SIGNATURES = {
     "wl_flow_set_solver": (_i, [_vp, _i, _d, _i]),
     "wl_mom_step": (_i, [_vp, _d, _d, _d, _P(_d), _P(C.c_int16)]),
     "wl_pressure_force": (_i, [_vp, _P(WlBody), _i, _d, _P(_d)]),
+    "wl_flow_metrics": (_i, [_vp, _P(WlBody), _i, _d, _P(_d), _P(_d), _P(_d), _P(_d), _P(_d)]),
+    "wl_meanflow_create": (_i, [_P(_vp), _vp, _i]),
+    "wl_meanflow_destroy": (_i, [_vp]),
+    "wl_meanflow_update": (_i, [_vp, _vp, _d]),
+    "wl_meanflow_reset": (_i, [_vp]),
+    "wl_meanflow_download": (_i, [_vp, _i, _vp, _sz]),
+    "wl_meanflow_copy_to_flow": (_i, [_vp, _vp]),
     "wl_flow_last_residuals": (_i, [_vp, _P(_d), _i, _P(_i)]),
     "wl_flow_profile": (_i, [_vp, _i]),
     "wl_flow_profile_read": (_i, [_vp, _P(WlKernelStat), _i, _P(_i)]),

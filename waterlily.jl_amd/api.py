@@ -230,7 +230,10 @@ class FlowPoisson:
 
 
 def time(flow):
-    """time(a::AbstractFlow) = sum(Δt[1:end-1]) (src/Flow.jl:174)."""
+    """time(a::AbstractFlow) = sum(Δt[1:end-1]) (src/Flow.jl:174); time(meanflow) =
+    t[end] − t[1] (src/Metrics.jl:220)."""
+    if isinstance(flow, MeanFlow):
+        return flow.T(flow.t[-1] - flow.t[0])
     return float(np.sum(np.asarray(flow.Δt[:-1], dtype=flow.T), dtype=flow.T)) if len(flow.Δt) > 1 else 0.0
 
 
@@ -346,8 +349,11 @@ def mult(pois, x):
     return z
 
 
-def update(pois):
-    """update!(p) (src/Poisson.jl:47, src/MultiLevelPoisson.jl:79-86)."""
+def update(pois, flow=None):
+    """update!(p) (src/Poisson.jl:47, src/MultiLevelPoisson.jl:79-86); update!(meanflow, flow)
+    (src/Metrics.jl:229-241) when given a MeanFlow."""
+    if isinstance(pois, MeanFlow):
+        return meanflow_update(pois, flow)
     if isinstance(pois, FlowPoisson):
         return pois.update()
     A.check(A.load().wl_pois_update(pois.h))
@@ -434,6 +440,110 @@ def pressure_force(sim):
         return np.zeros(sim.flow.D)
     A.check(A.load().wl_pressure_force(sim.flow.h, arr, nb, time(sim.flow), F))
     return np.array([F[k] for k in range(sim.flow.D)])
+
+
+def _metrics(sim, x0):
+    D = sim.flow.D
+    arr, nb = _body_array(sim.body)
+    outs = [(C.c_double * 3)() for _ in range(4)]
+    if nb == 0:
+        return [np.zeros(3) for _ in range(4)]
+    x = (C.c_double * 3)(*([float(v) for v in x0] + [0.0] * (3 - len(x0))))
+    A.check(A.load().wl_flow_metrics(sim.flow.h, arr, nb, time(sim.flow), x, *outs))
+    return [np.array(o[:]) for o in outs]
+
+
+def _moment(m, D):
+    # the 2-vector cross product is a scalar broadcast over df[I,:] (src/Metrics.jl:174,188)
+    return np.array([m[0], m[0]]) if D == 2 else np.array(m[:3])
+
+
+def viscous_force(sim):
+    """viscous_force(sim) (src/Metrics.jl:146-155): Σ -2ν·S(I,u)·nds(body) in Float64."""
+    return _metrics(sim, (0.0, 0.0, 0.0))[1][:sim.flow.D]
+
+
+def total_force(sim):
+    """total_force(sim) = pressure_force(sim) .+ viscous_force(sim) (src/Metrics.jl:157-162)."""
+    o = _metrics(sim, (0.0, 0.0, 0.0))
+    return o[0][:sim.flow.D] + o[1][:sim.flow.D]
+
+
+def pressure_moment(x0, sim):
+    """pressure_moment(x₀, sim) (src/Metrics.jl:164-175): Σ p·(x−x₀)×nds(body) in Float64."""
+    return _moment(_metrics(sim, x0)[2], sim.flow.D)
+
+
+def viscous_moment(x0, sim):
+    """viscous_moment(x₀, sim) (src/Metrics.jl:177-189): Σ -2ν·(x−x₀)×(S·nds) in Float64."""
+    return _moment(_metrics(sim, x0)[3], sim.flow.D)
+
+
+def total_moment(x0, sim):
+    """total_moment(x₀, sim) (src/Metrics.jl:191-196)."""
+    o = _metrics(sim, x0)
+    return _moment(o[2], sim.flow.D) + _moment(o[3], sim.flow.D)
+
+
+class MeanFlow:
+    """MeanFlow(flow; t_init=time(flow), uu_stats=false) (src/Metrics.jl:199-216): temporal
+    averages of p, u (and u⊗u) on the flow's device; the time vector `t` is kept here."""
+
+    def __init__(self, flow, t_init=None, uu_stats=False):
+        self.T, self.D = flow.T, flow.D
+        self.Ng = tuple(flow._shape("p"))
+        self.uu_stats = bool(uu_stats)
+        self.t = [self.T(time(flow) if t_init is None else t_init)]
+        h = C.c_void_p()
+        A.check(A.load().wl_meanflow_create(C.byref(h), flow.h, int(self.uu_stats)))
+        self.h = h
+
+    def _get(self, name):
+        shp = {"P": self.Ng, "U": self.Ng + (self.D,), "UU": self.Ng + (self.D, self.D),
+               "tau": self.Ng + (self.D, self.D)}[name]
+        a = np.zeros(shp, dtype=self.T, order="F")
+        A.check(A.load().wl_meanflow_download(self.h, A.MFIELDS[name], a.ctypes.data, a.nbytes))
+        return a
+
+    P = property(lambda s: s._get("P"))
+    U = property(lambda s: s._get("U"))
+    UU = property(lambda s: s._get("UU") if s.uu_stats else None)
+
+    def __del__(self):
+        try:
+            if getattr(self, "h", None):
+                A.load().wl_meanflow_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+
+def meanflow_update(meanflow, flow):
+    """update!(meanflow, flow) (src/Metrics.jl:229-241): ε = dt/(dt + time(meanflow) + eps(T)),
+    1 on the first update, formed in T; the blend runs on the device."""
+    T = meanflow.T
+    dt = T(time(flow)) - meanflow.t[-1]
+    e = dt / (dt + time(meanflow) + T(np.finfo(T).eps))
+    if len(meanflow.t) == 1:
+        e = T(1)
+    A.check(A.load().wl_meanflow_update(meanflow.h, flow.h, float(e)))
+    meanflow.t.append(T(meanflow.t[-1] + dt))
+
+
+def reset(meanflow, t_init=0.0):
+    """reset!(meanflow; t_init) (src/Metrics.jl:222-227)."""
+    A.check(A.load().wl_meanflow_reset(meanflow.h))
+    meanflow.t = [meanflow.T(t_init)]
+
+
+def uu(meanflow):
+    """uu(meanflow) (src/Metrics.jl:246-250): τ = UU − U⊗U."""
+    return meanflow._get("tau")
+
+
+def copy(flow, meanflow):
+    """copy!(flow, meanflow) (src/Metrics.jl:254-257): u .= U; p .= P."""
+    A.check(A.load().wl_meanflow_copy_to_flow(meanflow.h, flow.h))
 
 
 def L2(a):

@@ -11,6 +11,63 @@
 
 namespace wl {
 
+// Face coefficients L of a level, with optional uniform-row flags. Away from bodies and walls L is
+// a per-level constant: where rows[y + N1·z] is set, every coefficient the stencils of a cell in
+// the x-row (y, z) read is c[i], except the wall faces x = 1 and x = N0-1 of component 0, which
+// are 0 (the flags are built by comparing bit patterns, k_lrow_bits/k_lrow_flags). Such rows form
+// the coefficients instead of loading them: same values, same arithmetic, no L traffic.
+template <class T>
+struct LRows {
+  const T* L;
+  const uint8_t* rows;  // nullptr: every row loads L
+  T c[3];
+};
+template <class T>
+__host__ __device__ inline LRows<T> lrows_plain(const T* L) { return LRows<T>{L, nullptr, {T(0), T(0), T(0)}}; }
+// the coefficients seen from one cell (x, y, z)
+template <class T>
+struct LC {
+  const T* L;
+  int64_t n;
+  int x, N0m1;
+  bool u;
+  T c0, c1, c2;
+  __device__ __forceinline__ T cu(int i) const { return i == 0 ? c0 : (i == 1 ? c1 : c2); }
+  // L[I, i]
+  __device__ __forceinline__ T lo(int64_t I, int i) const {
+    return u ? ((i == 0 && x == 1) ? T(0) : cu(i)) : L[I + i * n];
+  }
+  // L[I + e_i, i]
+  __device__ __forceinline__ T hi(int64_t I, int i, int64_t si) const {
+    return u ? ((i == 0 && x + 1 == N0m1) ? T(0) : cu(i)) : L[I + si + i * n];
+  }
+};
+template <class T>
+__device__ __forceinline__ LC<T> lcell(const Geo& g, const LRows<T>& a, int x, int y, int z) {
+  LC<T> c;
+  c.L = a.L;
+  c.n = g.n;
+  c.x = x;
+  c.N0m1 = g.N[0] - 1;
+  c.u = a.rows != nullptr && a.rows[y + (int64_t)g.N[1] * z] != 0;
+  c.c0 = a.c[0]; c.c1 = a.c[1]; c.c2 = a.c[2];
+  return c;
+}
+// plain pointer view (every coefficient loaded)
+template <class T>
+struct LPtr {
+  const T* L;
+  int64_t n;
+  __device__ __forceinline__ T lo(int64_t I, int i) const { return L[I + i * n]; }
+  __device__ __forceinline__ T hi(int64_t I, int i, int64_t si) const { return L[I + si + i * n]; }
+};
+
+template <class T>
+__device__ __forceinline__ bool same_bits(T a, T b) {
+  if constexpr (sizeof(T) == 4) return __float_as_uint(a) == __float_as_uint(b);
+  else return __double_as_longlong(a) == __double_as_longlong(b);
+}
+
 // =============================================================================================
 // Momentum: conv_diff! + accelerate! + BDIM! first half  (src/Flow.jl:38-62,69-73,176-178)
 // =============================================================================================
@@ -73,8 +130,8 @@ constexpr int CBY = 8;
 template <class T, int S, bool PRED, int D>
 __global__ void __launch_bounds__(64 * CBY) k_conv_bdim1(Geo g, ZT t, const T* __restrict__ u, const T* __restrict__ u0,
                                                          T* __restrict__ f, T* __restrict__ sigma,
-                                                         const T* __restrict__ V, T nu, T dt, int perdir,
-                                                         Accel acc) {
+                                                         const T* __restrict__ V, const uint8_t* __restrict__ vrows,
+                                                         T nu, T dt, int perdir, Accel acc) {
   __shared__ T sFy[3][CBY][64];
   int cx, cy, z0, z1;
   zt_cell(t, cx, cy, z0, z1);
@@ -92,7 +149,8 @@ __global__ void __launch_bounds__(64 * CBY) k_conv_bdim1(Geo g, ZT t, const T* _
       if (acc.on) r += (T)acc.add[i];
       const T uv = u[I + i * n];
       const T u0v = PRED ? uv : u0[I + i * n];
-      f[I + i * n] = u0v + dt * r - V[I + i * n];
+      const T vv = vrows[cy + (int64_t)g.N[1] * cz] ? V[I + i * n] : T(0);  // V is +0 in unflagged rows
+      f[I + i * n] = u0v + dt * r - vv;
     }
   };
 
@@ -190,7 +248,8 @@ __global__ void __launch_bounds__(64 * CBY) k_conv_bdim1(Geo g, ZT t, const T* _
 template <class T, int S, bool PRED, int D, bool PF>
 __global__ void __launch_bounds__(64 * CBY, sizeof(T) == 4 ? 4 : 2) k_conv_lds(Geo g, ZT t, const T* __restrict__ u, const T* __restrict__ u0,
                                                        T* __restrict__ f, T* __restrict__ sigma,
-                                                       const T* __restrict__ V, T nu, T dt, Accel acc) {
+                                                       const T* __restrict__ V, const uint8_t* __restrict__ vrows,
+                                                       T nu, T dt, Accel acc) {
   constexpr int RX = 64 + 3, RY = CBY + 3;  // staged x range x0-2..x0+64, y range y0-2..y0+CBY
   constexpr int NH = 3 * RY + 3 * 64;       // halo cells per component and plane
   constexpr int NHT = (D * NH + 64 * CBY - 1) / (64 * CBY);  // halo loads per thread
@@ -273,7 +332,10 @@ __global__ void __launch_bounds__(64 * CBY, sizeof(T) == 4 ? 4 : 2) k_conv_lds(G
       for (int i = 0; i < D; ++i) Wn[i] = D == 3 ? ldc(i, cz + 2) : T(0);
       for (int k = 0; k < NHT; ++k) Hn[k] = ld_halo(k, cz + 1);
     }
-    for (int i = 0; i < D; ++i) Vn[i] = ld_cell(V, i, cz);
+    {
+      const bool vnz = outxy && cz <= N2 - 1 && vrows[cy + (int64_t)N1 * cz];  // V is +0 in unflagged rows
+      for (int i = 0; i < D; ++i) Vn[i] = vnz ? ld_cell(V, i, cz) : T(0);
+    }
     __syncthreads();
     const int c[3] = {cx, cy, cz};
     const bool valid = ingrid && cx >= 1 && cy >= 1 && (D == 2 || cz >= 1);
@@ -358,10 +420,15 @@ __global__ void __launch_bounds__(64 * CBY, sizeof(T) == 4 ? 4 : 2) k_conv_lds(G
 // BDIM! second half + scale_u!: u += μddn(μ₁,f) + V + μ₀·f on inside (src/Flow.jl:179,211-214).
 // PRED: u is u⁰·0 (mom_step!, :157) with u⁰ read from uold, a distinct buffer; CORR: u is updated
 // in place and the result is scaled by ½ (:209).
+// rows[y + N1·z] != 0 marks the x-rows holding a nonzero μ₁. Elsewhere every μ₁ term is
+// ±0·(finite difference), so the sum stays +0 and neither μ₁ nor the f-stencil is read there
+// (identical results for finite f). A wave covers 64 cells of one row, so the branch is uniform.
+// μ₀ is read through the finest level's LRows (μ₀ is its L) and V only in rows where it is not +0.
 template <class T, bool PRED>
 __global__ void __launch_bounds__(256) k_bdim2(Geo g, ZT t, T* __restrict__ u, const T* __restrict__ uold,
                                                const T* __restrict__ f, const T* __restrict__ V,
-                                               const T* __restrict__ mu0, const T* __restrict__ mu1) {
+                                               LRows<T> mu0, const T* __restrict__ mu1,
+                                               const uint8_t* __restrict__ rows, const uint8_t* __restrict__ vrows) {
   int c0, c1, z0, z1;
   zt_cell(t, c0, c1, z0, z1);
   if (c0 > t.b.hi[0] || c1 > t.b.hi[1]) return;
@@ -369,15 +436,72 @@ __global__ void __launch_bounds__(256) k_bdim2(Geo g, ZT t, T* __restrict__ u, c
   const int D = g.D;
   const int64_t I = at(g, c0, c1, c2);
   const int64_t n = g.n;
+  const bool band = rows[c1 + (int64_t)g.N[1] * c2] != 0;
+  const bool vnz = vrows[c1 + (int64_t)g.N[1] * c2] != 0;
+  const LC<T> m0 = lcell(g, mu0, c0, c1, c2);
   for (int i = 0; i < D; ++i) {
     const int64_t k = I + i * n;
     T s = 0;
-    for (int j = 0; j < D; ++j) s += mu1[I + (i + (int64_t)D * j) * n] * (f[k + g.s[j]] - f[k - g.s[j]]);
-    const T inc = s / T(2) + V[k] + mu0[k] * f[k];
+    if (band)
+      for (int j = 0; j < D; ++j) s += mu1[I + (i + (int64_t)D * j) * n] * (f[k + g.s[j]] - f[k - g.s[j]]);
+    const T inc = s / T(2) + (vnz ? V[k] : T(0)) + m0.lo(I, i) * f[k];
     if (PRED) u[k] = uold[k] * T(0) + inc;
     else u[k] = (u[k] + inc) * T(0.5);
   }
   }
+}
+
+// Uniform-row flags of a level's face coefficients (LRows). Pass 1, one wavefront per x-row
+// (y, z): bit 0 = component 0 over x = 1..N0-1 is c0, except 0 at x = 1 and x = N0-1; bit 1 =
+// component 1 over x = 1..N0-2 is c1; bit 2 = component 2 likewise (compared bit for bit).
+template <class T>
+__global__ void __launch_bounds__(256) k_lrow_bits(Geo g, const T* __restrict__ L, T c0, T c1, T c2,
+                                                   uint8_t* __restrict__ bits) {
+  const int64_t row = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (row >= (int64_t)g.N[1] * g.N[2]) return;  // uniform over the wavefront
+  const int y = (int)(row % g.N[1]), z = (int)(row / g.N[1]);
+  const int64_t base = at(g, 0, y, z);
+  bool ok0 = true, ok1 = true, ok2 = true;
+  for (int x = 1 + lane; x <= g.N[0] - 1; x += 64) {
+    ok0 &= same_bits(L[base + x], (x == 1 || x == g.N[0] - 1) ? T(0) : c0);
+    if (x <= g.N[0] - 2) {
+      ok1 &= same_bits(L[base + x + g.n], c1);
+      if (g.D == 3) ok2 &= same_bits(L[base + x + 2 * g.n], c2);
+    }
+  }
+  const bool a0 = __all(ok0), a1 = __all(ok1), a2 = __all(ok2);
+  if (lane == 0) bits[row] = (uint8_t)(a0 | (a1 << 1) | (a2 << 2));
+}
+// Pass 2: a row is uniform when it is an inside row, its own bits are all set, and the rows of
+// its upper y and z neighbours (whose faces the stencils read) have components 1 and 2 uniform.
+__global__ void __launch_bounds__(256) k_lrow_flags(Geo g, const uint8_t* __restrict__ bits,
+                                                    uint8_t* __restrict__ rows) {
+  const int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (row >= (int64_t)g.N[1] * g.N[2]) return;
+  const int y = (int)(row % g.N[1]), z = (int)(row / g.N[1]);
+  bool f = y >= 1 && y <= g.N[1] - 2 && (g.D == 2 || (z >= 1 && z <= g.N[2] - 2));
+  if (f) {
+    const int b = bits[row];
+    f = (b & 1) && (b & 2) && (bits[row + 1] & 2);
+    if (g.D == 3) f = f && (b & 4) && (bits[row + g.N[1]] & 4);
+  }
+  rows[row] = f ? 1 : 0;
+}
+
+// rows[y + N1·z] = 1 where some μ₁ component of the x-row is nonzero, vrows[y + N1·z] = 1 where
+// some V component is not +0 (both zeroed beforehand; concurrent writers store the same value)
+template <class T>
+__global__ void __launch_bounds__(256) k_mu1_rows(Geo g, Box b, const T* __restrict__ mu1, const T* __restrict__ V,
+                                                  uint8_t* __restrict__ rows, uint8_t* __restrict__ vrows) {
+  int c0, c1, c2;
+  if (!box_cell(b, c0, c1, c2)) return;
+  const int64_t I = at(g, c0, c1, c2);
+  bool nz = false, vnz = false;
+  for (int q = 0; q < g.D * g.D; ++q) nz |= mu1[I + q * g.n] != T(0);
+  for (int q = 0; q < g.D; ++q) vnz |= !same_bits(V[I + q * g.n], T(0));
+  if (nz) rows[c1 + (int64_t)g.N[1] * c2] = 1;
+  if (vnz) vrows[c1 + (int64_t)g.N[1] * c2] = 1;
 }
 
 // =============================================================================================
@@ -488,42 +612,55 @@ struct LevelP {
   T* z;
 };
 
+template <class T, class LA>
+__device__ __forceinline__ T diag_at(const Geo& g, const LA& L, int64_t I) {
+  T s = 0;
+  for (int i = 0; i < g.D; ++i) s -= (L.lo(I, i) + L.hi(I, i, g.s[i]));
+  return s;
+}
 template <class T>
 __device__ __forceinline__ T diag_at(const Geo& g, const T* __restrict__ L, int64_t I) {
-  T s = 0;
-  for (int i = 0; i < g.D; ++i) s -= (L[I + i * g.n] + L[I + g.s[i] + i * g.n]);
-  return s;
+  return diag_at<T>(g, LPtr<T>{L, g.n}, I);
 }
 template <class T>
 __device__ __forceinline__ T inv_diag(T Dg) { return Dg == T(0) ? Dg : T(1) / Dg; }
 
 // mult(I,L,D,x) with x read through a functor (so fused kernels can synthesize x)
-template <class T, class X>
-__device__ __forceinline__ T mult_at(const Geo& g, const T* __restrict__ L, int64_t I, T Dg, X xv) {
+template <class T, class LA, class X>
+__device__ __forceinline__ T mult_at(const Geo& g, const LA& L, int64_t I, T Dg, X xv) {
   T s = xv(I) * Dg;
   for (int i = 0; i < g.D; ++i) {
     const int64_t si = g.s[i];
-    s += (xv(I - si) * L[I + i * g.n] + xv(I + si) * L[I + si + i * g.n]);
+    s += (xv(I - si) * L.lo(I, i) + xv(I + si) * L.hi(I, i, si));
   }
   return s;
+}
+template <class T, class X>
+__device__ __forceinline__ T mult_at(const Geo& g, const T* __restrict__ L, int64_t I, T Dg, X xv) {
+  return mult_at<T>(g, LPtr<T>{L, g.n}, I, Dg, xv);
 }
 
 // ---- per-cell bodies shared by the tiled kernels and the single-workgroup coarse tail ----
 // increment! at cell I: r -= ω·Aϵ; x += ω·ϵ (src/Poisson.jl:100-104); returns the new r
-template <class T>
-__device__ __forceinline__ T increment_cell(const Geo& g, const T* __restrict__ L, T* __restrict__ r,
+template <class T, class LA>
+__device__ __forceinline__ T increment_cell(const Geo& g, const LA& L, T* __restrict__ r,
                                             T* __restrict__ x, const T* __restrict__ e, T w, int64_t I) {
-  const T Dg = diag_at(g, L, I);
+  const T Dg = diag_at<T>(g, L, I);
   auto xv = [&](int64_t J) { return e[J]; };
-  const T rn = r[I] - w * mult_at(g, L, I, Dg, xv);
+  const T rn = r[I] - w * mult_at<T>(g, L, I, Dg, xv);
   r[I] = rn;
   x[I] = x[I] + w * e[I];
   return rn;
 }
+template <class T>
+__device__ __forceinline__ T increment_cell(const Geo& g, const T* __restrict__ L, T* __restrict__ r,
+                                            T* __restrict__ x, const T* __restrict__ e, T w, int64_t I) {
+  return increment_cell<T>(g, LPtr<T>{L, g.n}, r, x, e, w, I);
+}
 // one GS-RB update at half-range cell c (src/Poisson.jl:116-128): the last index is doubled and
 // shifted so colour `it` updates cells whose 1-based index sum is even for odd it
 template <class T>
-__device__ __forceinline__ void gs_cell(const Geo& g, const T* __restrict__ L, const T* __restrict__ r,
+__device__ __forceinline__ void gs_cell(const Geo& g, const LRows<T>& LR, const T* __restrict__ r,
                                         T* __restrict__ e, int it, int c0, int c1, int c2) {
   int q[3] = {c0, c1, c2};
   const int Ld = g.D - 1;
@@ -532,20 +669,21 @@ __device__ __forceinline__ void gs_cell(const Geo& g, const T* __restrict__ L, c
   const int kv = q[Ld] + 1;                        // 1-based half index
   q[Ld] = 2 * kv - 1 - (front + it) % 2 - 1;       // back to 0-based
   const int64_t I = at(g, q[0], q[1], q[2]);
-  const T Dg = diag_at(g, L, I);
+  const LC<T> L = lcell(g, LR, q[0], q[1], q[2]);
+  const T Dg = diag_at<T>(g, L, I);
   T s = r[I];
   for (int i = 0; i < g.D; ++i) {
     const int64_t si = g.s[i];
-    s -= (e[I - si] * L[I + i * g.n] + e[I + si] * L[I + si + i * g.n]);
+    s -= (e[I - si] * L.lo(I, i) + e[I + si] * L.hi(I, i, si));
   }
   e[I] = s * inv_diag(Dg);
 }
 // Jacobi!'s increment (ω=1) of the 2^D children of coarse cell c, in x-fastest order, their new
 // residuals summed into the coarse r (restrict!), coarse x = 0 (fill!) and coarse ϵ = r·iD
 template <class T>
-__device__ __forceinline__ void inc_restrict_cell(const Geo& gf, const Geo& gc, int cm, const T* __restrict__ Lf,
+__device__ __forceinline__ void inc_restrict_cell(const Geo& gf, const Geo& gc, int cm, const LRows<T>& Lf,
                                                   T* __restrict__ rf, T* __restrict__ xf, const T* __restrict__ ef,
-                                                  const T* __restrict__ Lc, T* __restrict__ rc, T* __restrict__ xc,
+                                                  const LRows<T>& Lc, T* __restrict__ rc, T* __restrict__ xc,
                                                   T* __restrict__ ec, const int* c) {
   int lo[3], hi[3];
   for (int d = 0; d < 3; ++d) {
@@ -555,11 +693,12 @@ __device__ __forceinline__ void inc_restrict_cell(const Geo& gf, const Geo& gc, 
   T s = 0;
   for (int k = lo[2]; k <= hi[2]; ++k)
     for (int j = lo[1]; j <= hi[1]; ++j)
-      for (int i = lo[0]; i <= hi[0]; ++i) s += increment_cell(gf, Lf, rf, xf, ef, T(1), at(gf, i, j, k));
+      for (int i = lo[0]; i <= hi[0]; ++i)
+        s += increment_cell<T>(gf, lcell(gf, Lf, i, j, k), rf, xf, ef, T(1), at(gf, i, j, k));
   const int64_t Ic = at(gc, c[0], c[1], c[2]);
   rc[Ic] = s;
   xc[Ic] = 0;
-  ec[Ic] = s * inv_diag(diag_at(gc, Lc, Ic));
+  ec[Ic] = s * inv_diag(diag_at<T>(gc, lcell(gc, Lc, c[0], c[1], c[2]), Ic));
 }
 // prolongate! + increment! at fine cell c, then ϵ := r·iD (src/MultiLevelPoisson.jl:98-100,50).
 // The coarse index of a fine index is (c+1)>>1 along a coarsened direction (down(I)=(I+2)÷2 in
@@ -572,7 +711,7 @@ __device__ __forceinline__ int pro_map(int c, int N, bool per, bool coarse) {
   return coarse ? (c + 1) >> 1 : c;
 }
 template <class T>
-__device__ __forceinline__ void prolong_cell(const Geo& gf, const Geo& gc, int cm, int perdir, const T* __restrict__ Lf,
+__device__ __forceinline__ void prolong_cell(const Geo& gf, const Geo& gc, int cm, int perdir, const LRows<T>& LR,
                                              T* __restrict__ rf, T* __restrict__ xf, T* __restrict__ ef,
                                              const T* __restrict__ xc, T w, const int* c) {
   int q[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};  // q[d][o]: coarse index of c_d + o - 1
@@ -582,14 +721,14 @@ __device__ __forceinline__ void prolong_cell(const Geo& gf, const Geo& gc, int c
     return (a < 0 || b < 0 || z < 0) ? T(0) : xc[(int64_t)a + gc.s[1] * b + gc.s[2] * z];
   };
   const int64_t I = at(gf, c[0], c[1], c[2]);
-  const T Dg = diag_at(gf, Lf, I);
+  const LC<T> Lf = lcell(gf, LR, c[0], c[1], c[2]);
+  const T Dg = diag_at<T>(gf, Lf, I);
   const T e0 = xcv(q[0][1], q[1][1], q[2][1]);
   T s = e0 * Dg;
-  s += (xcv(q[0][0], q[1][1], q[2][1]) * Lf[I] + xcv(q[0][2], q[1][1], q[2][1]) * Lf[I + 1]);
-  s += (xcv(q[0][1], q[1][0], q[2][1]) * Lf[I + gf.n] + xcv(q[0][1], q[1][2], q[2][1]) * Lf[I + gf.s[1] + gf.n]);
+  s += (xcv(q[0][0], q[1][1], q[2][1]) * Lf.lo(I, 0) + xcv(q[0][2], q[1][1], q[2][1]) * Lf.hi(I, 0, 1));
+  s += (xcv(q[0][1], q[1][0], q[2][1]) * Lf.lo(I, 1) + xcv(q[0][1], q[1][2], q[2][1]) * Lf.hi(I, 1, gf.s[1]));
   if (gf.D == 3)
-    s += (xcv(q[0][1], q[1][1], q[2][0]) * Lf[I + 2 * gf.n] +
-          xcv(q[0][1], q[1][1], q[2][2]) * Lf[I + gf.s[2] + 2 * gf.n]);
+    s += (xcv(q[0][1], q[1][1], q[2][0]) * Lf.lo(I, 2) + xcv(q[0][1], q[1][1], q[2][2]) * Lf.hi(I, 2, gf.s[2]));
   const T rn = rf[I] - w * s;
   rf[I] = rn;
   xf[I] = xf[I] + w * e0;
@@ -599,7 +738,7 @@ __device__ __forceinline__ void prolong_cell(const Geo& gf, const Geo& gc, int c
 // ---- residual! (src/Poisson.jl:92-98) fused with mom_project!'s z = div(u) and x .*= dt
 // (src/Flow.jl:225).  xin·scale is written to xout; with FLOW the source z is div(u).
 template <class T, bool FLOW>
-__global__ void __launch_bounds__(256) k_residual(Geo g, ZT t, const T* __restrict__ L, const T* __restrict__ xin,
+__global__ void __launch_bounds__(256) k_residual(Geo g, ZT t, LRows<T> LR, const T* __restrict__ xin,
                                                   T* __restrict__ xout, T scale, int write_x, T* __restrict__ z,
                                                   const T* __restrict__ u, T* __restrict__ r,
                                                   double* __restrict__ partial) {
@@ -618,10 +757,11 @@ __global__ void __launch_bounds__(256) k_residual(Geo g, ZT t, const T* __restri
       } else {
         zv = z[I];
       }
-      const T Dg = diag_at(g, L, I);
+      const LC<T> L = lcell(g, LR, c0, c1, c2);
+      const T Dg = diag_at<T>(g, L, I);
       const T iD = inv_diag(Dg);
       auto xv = [&](int64_t J) { return xin[J] * scale; };
-      const T rv = iD == T(0) ? T(0) : zv - mult_at(g, L, I, Dg, xv);
+      const T rv = iD == T(0) ? T(0) : zv - mult_at<T>(g, L, I, Dg, xv);
       r[I] = rv;
       if (write_x) xout[I] = xv(I);
       acc += (double)rv;
@@ -633,7 +773,7 @@ __global__ void __launch_bounds__(256) k_residual(Geo g, ZT t, const T* __restri
 
 // r -= s (when |s| > 2eps) ; Σr² partials ; ϵ = r·iD (Jacobi!/GS init of the first V-cycle)
 template <class T>
-__global__ void __launch_bounds__(256) k_submean_l2(Geo g, ZT t, const T* __restrict__ L, T* __restrict__ r,
+__global__ void __launch_bounds__(256) k_submean_l2(Geo g, ZT t, LRows<T> LR, T* __restrict__ r,
                                                     T* __restrict__ e, const T* __restrict__ sdev,
                                                     double* __restrict__ partial) {
   int c0, c1, z0, z1;
@@ -646,7 +786,7 @@ __global__ void __launch_bounds__(256) k_submean_l2(Geo g, ZT t, const T* __rest
       const int64_t I = at(g, c0, c1, c2);
       T rv = r[I];
       if (sub) { rv = rv - s; r[I] = rv; }
-      e[I] = rv * inv_diag(diag_at(g, L, I));
+      e[I] = rv * inv_diag(diag_at<T>(g, lcell(g, LR, c0, c1, c2), I));
       acc += (double)rv * (double)rv;
     }
   }
@@ -656,21 +796,21 @@ __global__ void __launch_bounds__(256) k_submean_l2(Geo g, ZT t, const T* __rest
 
 // ϵ = r·iD on inside (Jacobi!/GS-RB init, src/Poisson.jl:112,142)
 template <class T>
-__global__ void __launch_bounds__(256) k_eps_init(Geo g, ZT t, const T* __restrict__ L, const T* __restrict__ r,
+__global__ void __launch_bounds__(256) k_eps_init(Geo g, ZT t, LRows<T> LR, const T* __restrict__ r,
                                                   T* __restrict__ e) {
   int c0, c1, z0, z1;
   zt_cell(t, c0, c1, z0, z1);
   if (c0 > t.b.hi[0] || c1 > t.b.hi[1]) return;
   for (int c2 = z0; c2 <= z1; ++c2) {
     const int64_t I = at(g, c0, c1, c2);
-    e[I] = r[I] * inv_diag(diag_at(g, L, I));
+    e[I] = r[I] * inv_diag(diag_at<T>(g, lcell(g, LR, c0, c1, c2), I));
   }
 }
 
 // increment! (src/Poisson.jl:100-104): r -= ω·Aϵ; x += ω·ϵ.  Optionally the Σr² partials (L₂ of
 // the finest level after a V-cycle, src/MultiLevelPoisson.jl:115).
 template <class T, bool L2>
-__global__ void __launch_bounds__(256) k_increment(Geo g, ZT t, const T* __restrict__ L, T* __restrict__ r,
+__global__ void __launch_bounds__(256) k_increment(Geo g, ZT t, LRows<T> LR, T* __restrict__ r,
                                                    T* __restrict__ x, const T* __restrict__ e, const T* __restrict__ wp,
                                                    double* __restrict__ partial) {
   const T w = *wp;
@@ -679,7 +819,7 @@ __global__ void __launch_bounds__(256) k_increment(Geo g, ZT t, const T* __restr
   double acc = 0;
   if (c0 <= t.b.hi[0] && c1 <= t.b.hi[1]) {
     for (int c2 = z0; c2 <= z1; ++c2) {
-      const T rn = increment_cell(g, L, r, x, e, w, at(g, c0, c1, c2));
+      const T rn = increment_cell<T>(g, lcell(g, LR, c0, c1, c2), r, x, e, w, at(g, c0, c1, c2));
       acc += (double)rn * (double)rn;
     }
   }
@@ -692,9 +832,9 @@ __global__ void __launch_bounds__(256) k_increment(Geo g, ZT t, const T* __restr
 // Jacobi!'s increment! (ω=1) on the fine level fused with restrict! and fill!(coarse.x,0)
 // (src/MultiLevelPoisson.jl:92-94,13-19,49): one thread per coarse inside cell (any coarsening mask).
 template <class T>
-__global__ void __launch_bounds__(256) k_inc_restrict(Geo gf, Geo gc, ZT t, int cm, const T* __restrict__ Lf,
+__global__ void __launch_bounds__(256) k_inc_restrict(Geo gf, Geo gc, ZT t, int cm, LRows<T> Lf,
                                                       T* __restrict__ rf, T* __restrict__ xf,
-                                                      const T* __restrict__ ef, const T* __restrict__ Lc,
+                                                      const T* __restrict__ ef, LRows<T> Lc,
                                                       T* __restrict__ rc, T* __restrict__ xc, T* __restrict__ ec) {
   int c[3], z0, z1;
   zt_cell(t, c[0], c[1], z0, z1);
@@ -708,9 +848,9 @@ __global__ void __launch_bounds__(256) k_inc_restrict(Geo gf, Geo gc, ZT t, int 
 // the reference's x-fastest order (src/MultiLevelPoisson.jl:13-19).  The tile's x and y ranges
 // are fine-cell ranges, its z range is in coarse planes.
 template <class T, int D>
-__global__ void __launch_bounds__(256) k_inc_restrict_full(Geo gf, Geo gc, ZT t, const T* __restrict__ Lf,
+__global__ void __launch_bounds__(256) k_inc_restrict_full(Geo gf, Geo gc, ZT t, LRows<T> Lf,
                                                            T* __restrict__ rf, T* __restrict__ xf,
-                                                           const T* __restrict__ ef, const T* __restrict__ Lc,
+                                                           const T* __restrict__ ef, LRows<T> Lc,
                                                            T* __restrict__ rc, T* __restrict__ xc,
                                                            T* __restrict__ ec) {
   __shared__ T sR[2][BY][64];
@@ -726,8 +866,9 @@ __global__ void __launch_bounds__(256) k_inc_restrict_full(Geo gf, Geo gc, ZT t,
       T rn = 0;
       if (in) {
         const int64_t J = at(gf, cx, cy, cz);
-        const T Dg = diag_at(gf, Lf, J);
-        rn = rf[J] - T(1) * mult_at(gf, Lf, J, Dg, xv);
+        const LC<T> L = lcell(gf, Lf, cx, cy, cz);
+        const T Dg = diag_at<T>(gf, L, J);
+        rn = rf[J] - T(1) * mult_at<T>(gf, L, J, Dg, xv);
         rf[J] = rn;
         xf[J] = xf[J] + T(1) * ef[J];
       }
@@ -739,10 +880,11 @@ __global__ void __launch_bounds__(256) k_inc_restrict_full(Geo gf, Geo gc, ZT t,
       for (int q = 0; q < NZ; ++q)
         for (int yy = 0; yy < 2; ++yy)
           for (int xx = 0; xx < 2; ++xx) s += sR[q][ty + yy][tx + xx];
-      const int64_t Ic = at(gc, (cx + 1) >> 1, (cy + 1) >> 1, D == 3 ? kc : 0);
+      const int ccx = (cx + 1) >> 1, ccy = (cy + 1) >> 1, ccz = D == 3 ? kc : 0;
+      const int64_t Ic = at(gc, ccx, ccy, ccz);
       rc[Ic] = s;
       xc[Ic] = 0;
-      ec[Ic] = s * inv_diag(diag_at(gc, Lc, Ic));
+      ec[Ic] = s * inv_diag(diag_at<T>(gc, lcell(gc, Lc, ccx, ccy, ccz), Ic));
     }
     __syncthreads();
   }
@@ -750,18 +892,18 @@ __global__ void __launch_bounds__(256) k_inc_restrict_full(Geo gf, Geo gc, ZT t,
 
 // GS-RB half sweep (src/Poisson.jl:124-128,144-146) over half_rangek
 template <class T>
-__global__ void __launch_bounds__(256) k_gs_half(Geo g, ZT t, const T* __restrict__ L, const T* __restrict__ r,
+__global__ void __launch_bounds__(256) k_gs_half(Geo g, ZT t, LRows<T> LR, const T* __restrict__ r,
                                                  T* __restrict__ e, int it) {
   int c0, c1, z0, z1;
   zt_cell(t, c0, c1, z0, z1);
   if (c0 > t.b.hi[0] || c1 > t.b.hi[1]) return;
-  for (int kz = z0; kz <= z1; ++kz) gs_cell(g, L, r, e, it, c0, c1, kz);
+  for (int kz = z0; kz <= z1; ++kz) gs_cell(g, LR, r, e, it, c0, c1, kz);
 }
 
 // prolongate! + increment! on the fine level, then ϵ := r·iD (the fine level's GS-RB init)
 template <class T>
 __global__ void __launch_bounds__(256) k_prolong_inc(Geo gf, Geo gc, ZT t, int cm, int perdir,
-                                                     const T* __restrict__ Lf, T* __restrict__ rf,
+                                                     LRows<T> Lf, T* __restrict__ rf,
                                                      T* __restrict__ xf, T* __restrict__ ef,
                                                      const T* __restrict__ xc, const T* __restrict__ wp) {
   const T w = *wp;
@@ -880,7 +1022,8 @@ __global__ void __launch_bounds__(1024) k_mg_tail(TailDesc<T> d, const T* __rest
     for (int idx = threadIdx.x; idx < cnt; idx += blockDim.x) {
       int q[3];
       box_decode(bc, idx, q);
-      inc_restrict_cell(f.g, c.g, f.cm, (const T*)f.L, f.r, f.x, (const T*)f.e, (const T*)c.L, c.r, c.x, c.e, q);
+      inc_restrict_cell(f.g, c.g, f.cm, lrows_plain((const T*)f.L), f.r, f.x, (const T*)f.e,
+                        lrows_plain((const T*)c.L), c.r, c.x, c.e, q);
     }
     __syncthreads();
   }
@@ -894,7 +1037,7 @@ __global__ void __launch_bounds__(1024) k_mg_tail(TailDesc<T> d, const T* __rest
       for (int idx = threadIdx.x; idx < nh; idx += blockDim.x) {
         int q[3];
         box_decode(bh, idx, q);
-        gs_cell(p.g, (const T*)p.L, (const T*)p.r, p.e, it, q[0], q[1], q[2]);
+        gs_cell(p.g, lrows_plain((const T*)p.L), (const T*)p.r, p.e, it, q[0], q[1], q[2]);
       }
       __syncthreads();
     }
@@ -914,7 +1057,7 @@ __global__ void __launch_bounds__(1024) k_mg_tail(TailDesc<T> d, const T* __rest
       for (int idx = threadIdx.x; idx < nf; idx += blockDim.x) {
         int q[3];
         box_decode(bf, idx, q);
-        prolong_cell(f.g, p.g, f.cm, d.perdir, (const T*)f.L, f.r, f.x, f.e, (const T*)p.x, w, q);
+        prolong_cell(f.g, p.g, f.cm, d.perdir, lrows_plain((const T*)f.L), f.r, f.x, f.e, (const T*)p.x, w, q);
       }
       __syncthreads();
     }
@@ -1113,7 +1256,7 @@ __global__ void __launch_bounds__(1024) k_mg_tail_lds(TailDesc<T> d, const T* __
 // mom_project! correction (src/Flow.jl:227-230): u[I,i] -= L[I,i]·(x[I]-x[I-e_i]) on inside,
 // p = x/dt on every cell.
 template <class T>
-__global__ void __launch_bounds__(256) k_correct(Geo g, ZT t, T* __restrict__ u, const T* __restrict__ L,
+__global__ void __launch_bounds__(256) k_correct(Geo g, ZT t, T* __restrict__ u, LRows<T> LR,
                                                  const T* __restrict__ x, T* __restrict__ p, T dt) {
   int c[3], z0, z1;
   zt_cell(t, c[0], c[1], z0, z1);
@@ -1123,8 +1266,10 @@ __global__ void __launch_bounds__(256) k_correct(Geo g, ZT t, T* __restrict__ u,
     bool in = true;
     for (int d = 0; d < g.D; ++d) in &= c[d] >= 1 && c[d] <= g.N[d] - 2;
     const T xI = x[I];
-    if (in)
-      for (int i = 0; i < g.D; ++i) u[I + i * g.n] -= L[I + i * g.n] * (xI - x[I - g.s[i]]);
+    if (in) {
+      const LC<T> L = lcell(g, LR, c[0], c[1], c[2]);
+      for (int i = 0; i < g.D; ++i) u[I + i * g.n] -= L.lo(I, i) * (xI - x[I - g.s[i]]);
+    }
     p[I] = xI / dt;
   }
 }
@@ -1352,6 +1497,122 @@ __global__ void __launch_bounds__(256) k_pressure_force(Geo g, Box b, DBodies bs
   }
 }
 
+// pressure_force, viscous_force, pressure_moment, viscous_moment (src/Metrics.jl:126-197): per
+// cell the df values the reference forms (force products in T; moments with loc − x₀ and the
+// cross product in double, rounded to T), block sums in double.  partial[blk + q·pstride]:
+// q = 0..2 pressure force, 3..5 viscous force, 6..8 pressure moment, 9..11 viscous moment
+// (moment entry 0 is the 2D cross product).  Cells outside the band (nds = 0) add zeros.
+struct X0 { double v[3]; };
+template <class T>
+__global__ void __launch_bounds__(256) k_body_metrics(Geo g, Box b, DBodies bs, const T* __restrict__ p,
+                                                      const T* __restrict__ u, T nu, X0 x0,
+                                                      double* __restrict__ partial, int64_t pstride) {
+  int c[3];
+  double acc[12] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+  const int D = g.D;
+  if (box_cell(b, c[0], c[1], c[2])) {
+    const int64_t I = at(g, c[0], c[1], c[2]);
+    T x[3] = {T(c[0] + 1) - T(1.5), T(c[1] + 1) - T(1.5), T(c[2] + 1) - T(1.5)};
+    T d, nv[3];
+    bodies_measure_d<T>(bs, D, x, T(1), d, nv);
+    const T dc = d < T(-1) ? T(-1) : (d > T(1) ? T(1) : d);
+    const T k = kern_d<T>(dc);
+    T nds[3] = {0, 0, 0};
+    bool any = false;
+    for (int i = 0; i < D; ++i) { nds[i] = nv[i] * k; any |= nds[i] != T(0); }
+    if (any) {
+      double a[3] = {0, 0, 0}, nd[3] = {0, 0, 0};
+      for (int i = 0; i < D; ++i) { a[i] = (double)x[i] - x0.v[i]; nd[i] = (double)nds[i]; }
+      auto cross = [&](const double* q, const double* w, double* r) {
+        if (D == 2) r[0] = q[0] * w[1] - q[1] * w[0];
+        else { r[0] = q[1] * w[2] - q[2] * w[1]; r[1] = q[2] * w[0] - q[0] * w[2]; r[2] = q[0] * w[1] - q[1] * w[0]; }
+      };
+      const int nM = D == 2 ? 1 : 3;
+      double cr[3];
+      if (p) {
+        const T pv = p[I];
+        for (int i = 0; i < D; ++i) acc[i] = (double)(pv * nds[i]);
+        cross(a, nd, cr);
+        for (int q = 0; q < nM; ++q) acc[6 + q] = (double)(T)((double)pv * cr[q]);
+      }
+      if (u) {
+        const int64_t n = g.n;
+        T J[3][3];
+        for (int i = 0; i < D; ++i)
+          for (int j = 0; j < D; ++j) {
+            const int64_t si = g.s[i], sj = g.s[j];
+            const T* ui = u + i * n;
+            J[i][j] = i == j ? ui[I + si] - ui[I]
+                             : (((ui[I + sj] + ui[I + sj + si]) - ui[I - sj]) - ui[I - sj + si]) / T(4);
+          }
+        const T cf = T(-2) * nu;
+        double bv[3] = {0, 0, 0};
+        for (int i = 0; i < D; ++i) {
+          const T S0 = (J[i][0] + J[0][i]) / T(2);
+          T w = (cf * S0) * nds[0];
+          T sn = S0 * nds[0];
+          for (int j = 1; j < D; ++j) {
+            const T Sij = (J[i][j] + J[j][i]) / T(2);
+            w = w + (cf * Sij) * nds[j];
+            sn = sn + Sij * nds[j];
+          }
+          acc[3 + i] = (double)w;
+          bv[i] = (double)sn;
+        }
+        cross(a, bv, cr);
+        for (int q = 0; q < nM; ++q) acc[9 + q] = (double)(T)((double)cf * cr[q]);
+      }
+    }
+  }
+  for (int q = 0; q < 12; ++q) {
+    const double s = block_sum(acc[q]);
+    if (threadIdx.x == 0 && threadIdx.y == 0) partial[block_linear_id() + q * pstride] = s;
+  }
+}
+// one workgroup per row q of partial: out[q] = Σ partial[q·pstride .. q·pstride + np)
+__global__ void __launch_bounds__(256) k_finish_rows(const double* __restrict__ partial, int64_t np, int64_t pstride,
+                                                     double* __restrict__ out) {
+  double acc = 0;
+  const double* r = partial + blockIdx.x * pstride;
+  for (int64_t q = threadIdx.x; q < np; q += blockDim.x) acc += r[q];
+  const double s = block_sum(acc);
+  if (threadIdx.x == 0) out[blockIdx.x] = s;
+}
+
+// MeanFlow update! blend (src/Metrics.jl:234-240) with weight e: P = e·p + (1−e)·P, U likewise,
+// UU[I,i,j] = e·(u[I,i]·u[I,j]) + (1−e)·UU[I,i,j]; one thread per cell.
+template <class T>
+__global__ void __launch_bounds__(256) k_meanflow(int64_t n, int D, T e, T* __restrict__ P, const T* __restrict__ p,
+                                                  T* __restrict__ U, const T* __restrict__ u, T* __restrict__ UU) {
+  const int64_t I = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (I >= n) return;
+  const T om = T(1) - e;
+  P[I] = e * p[I] + om * P[I];
+  T uv[3];
+  for (int i = 0; i < D; ++i) {
+    uv[i] = u[I + i * n];
+    U[I + i * n] = e * uv[i] + om * U[I + i * n];
+  }
+  if (UU)
+    for (int j = 0; j < D; ++j)
+      for (int i = 0; i < D; ++i) {
+        T& v = UU[I + (i + (int64_t)D * j) * n];
+        v = e * (uv[i] * uv[j]) + om * v;
+      }
+}
+// uu!(τ, meanflow) (src/Metrics.jl:243-245): τ[I,i,j] = UU[I,i,j] − U[I,i]·U[I,j]
+template <class T>
+__global__ void __launch_bounds__(256) k_uu(int64_t n, int D, const T* __restrict__ UU, const T* __restrict__ U,
+                                            T* __restrict__ tau) {
+  const int64_t I = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (I >= n) return;
+  for (int j = 0; j < D; ++j)
+    for (int i = 0; i < D; ++i) {
+      const int64_t q = I + (i + (int64_t)D * j) * n;
+      tau[q] = UU[q] - U[I + i * n] * U[I + j * n];
+    }
+}
+
 // ω of solver!(ml) kept on the device (src/MultiLevelPoisson.jl:110,117-121): st[3] = ω,
 // st[4] = r₂ (both in T).  k_mg_init sets ω = 1 and r₂ = Σr² after residual!; k_mg_control
 // takes the new Σr² (dsc[1], rounded to T), adapts ω, stores r₂ and keeps the old r₂ in dsc[6].
@@ -1370,6 +1631,102 @@ __global__ void k_mg_control(double* __restrict__ dsc, T* __restrict__ st) {
   st[3] = w;
   st[4] = rnew;
   dsc[6] = (double)r2;
+}
+
+// ---- mixed precision: FP32 V-cycles inside an FP64 defect-correction loop ----
+// The outer residual r = z − A·x is evaluated in double from the FP32 coefficients and an FP64
+// solution x64, then rounded to T for the V-cycle, which solves A·c = r for the correction c
+// (level-0 x, started from 0).  FLOW: first pass of a projection, z = div(u) and
+// x64 = (double)(p·scale) (the warm start x .*= w·Δt, src/Flow.jl:225); otherwise
+// x64 ← x64 + c, read on the fly at the stencil's cells (xa and xb are distinct buffers).
+// Partials: Σr in partial[b], Σr² in partial[nb + b], both in double.
+template <class T, bool FLOW>
+__global__ void __launch_bounds__(256) k_residual64(Geo g, ZT t, LRows<T> LR, const T* __restrict__ pin, T scale,
+                                                    T* __restrict__ z, const T* __restrict__ u,
+                                                    const double* __restrict__ xa, const T* __restrict__ xc,
+                                                    double* __restrict__ xb, T* __restrict__ r,
+                                                    double* __restrict__ partial, int64_t nb) {
+  int c0, c1, z0, z1;
+  zt_cell(t, c0, c1, z0, z1);
+  double acc = 0, acc2 = 0;
+  if (c0 <= t.b.hi[0] && c1 <= t.b.hi[1]) {
+    for (int c2 = z0; c2 <= z1; ++c2) {
+      const int64_t I = at(g, c0, c1, c2);
+      T zv;
+      if (FLOW) {
+        T s = 0;
+        for (int i = 0; i < g.D; ++i) s += (u[I + g.s[i] + i * g.n] - u[I + i * g.n]);
+        zv = s;
+        z[I] = zv;
+      } else {
+        zv = z[I];
+      }
+      auto xv = [&](int64_t J) -> double { return FLOW ? (double)(pin[J] * scale) : xa[J] + (double)xc[J]; };
+      const LC<T> L = lcell(g, LR, c0, c1, c2);
+      double Dg = 0;
+      for (int i = 0; i < g.D; ++i) Dg -= ((double)L.lo(I, i) + (double)L.hi(I, i, g.s[i]));
+      const double xI = xv(I);
+      double ax = xI * Dg;
+      for (int i = 0; i < g.D; ++i) {
+        const int64_t si = g.s[i];
+        ax += (xv(I - si) * (double)L.lo(I, i) + xv(I + si) * (double)L.hi(I, i, si));
+      }
+      const double rv = Dg == 0.0 ? 0.0 : (double)zv - ax;
+      xb[I] = xI;
+      r[I] = (T)rv;
+      acc += rv;
+      acc2 += rv * rv;
+    }
+  }
+  const double s1 = block_sum(acc);
+  const double s2 = block_sum(acc2);
+  if (threadIdx.x == 0 && threadIdx.y == 0) { partial[blockIdx.x] = s1; partial[nb + blockIdx.x] = s2; }
+}
+// Σr, the mean s = Σr/N and Σ(r − s)² = Σr² − N·s² (the mean is removed when |s| > 2eps(T),
+// src/Poisson.jl:95-97): dsc[0] = Σr, dsc[1] = the residual norm; st[0] = (T)s, st[1] = flag.
+template <class T>
+__global__ void __launch_bounds__(256) k_finish_sum2(const double* __restrict__ partial, int64_t nb,
+                                                     double* __restrict__ dsc, T* __restrict__ st, double ninside) {
+  double a = 0, b = 0;
+  for (int64_t q = threadIdx.x; q < nb; q += blockDim.x) { a += partial[q]; b += partial[nb + q]; }
+  a = block_sum(a);
+  b = block_sum(b);
+  if (threadIdx.x == 0) {
+    const double s = a / ninside;
+    const bool sub = fabs(s) > 2.0 * (double)eps_t<T>();
+    dsc[0] = a;
+    dsc[1] = sub ? b - ninside * s * s : b;
+    st[0] = (T)s;
+    st[1] = sub ? T(1) : T(0);
+  }
+}
+// r −= s (when flagged), ϵ = r·iD, correction c = 0: the V-cycle's start on the defect
+template <class T>
+__global__ void __launch_bounds__(256) k_mixed_prep(Geo g, ZT t, LRows<T> LR, T* __restrict__ r, T* __restrict__ e,
+                                                    T* __restrict__ c, const T* __restrict__ sdev) {
+  int c0, c1, z0, z1;
+  zt_cell(t, c0, c1, z0, z1);
+  if (c0 > t.b.hi[0] || c1 > t.b.hi[1]) return;
+  const T s = sdev[0];
+  const bool sub = sdev[1] != T(0);
+  for (int c2 = z0; c2 <= z1; ++c2) {
+    const int64_t I = at(g, c0, c1, c2);
+    T rv = r[I];
+    if (sub) { rv = rv - s; r[I] = rv; }
+    e[I] = rv * inv_diag(diag_at<T>(g, lcell(g, LR, c0, c1, c2), I));
+    c[I] = T(0);
+  }
+}
+// x = (T)x64 on inside (the solution handed back to mom_project!)
+template <class T>
+__global__ void __launch_bounds__(256) k_round_x(Geo g, ZT t, const double* __restrict__ x64, T* __restrict__ x) {
+  int c0, c1, z0, z1;
+  zt_cell(t, c0, c1, z0, z1);
+  if (c0 > t.b.hi[0] || c1 > t.b.hi[1]) return;
+  for (int c2 = z0; c2 <= z1; ++c2) {
+    const int64_t I = at(g, c0, c1, c2);
+    x[I] = (T)x64[I];
+  }
 }
 
 // Device self-test: counts floats x (all 2^32 bit patterns, finite ones) for which div6(x)

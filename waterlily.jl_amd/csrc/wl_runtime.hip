@@ -188,6 +188,10 @@ struct Level {
   T *L = nullptr, *x = nullptr, *r = nullptr, *e = nullptr, *z = nullptr;
   bool ownL = true, ownx = true, ownz = true;
   int cmask = 0;  // coarsening mask towards the next level
+  // uniform-row flags of L: lrow[0, N1·N2) row bits, lrow[N1·N2, 2·N1·N2) the flags
+  uint8_t* lrow = nullptr;
+  bool lrow_ok = false;  // flags match the current L
+  T lc[3] = {1, 1, 1};   // the level's uniform face value per component
 };
 
 struct Bufs {  // reduction scratch shared by a handle
@@ -208,14 +212,19 @@ struct MG {
   Bufs* B = nullptr;
   Prof* P = nullptr;
 
+  double* x64[2] = {nullptr, nullptr};  // FP64 solutions of the mixed-precision loop (ping-pong)
+  int x64cur = 0;
   ~MG() {
     free_graphs();
+    dfree(x64[0]);
+    dfree(x64[1]);
     for (auto& l : lv) {
       if (l.ownL) dfree(l.L);
       if (l.ownx) dfree(l.x);
       if (l.ownz) dfree(l.z);
       dfree(l.r);
       dfree(l.e);
+      dfree(l.lrow);
     }
   }
 
@@ -237,6 +246,11 @@ struct MG {
     l0.ownz = !z0;
     l0.r = dalloc<T>(l0.g.n);
     l0.e = dalloc<T>(l0.g.n);
+    l0.lrow = dalloc<uint8_t>(2 * (int64_t)l0.g.N[1] * l0.g.N[2]);
+    {
+      const char* e = std::getenv("WL_LROWS");
+      use_lrows = !e || std::atoi(e) != 0;
+    }
     lv.push_back(l0);
     if (!multilevel) return;
     while ((int)lv.size() <= maxlevels) {
@@ -256,6 +270,16 @@ struct MG {
       lc.z = dalloc<T>(lc.g.n);
       lc.r = dalloc<T>(lc.g.n);
       lc.e = dalloc<T>(lc.g.n);
+      lc.lrow = dalloc<uint8_t>(2 * (int64_t)lc.g.N[1] * lc.g.N[2]);
+      // restrictL of a uniform fine level: Σ over the 2 children of each coarsened transverse
+      // direction, halved when the face's own direction is coarsened (all powers of two: exact)
+      const Level<T>& fl = lv.back();
+      for (int i = 0; i < D; ++i) {
+        T v = fl.lc[i];
+        for (int j = 0; j < D; ++j)
+          if (j != i && ((cm >> j) & 1)) v = v + v;
+        lc.lc[i] = ((cm >> i) & 1) ? v / T(2) : v;
+      }
       lv.push_back(lc);
     }
     if (lv.size() <= 2) throw WlErr(WL_EINVAL, "MultiLevelPoisson requires size=a2ⁿ, where n>2");
@@ -279,7 +303,8 @@ struct MG {
     hipLaunchKernelGGL(k_bc<T>, gd, dim3(256), 0, st, g, a, 1, U[0], U[1], g.D > 2 ? U[2] : 0.0, saveexit, perdir);
     KCK();
   }
-  void perbc(const Geo& g, T* a) {
+  template <class U>
+  void perbc(const Geo& g, U* a) {
     if (!perdir) return;
     int maxplane = 0;
     for (int d = 0; d < g.D; ++d) {
@@ -288,11 +313,35 @@ struct MG {
       maxplane = std::max(maxplane, p);
     }
     dim3 gd((maxplane + 255) / 256, 1, 2 * g.D);
-    hipLaunchKernelGGL(k_bc<T>, gd, dim3(256), 0, st, g, a, 0, 0.0, 0.0, 0.0, 0, perdir);
+    hipLaunchKernelGGL(k_bc<U>, gd, dim3(256), 0, st, g, a, 0, 0.0, 0.0, 0.0, 0, perdir);
     KCK();
   }
 
-  // update!(ml): coarse L by restrictL!, then BC!(L,0) (src/MultiLevelPoisson.jl:42-48,79-86)
+  // Uniform-row flags of level l from its current L (WL_LROWS=0 disables the fast rows).
+  bool use_lrows = true;
+  void build_lrows(int l) {
+    Level<T>& p = lv[l];
+    const int64_t nr = (int64_t)p.g.N[1] * p.g.N[2];
+    p.lrow_ok = false;
+    if (!use_lrows) return;
+    hipLaunchKernelGGL(k_lrow_bits<T>, dim3((unsigned)((nr + 3) / 4)), dim3(256), 0, st, p.g, (const T*)p.L,
+                       p.lc[0], p.lc[1], p.lc[2], p.lrow);
+    KCK();
+    hipLaunchKernelGGL(k_lrow_flags, dim3((unsigned)((nr + 255) / 256)), dim3(256), 0, st, p.g,
+                       (const uint8_t*)p.lrow, p.lrow + nr);
+    KCK();
+    p.lrow_ok = true;
+  }
+  void ensure_lrows() {
+    for (size_t l = 0; l < lv.size(); ++l)
+      if (!lv[l].lrow_ok) build_lrows((int)l);
+  }
+  LRows<T> lr(int l) const {
+    const Level<T>& p = lv[l];
+    return LRows<T>{p.L, p.lrow_ok ? p.lrow + (int64_t)p.g.N[1] * p.g.N[2] : nullptr, {p.lc[0], p.lc[1], p.lc[2]}};
+  }
+  // update!(ml): coarse L by restrictL!, then BC!(L,0) (src/MultiLevelPoisson.jl:42-48,79-86);
+  // then the uniform-row flags of every level
   void update() {
     const double zero[3] = {0, 0, 0};
     for (size_t l = 1; l < lv.size(); ++l) {
@@ -305,6 +354,7 @@ struct MG {
       }
       bc_vec(c.g, c.L, zero, 0);
     }
+    for (size_t l = 0; l < lv.size(); ++l) build_lrows((int)l);
   }
 
   double finish_sum(int64_t np, int slot, int mode = 0, double ninside = 1) {
@@ -336,7 +386,7 @@ struct MG {
     perbc(p.g, p.x);
     ZT b = make_zt(box_inside(p.g));
     const int64_t nb = zt_blocks(b);
-    hipLaunchKernelGGL((k_residual<T, false>), zt_grid(b), kBlock, 0, st, p.g, b, p.L, p.x, p.x, T(1), 0, p.z,
+    hipLaunchKernelGGL((k_residual<T, false>), zt_grid(b), kBlock, 0, st, p.g, b, lr(0), p.x, p.x, T(1), 0, p.z,
                        (const T*)nullptr, p.r, B->partial);
     KCK();
     finish_sum(nb, 0, 1, (double)ninside(p.g));
@@ -347,7 +397,7 @@ struct MG {
     ZT b = make_zt(box_inside(p.g));
     const int64_t nb = zt_blocks(b);
     ProfScope ps(P, st, "submean_l2", 0, (double)sizeof(T) * (p.g.D + 3) * ncells_inside(p.g));
-    hipLaunchKernelGGL(k_submean_l2<T>, zt_grid(b), kBlock, 0, st, p.g, b, p.L, p.r, p.e, (const T*)B->tsc,
+    hipLaunchKernelGGL(k_submean_l2<T>, zt_grid(b), kBlock, 0, st, p.g, b, lr(0), p.r, p.e, (const T*)B->tsc,
                        B->partial);
     KCK();
     finish_sum(nb, 1);
@@ -362,7 +412,7 @@ struct MG {
     Level<T>& p = lv[l];
     ZT b = make_zt(box_inside(p.g));
     ProfScope ps(P, st, "eps_init", l, (double)sizeof(T) * (p.g.D + 2) * ncells_inside(p.g));
-    hipLaunchKernelGGL(k_eps_init<T>, zt_grid(b), kBlock, 0, st, p.g, b, p.L, p.r, p.e);
+    hipLaunchKernelGGL(k_eps_init<T>, zt_grid(b), kBlock, 0, st, p.g, b, lr(l), p.r, p.e);
     KCK();
   }
   void gs_sweeps(int l) {
@@ -373,7 +423,7 @@ struct MG {
     ZT b = make_zt(bh);
     for (int it = 1; it <= 4; ++it) {
       ProfScope ps(P, st, "gs_half", l, (double)sizeof(T) * (p.g.D + 1.5) * ncells_inside(p.g));
-      hipLaunchKernelGGL(k_gs_half<T>, zt_grid(b), kBlock, 0, st, p.g, b, p.L, p.r, p.e, it);
+      hipLaunchKernelGGL(k_gs_half<T>, zt_grid(b), kBlock, 0, st, p.g, b, lr(l), p.r, p.e, it);
       KCK();
     }
   }
@@ -383,9 +433,9 @@ struct MG {
     ZT b = make_zt(box_inside(p.g));
     ProfScope ps(P, st, "increment", l, (double)sizeof(T) * (p.g.D + 5) * ncells_inside(p.g));
     if (l2)
-      hipLaunchKernelGGL((k_increment<T, true>), zt_grid(b), kBlock, 0, st, p.g, b, p.L, p.r, p.x, p.e, w, B->partial);
+      hipLaunchKernelGGL((k_increment<T, true>), zt_grid(b), kBlock, 0, st, p.g, b, lr(l), p.r, p.x, p.e, w, B->partial);
     else
-      hipLaunchKernelGGL((k_increment<T, false>), zt_grid(b), kBlock, 0, st, p.g, b, p.L, p.r, p.x, p.e, w,
+      hipLaunchKernelGGL((k_increment<T, false>), zt_grid(b), kBlock, 0, st, p.g, b, lr(l), p.r, p.x, p.e, w,
                          B->partial);
     KCK();
     if (l2) finish_sum(zt_blocks(b), 1);
@@ -460,15 +510,15 @@ struct MG {
         if (f.g.D == 3) { b.lo[2] = 1; b.hi[2] = c.g.N[2] - 2; }
         ZT bc = make_zt(b);
         if (f.g.D == 3)
-          hipLaunchKernelGGL((k_inc_restrict_full<T, 3>), zt_grid(bc), kBlock, 0, st, f.g, c.g, bc, f.L, f.r, f.x,
-                             f.e, c.L, c.r, c.x, c.e);
+          hipLaunchKernelGGL((k_inc_restrict_full<T, 3>), zt_grid(bc), kBlock, 0, st, f.g, c.g, bc, lr(l), f.r, f.x,
+                             f.e, lr(l + 1), c.r, c.x, c.e);
         else
-          hipLaunchKernelGGL((k_inc_restrict_full<T, 2>), zt_grid(bc), kBlock, 0, st, f.g, c.g, bc, f.L, f.r, f.x,
-                             f.e, c.L, c.r, c.x, c.e);
+          hipLaunchKernelGGL((k_inc_restrict_full<T, 2>), zt_grid(bc), kBlock, 0, st, f.g, c.g, bc, lr(l), f.r, f.x,
+                             f.e, lr(l + 1), c.r, c.x, c.e);
       } else {
         ZT bc = make_zt(box_inside(c.g));
-        hipLaunchKernelGGL(k_inc_restrict<T>, zt_grid(bc), kBlock, 0, st, f.g, c.g, bc, f.cmask, f.L, f.r, f.x, f.e,
-                           c.L, c.r, c.x, c.e);
+        hipLaunchKernelGGL(k_inc_restrict<T>, zt_grid(bc), kBlock, 0, st, f.g, c.g, bc, f.cmask, lr(l), f.r, f.x, f.e,
+                           lr(l + 1), c.r, c.x, c.e);
       }
       KCK();
     }
@@ -482,7 +532,7 @@ struct MG {
       ZT bf = make_zt(box_inside(f.g));
       ProfScope ps(P, st, "prolong_inc", c - 1,
                    (double)sizeof(T) * ((f.g.D + 5) * ncells_inside(f.g) + ncells_inside(lv[c].g)));
-      hipLaunchKernelGGL(k_prolong_inc<T>, zt_grid(bf), kBlock, 0, st, f.g, lv[c].g, bf, f.cmask, perdir, f.L, f.r,
+      hipLaunchKernelGGL(k_prolong_inc<T>, zt_grid(bf), kBlock, 0, st, f.g, lv[c].g, bf, f.cmask, perdir, lr(c - 1), f.r,
                          f.x, f.e, (const T*)lv[c].x, w);
       KCK();
     }
@@ -553,6 +603,62 @@ struct MG {
       if ((double)rnew < tol) break;
     }
     perbc(lv[0].g, lv[0].x);
+    return n;
+  }
+
+  // Mixed precision (WL_SOLVER_MIXED): each iteration runs the FP32 V-cycle and finest smooth! on
+  // the defect r (level-0 x is the correction c, from 0), then x64 += c and r = z − A·x64 in FP64
+  // (k_residual64); ω adapts on, and the loop stops on, that FP64 Σr² (src/MultiLevelPoisson.jl:
+  // 108-127 otherwise).  Expects x64[x64cur] and dsc[1] from the projection's first pass.
+  void ensure_x64() {
+    for (auto& b : x64)
+      if (!b) b = dalloc<double>(lv[0].g.n);
+  }
+  void finish_sum2(int64_t nb) {
+    hipLaunchKernelGGL(k_finish_sum2<T>, dim3(1), dim3(256), 0, st, (const double*)B->partial, nb, B->dsc, (T*)B->tsc,
+                       (double)ninside(lv[0].g));
+    KCK();
+  }
+  int mixed_loop(double tol, int itmx, std::vector<double>* hist) {
+    Level<T>& p = lv[0];
+    ZT bi = make_zt(box_inside(p.g));
+    const int64_t nb = zt_blocks(bi);
+    const T* w = (const T*)B->tsc + 3;
+    hipLaunchKernelGGL(k_mg_init<T>, dim3(1), dim3(64), 0, st, (const double*)B->dsc, (T*)B->tsc);
+    KCK();
+    if (hist) hist->push_back((double)(T)read_dsc(1));
+    int n = 0;
+    while (n < itmx) {
+      {
+        ProfScope ps(P, st, "mixed_prep", 0, (double)sizeof(T) * (p.g.D + 4) * ncells_inside(p.g));
+        hipLaunchKernelGGL(k_mixed_prep<T>, zt_grid(bi), kBlock, 0, st, p.g, bi, lr(0), p.r, p.e, p.x,
+                           (const T*)B->tsc);
+        KCK();
+      }
+      vcycle(w);
+      gs_sweeps(0);
+      increment(0, w, false);
+      perbc(p.g, p.x);
+      {
+        ProfScope ps(P, st, "residual64", 0,
+                     (double)ncells_inside(p.g) * (sizeof(T) * (p.g.D + 2) + 2 * sizeof(double)));
+        hipLaunchKernelGGL((k_residual64<T, false>), zt_grid(bi), kBlock, 0, st, p.g, bi, lr(0), (const T*)nullptr,
+                           T(0), p.z, (const T*)nullptr, (const double*)x64[x64cur], (const T*)p.x,
+                           x64[1 - x64cur], p.r, B->partial, nb);
+        KCK();
+      }
+      x64cur = 1 - x64cur;
+      perbc(p.g, x64[x64cur]);
+      finish_sum2(nb);
+      hipLaunchKernelGGL(k_mg_control<T>, dim3(1), dim3(64), 0, st, B->dsc, (T*)B->tsc);
+      KCK();
+      HIPCK(hipMemcpyAsync(B->hsc, B->dsc, 8 * sizeof(double), hipMemcpyDeviceToHost, st));
+      HIPCK(hipStreamSynchronize(st));
+      const T rnew = (T)B->hsc[1];
+      if (hist) hist->push_back((double)rnew);
+      n++;
+      if ((double)rnew < tol) break;
+    }
     return n;
   }
 
@@ -630,11 +736,15 @@ struct FlowBase {
   virtual void pois_update() = 0;
   virtual int step(double dt, double t0, double t1, double* dt_next, int16_t* nit) = 0;
   virtual void pressure_force(const wl_body* b, int nb, double t, double* F) = 0;
+  virtual void metrics(const wl_body* b, int nb, const double* x0, double* out) = 0;
+  virtual hipStream_t stream() const = 0;
   virtual void sync() = 0;
   virtual Prof& profiler() = 0;
   double tol = 1e-4;
   int itmx = 32;
   int mode = WL_SOLVER_PLAIN;
+  bool mu1_dirty = true;  // μ₁ changed since its row flags were built
+  virtual void field_uploaded(int field) = 0;
 };
 
 static DBodies to_dbodies(const wl_body* b, int nb) {
@@ -655,6 +765,7 @@ struct FlowT : FlowBase {
   Geo g;
   int D;
   T *u, *u0, *f, *p, *sigma, *V, *mu0, *mu1, *xw;
+  uint8_t* rows = nullptr;  // [0, N1·N2): x-rows holding a nonzero μ₁; then rows where V is not +0
   MG<T> mg;
   Bufs B;
   hipStream_t st;
@@ -672,6 +783,7 @@ struct FlowT : FlowBase {
     u = dalloc<T>(n * D); u0 = dalloc<T>(n * D); f = dalloc<T>(n * D); V = dalloc<T>(n * D);
     mu0 = dalloc<T>(n * D); mu1 = dalloc<T>(n * D * D);
     p = dalloc<T>(n); sigma = dalloc<T>(n); xw = dalloc<T>(n);
+    rows = dalloc<uint8_t>(2 * (int64_t)g.N[1] * g.N[2]);
     alloc_bufs(B, 3 * nblocks(box_all(g)) + 16, sizeof(T));
     mg.st = st;
     mg.B = &B;
@@ -694,7 +806,8 @@ struct FlowT : FlowBase {
   }
   ~FlowT() override {
     (void)hipStreamSynchronize(st);
-    dfree(u); dfree(u0); dfree(f); dfree(V); dfree(mu0); dfree(mu1); dfree(p); dfree(sigma); dfree(xw);
+    dfree(u); dfree(u0); dfree(f); dfree(V); dfree(mu0); dfree(mu1); dfree(p); dfree(sigma); dfree(xw); dfree(rows);
+    dfree(mpart);
     free_bufs(B);
     mg.lv[0].ownL = mg.lv[0].ownx = mg.lv[0].ownz = false;
     (void)hipStreamDestroy(st);
@@ -745,7 +858,24 @@ struct FlowT : FlowBase {
     const double zero[3] = {0, 0, 0};
     mg.bc_vec(g, mu0, zero, 0);
     mg.bc_vec(g, V, zero, desc.exit_bc);
+    mu1_dirty = true;
+    mg.lv[0].lrow_ok = false;
     sync();
+  }
+  uint8_t* vrows() const { return rows + (int64_t)g.N[1] * g.N[2]; }
+  void field_uploaded(int field) override {
+    if (field == WL_MU1 || field == WL_V) mu1_dirty = true;
+    if (field == WL_MU0) mg.lv[0].lrow_ok = false;  // L of the finest level is μ₀
+  }
+  void mu1_rows() {
+    const int64_t nr = (int64_t)g.N[1] * g.N[2];
+    hipLaunchKernelGGL(k_fill<uint8_t>, dim3((unsigned)((2 * nr + 255) / 256)), dim3(256), 0, st, rows, 2 * nr,
+                       (uint8_t)0);
+    KCK();
+    Box ba = box_all(g);
+    hipLaunchKernelGGL(k_mu1_rows<T>, grid_of(ba), kBlock, 0, st, g, ba, (const T*)mu1, (const T*)V, rows, rows + nr);
+    KCK();
+    mu1_dirty = false;
   }
   void pois_update() override {
     mg.update();
@@ -781,24 +911,24 @@ struct FlowT : FlowBase {
       if (desc.perdir_mask) {
         if (pred)
           hipLaunchKernelGGL((k_conv_bdim1<T, S, true, DD>), zt_grid(ba), cblk, 0, st, g, ba, (const T*)u, (const T*)u0, f,
-                             sigma, (const T*)V, nu, dt, desc.perdir_mask, acc);
+                             sigma, (const T*)V, (const uint8_t*)vrows(), nu, dt, desc.perdir_mask, acc);
         else
           hipLaunchKernelGGL((k_conv_bdim1<T, S, false, DD>), zt_grid(ba), cblk, 0, st, g, ba, (const T*)u, (const T*)u0, f,
-                             sigma, (const T*)V, nu, dt, desc.perdir_mask, acc);
+                             sigma, (const T*)V, (const uint8_t*)vrows(), nu, dt, desc.perdir_mask, acc);
       } else if (conv_prefetch()) {
         if (pred)
           hipLaunchKernelGGL((k_conv_lds<T, S, true, DD, true>), zt_grid(ba), cblk, 0, st, g, ba, (const T*)u, (const T*)u0, f,
-                             sigma, (const T*)V, nu, dt, acc);
+                             sigma, (const T*)V, (const uint8_t*)vrows(), nu, dt, acc);
         else
           hipLaunchKernelGGL((k_conv_lds<T, S, false, DD, true>), zt_grid(ba), cblk, 0, st, g, ba, (const T*)u, (const T*)u0, f,
-                             sigma, (const T*)V, nu, dt, acc);
+                             sigma, (const T*)V, (const uint8_t*)vrows(), nu, dt, acc);
       } else {
         if (pred)
           hipLaunchKernelGGL((k_conv_lds<T, S, true, DD, false>), zt_grid(ba), cblk, 0, st, g, ba, (const T*)u, (const T*)u0, f,
-                             sigma, (const T*)V, nu, dt, acc);
+                             sigma, (const T*)V, (const uint8_t*)vrows(), nu, dt, acc);
         else
           hipLaunchKernelGGL((k_conv_lds<T, S, false, DD, false>), zt_grid(ba), cblk, 0, st, g, ba, (const T*)u, (const T*)u0, f,
-                             sigma, (const T*)V, nu, dt, acc);
+                             sigma, (const T*)V, (const uint8_t*)vrows(), nu, dt, acc);
       }
       KCK();
     }
@@ -809,10 +939,10 @@ struct FlowT : FlowBase {
     ProfScope ps(&prof, st, "bdim2", 0, (double)sizeof(T) * (5 * D + D * D) * ncells_inside(g));
     if (pred)
       hipLaunchKernelGGL((k_bdim2<T, true>), zt_grid(bi), kBlock, 0, st, g, bi, u, (const T*)u0, (const T*)f,
-                         (const T*)V, (const T*)mu0, (const T*)mu1);
+                         (const T*)V, mg.lr(0), (const T*)mu1, (const uint8_t*)rows, (const uint8_t*)vrows());
     else
       hipLaunchKernelGGL((k_bdim2<T, false>), zt_grid(bi), kBlock, 0, st, g, bi, u, (const T*)u0, (const T*)f,
-                         (const T*)V, (const T*)mu0, (const T*)mu1);
+                         (const T*)V, mg.lr(0), (const T*)mu1, (const uint8_t*)rows, (const uint8_t*)vrows());
     KCK();
   }
   void momentum(bool pred, T dt, double t, double t1) {
@@ -841,15 +971,49 @@ struct FlowT : FlowBase {
     KCK();
   }
 
+  // mom_project! with the mixed-precision solver: the same projection around the FP64 defect-
+  // correction loop; x (= p·Δt) is the FP64 solution rounded to T.
+  int project_mixed(T w, T dt_end, double t1) {
+    const T dt = w * dt_end;
+    Level<T>& l0 = mg.lv[0];
+    ZT bi = make_zt(box_inside(g));
+    const int64_t nb = zt_blocks(bi);
+    mg.ensure_x64();
+    mg.x64cur = 0;
+    {
+      ProfScope ps(&prof, st, "residual64", 0, (double)ncells_inside(g) * (sizeof(T) * (3 * D + 3) + sizeof(double)));
+      hipLaunchKernelGGL((k_residual64<T, true>), zt_grid(bi), kBlock, 0, st, g, bi, mg.lr(0), (const T*)p, dt, sigma,
+                         (const T*)u, (const double*)nullptr, (const T*)nullptr, mg.x64[0], l0.r, B.partial, nb);
+      KCK();
+    }
+    mg.perbc(g, mg.x64[0]);
+    mg.finish_sum2(nb);
+    const int nit = mg.mixed_loop(tol, itmx, &r2hist);
+    hipLaunchKernelGGL(k_round_x<T>, zt_grid(bi), kBlock, 0, st, g, bi, (const double*)mg.x64[mg.x64cur], xw);
+    KCK();
+    mg.perbc(g, xw);
+    ZT ba = make_zt(box_all(g));
+    {
+    ProfScope ps(&prof, st, "correct", 0, (double)sizeof(T) * (2 * g.n + 3 * D * ncells_inside(g)));
+    hipLaunchKernelGGL(k_correct<T>, zt_grid(ba), kBlock, 0, st, g, ba, u, mg.lr(0), (const T*)xw, p, dt);
+    KCK();
+    }
+    double Ub[3];
+    Ut(t1, Ub);
+    mg.bc_vec(g, u, Ub, desc.exit_bc);
+    return nit;
+  }
+
   // mom_project! (src/Flow.jl:223-232)
   int project(T w, T dt_end, double t1) {
+    if (mode == WL_SOLVER_MIXED) return project_mixed(w, dt_end, t1);
     const T dt = w * dt_end;
     Level<T>& l0 = mg.lv[0];
     ZT bi = make_zt(box_inside(g));
     const int64_t nb = zt_blocks(bi);
     {
     ProfScope ps(&prof, st, "residual", 0, (double)sizeof(T) * (2 * D + 4) * ncells_inside(g));
-    hipLaunchKernelGGL((k_residual<T, true>), zt_grid(bi), kBlock, 0, st, g, bi, (const T*)mu0, (const T*)p, xw, dt, 1,
+    hipLaunchKernelGGL((k_residual<T, true>), zt_grid(bi), kBlock, 0, st, g, bi, mg.lr(0), (const T*)p, xw, dt, 1,
                        sigma, (const T*)u, l0.r, B.partial);
     KCK();
     }
@@ -860,7 +1024,7 @@ struct FlowT : FlowBase {
     ZT ba = make_zt(box_all(g));
     {
     ProfScope ps(&prof, st, "correct", 0, (double)sizeof(T) * (2 * g.n + 3 * D * ncells_inside(g)));
-    hipLaunchKernelGGL(k_correct<T>, zt_grid(ba), kBlock, 0, st, g, ba, u, (const T*)mu0, (const T*)xw, p, dt);
+    hipLaunchKernelGGL(k_correct<T>, zt_grid(ba), kBlock, 0, st, g, ba, u, mg.lr(0), (const T*)xw, p, dt);
     KCK();
     }
     double Ub[3];
@@ -870,6 +1034,8 @@ struct FlowT : FlowBase {
   }
 
   int step(double dt_d, double t0, double t1, double* dt_next, int16_t* nit) override {
+    if (mu1_dirty) mu1_rows();
+    mg.ensure_lrows();
     r2hist.clear();
     const T dt = (T)dt_d;
     momentum(true, dt, t0, t1);
@@ -916,6 +1082,29 @@ struct FlowT : FlowBase {
     HIPCK(hipStreamSynchronize(st));
     for (int i = 0; i < D; ++i) F[i] = B.hsc[i];
   }
+  // the four body integrals in one pass; out[12] as k_body_metrics' rows
+  double* mpart = nullptr;
+  int64_t mpart_n = 0;
+  void metrics(const wl_body* b, int nb, const double* x0, double* out) override {
+    DBodies bs = to_dbodies(b, nb);
+    Box bi = box_inside(g);
+    const int64_t np = nblocks(bi);
+    if (mpart_n < 12 * np + 12) {
+      dfree(mpart);
+      mpart = dalloc<double>(12 * np + 12);
+      mpart_n = 12 * np + 12;
+    }
+    X0 x;
+    for (int k = 0; k < 3; ++k) x.v[k] = k < D ? x0[k] : 0.0;
+    hipLaunchKernelGGL(k_body_metrics<T>, grid_of(bi), kBlock, 0, st, g, bi, bs, (const T*)p, (const T*)u,
+                       (T)desc.nu, x, mpart, np);
+    KCK();
+    hipLaunchKernelGGL(k_finish_rows, dim3(12), dim3(256), 0, st, (const double*)mpart, np, np, mpart + 12 * np);
+    KCK();
+    HIPCK(hipMemcpyAsync(out, mpart + 12 * np, 12 * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCK(hipStreamSynchronize(st));
+  }
+  hipStream_t stream() const override { return st; }
 };
 
 struct wl_flow_s {
@@ -991,6 +1180,7 @@ struct PoisT : PoisBase {
       default: throw WlErr(WL_EINVAL, "unknown Poisson field id");
     }
     if (nbytes != cnt * sizeof(T)) { dfree(tmp); throw WlErr(WL_EINVAL, "buffer size does not match the field"); }
+    if (up && field == WL_PL) lv.lrow_ok = false;
     HIPCK(hipStreamSynchronize(st));
     HIPCK(hipMemcpy(up ? (void*)ptr : host, up ? host : (void*)ptr, nbytes,
                     up ? hipMemcpyHostToDevice : hipMemcpyDeviceToHost));
@@ -1016,6 +1206,7 @@ struct PoisT : PoisBase {
     HIPCK(hipStreamSynchronize(st));
   }
   int solve(double tol, int itmx) override {
+    mg.ensure_lrows();
     mg.residual_generic();
     const int n = mg.multilevel ? mg.mg_loop(tol, itmx, nullptr) : mg.pcg_loop(tol, itmx);
     HIPCK(hipStreamSynchronize(st));
@@ -1075,6 +1266,7 @@ static void flow_io(wl_flow_t h, int field, void* buf, size_t nbytes, bool up) {
   h->impl->sync();
   void* dp = h->impl->field_ptr(field);
   HIPCK(hipMemcpy(up ? dp : buf, up ? buf : dp, nbytes, up ? hipMemcpyHostToDevice : hipMemcpyDeviceToHost));
+  if (up) h->impl->field_uploaded(field);
 }
 int wl_flow_upload(wl_flow_t h, int field, const void* src, size_t nbytes) {
   API_BEGIN
@@ -1098,7 +1290,7 @@ int wl_flow_pois_update(wl_flow_t h) {
 }
 int wl_flow_set_solver(wl_flow_t h, int mode, double tol, int itmx) {
   API_BEGIN
-  if (mode != WL_SOLVER_PLAIN) throw WlErr(WL_EINVAL, "solver mode not available");
+  if (mode != WL_SOLVER_PLAIN && mode != WL_SOLVER_MIXED) throw WlErr(WL_EINVAL, "unknown solver mode");
   if (itmx < 1) throw WlErr(WL_EINVAL, "itmx must be >= 1");
   h->impl->mode = mode;
   h->impl->tol = tol;
@@ -1113,6 +1305,138 @@ int wl_mom_step(wl_flow_t h, double dt, double t0, double t1, double* dt_next, i
 int wl_pressure_force(wl_flow_t h, const wl_body* bodies, int nbodies, double t, double* F) {
   API_BEGIN
   h->impl->pressure_force(bodies, nbodies, t, F);
+  API_END
+}
+int wl_flow_metrics(wl_flow_t h, const wl_body* bodies, int nbodies, double t, const double* x0, double* Fp,
+                    double* Fv, double* Mp, double* Mv) {
+  API_BEGIN
+  (void)t;
+  if (!x0) throw WlErr(WL_EINVAL, "x0 is required");
+  double out[12];
+  h->impl->metrics(bodies, nbodies, x0, out);
+  const int D = h->impl->desc.D, nM = D == 2 ? 1 : 3;
+  if (Fp) for (int i = 0; i < D; ++i) Fp[i] = out[i];
+  if (Fv) for (int i = 0; i < D; ++i) Fv[i] = out[3 + i];
+  if (Mp) for (int i = 0; i < nM; ++i) Mp[i] = out[6 + i];
+  if (Mv) for (int i = 0; i < nM; ++i) Mv[i] = out[9 + i];
+  API_END
+}
+
+// MeanFlow (src/Metrics.jl:199-261): the averaged fields live on the flow's device; the time
+// vector stays with the caller, which forms each update's weight.
+struct wl_meanflow_s {
+  int dtype = WL_F32, D = 3;
+  int64_t n = 0;
+  bool uu = false;
+  void *P = nullptr, *U = nullptr, *UU = nullptr;
+  hipStream_t st = nullptr;
+  size_t es() const { return dtype == WL_F64 ? 8 : 4; }
+  ~wl_meanflow_s() {
+    if (st) (void)hipStreamSynchronize(st);
+    dfree(P); dfree(U); dfree(UU);
+  }
+};
+int wl_meanflow_create(wl_meanflow_t* out, wl_flow_t flow, int uu_stats) {
+  API_BEGIN
+  if (!out || !flow) throw WlErr(WL_EINVAL, "null handle");
+  auto m = std::make_unique<wl_meanflow_s>();
+  m->dtype = flow->impl->dtype;
+  m->D = flow->impl->desc.D;
+  m->n = (int64_t)flow->impl->field_count(WL_P);
+  m->uu = uu_stats != 0;
+  m->st = flow->impl->stream();
+  if (m->dtype == WL_F64) {
+    m->P = dalloc<double>(m->n); m->U = dalloc<double>(m->n * m->D);
+    if (m->uu) m->UU = dalloc<double>(m->n * m->D * m->D);
+  } else {
+    m->P = dalloc<float>(m->n); m->U = dalloc<float>(m->n * m->D);
+    if (m->uu) m->UU = dalloc<float>(m->n * m->D * m->D);
+  }
+  *out = m.release();
+  API_END
+}
+int wl_meanflow_destroy(wl_meanflow_t m) {
+  API_BEGIN
+  delete m;
+  API_END
+}
+int wl_meanflow_update(wl_meanflow_t m, wl_flow_t flow, double eps) {
+  API_BEGIN
+  if (!m || !flow) throw WlErr(WL_EINVAL, "null handle");
+  if (flow->impl->dtype != m->dtype || (int64_t)flow->impl->field_count(WL_P) != m->n)
+    throw WlErr(WL_EINVAL, "flow does not match the MeanFlow");
+  const dim3 gd((unsigned)((m->n + 255) / 256));
+  hipStream_t st = flow->impl->stream();
+  if (m->dtype == WL_F64)
+    hipLaunchKernelGGL(k_meanflow<double>, gd, dim3(256), 0, st, m->n, m->D, eps, (double*)m->P,
+                       (const double*)flow->impl->field_ptr(WL_P), (double*)m->U,
+                       (const double*)flow->impl->field_ptr(WL_U), (double*)m->UU);
+  else
+    hipLaunchKernelGGL(k_meanflow<float>, gd, dim3(256), 0, st, m->n, m->D, (float)eps, (float*)m->P,
+                       (const float*)flow->impl->field_ptr(WL_P), (float*)m->U,
+                       (const float*)flow->impl->field_ptr(WL_U), (float*)m->UU);
+  KCK();
+  HIPCK(hipStreamSynchronize(st));
+  API_END
+}
+int wl_meanflow_reset(wl_meanflow_t m) {
+  API_BEGIN
+  if (!m) throw WlErr(WL_EINVAL, "null handle");
+  HIPCK(hipStreamSynchronize(m->st));
+  HIPCK(hipMemset(m->P, 0, m->n * m->es()));
+  HIPCK(hipMemset(m->U, 0, m->n * m->D * m->es()));
+  if (m->UU) HIPCK(hipMemset(m->UU, 0, m->n * m->D * m->D * m->es()));
+  HIPCK(hipDeviceSynchronize());
+  API_END
+}
+int wl_meanflow_download(wl_meanflow_t m, int field, void* dst, size_t nbytes) {
+  API_BEGIN
+  if (!m || !dst) throw WlErr(WL_EINVAL, "null argument");
+  const int64_t D = m->D, DD = D * D;
+  size_t cnt = 0;
+  const void* src = nullptr;
+  void* tmp = nullptr;
+  switch (field) {
+    case WL_MF_P: cnt = m->n; src = m->P; break;
+    case WL_MF_U: cnt = m->n * D; src = m->U; break;
+    case WL_MF_UU: case WL_MF_TAU:
+      if (!m->UU) throw WlErr(WL_EINVAL, "MeanFlow was created without uu_stats");
+      cnt = m->n * DD;
+      src = m->UU;
+      break;
+    default: throw WlErr(WL_EINVAL, "unknown MeanFlow field id");
+  }
+  if (nbytes != cnt * m->es()) throw WlErr(WL_EINVAL, "buffer size does not match the field");
+  HIPCK(hipStreamSynchronize(m->st));
+  if (field == WL_MF_TAU) {
+    const dim3 gd((unsigned)((m->n + 255) / 256));
+    if (m->dtype == WL_F64) {
+      tmp = dalloc<double>(cnt);
+      hipLaunchKernelGGL(k_uu<double>, gd, dim3(256), 0, m->st, m->n, m->D, (const double*)m->UU,
+                         (const double*)m->U, (double*)tmp);
+    } else {
+      tmp = dalloc<float>(cnt);
+      hipLaunchKernelGGL(k_uu<float>, gd, dim3(256), 0, m->st, m->n, m->D, (const float*)m->UU,
+                         (const float*)m->U, (float*)tmp);
+    }
+    KCK();
+    HIPCK(hipStreamSynchronize(m->st));
+    src = tmp;
+  }
+  const hipError_t e = hipMemcpy(dst, src, nbytes, hipMemcpyDeviceToHost);
+  dfree(tmp);
+  HIPCK(e);
+  API_END
+}
+int wl_meanflow_copy_to_flow(wl_meanflow_t m, wl_flow_t flow) {
+  API_BEGIN
+  if (!m || !flow) throw WlErr(WL_EINVAL, "null handle");
+  if (flow->impl->dtype != m->dtype || (int64_t)flow->impl->field_count(WL_P) != m->n)
+    throw WlErr(WL_EINVAL, "flow does not match the MeanFlow");
+  hipStream_t st = flow->impl->stream();
+  HIPCK(hipMemcpyAsync(flow->impl->field_ptr(WL_U), m->U, m->n * m->D * m->es(), hipMemcpyDeviceToDevice, st));
+  HIPCK(hipMemcpyAsync(flow->impl->field_ptr(WL_P), m->P, m->n * m->es(), hipMemcpyDeviceToDevice, st));
+  HIPCK(hipStreamSynchronize(st));
   API_END
 }
 int wl_flow_profile(wl_flow_t h, int enable) {
