@@ -32,10 +32,11 @@ def close(a, b, scale):
 
 @pytest.mark.parametrize("D", [2, 3])
 def test_metrics_synthetic_fields(wl, D):
-    """p = y and u = (y², 0, …) on a 32^D grid with a sphere of radius 8 (the closed-form cases of
-    the oracle tests), uploaded to a device flow with ν = 0.7."""
+    """p = y and u = (y², 0, …) on a 34^D grid (32 inside cells, so the flow's multigrid builds) with
+    a sphere of radius 8 (the closed-form cases of the oracle tests), uploaded to a device flow with
+    ν = 0.7."""
     say(f"case: body metrics on synthetic fields D={D}")
-    N, R, nu = 32, 8, 0.7
+    N, R, nu = 34, 8, 0.7
     Ng = (N,) * D
     c = (N / 2,) * D
     sim = wl.Simulation(tuple(n - 2 for n in Ng), (1,) + (0,) * (D - 1), 2 * R, ν=nu, body=wl.Sphere(c, R),

@@ -153,14 +153,17 @@ def test_exit_bc(wl, D):
     assert abs(float(np.mean(ex, dtype=np.float64)) - float(np.mean(inl, dtype=np.float64))) < 1e-5
 
 
-def test_uniform_rows_match_loaded(wl, monkeypatch):
-    """The uniform-coefficient rows (L formed, not loaded; V and μ₁ skipped where they vanish)
-    give the same bits as loading every coefficient (WL_LROWS=0 at build), over several steps."""
-    say("case: uniform rows vs loaded coefficients, 3D 64x32x32 sphere, 4 steps")
+@pytest.mark.parametrize("var", ["WL_LROWS", "WL_GS_PAIR"])
+def test_fast_paths_match_plain(wl, monkeypatch, var):
+    """WL_LROWS: the uniform-coefficient rows (L formed, not loaded; V and μ₁ skipped where they
+    vanish; on by default); WL_GS_PAIR: two GS-RB half sweeps fused per pass (off by default).
+    Each gives the same bits as the path without it (the variable set to 0 when the solver is
+    built), over several steps."""
+    say(f"case: {var}=1 vs 0, 3D 64x32x32 sphere, 4 steps")
     n, c = (64, 32, 32), (24, 16, 16)
-    monkeypatch.setenv("WL_LROWS", "1")
+    monkeypatch.setenv(var, "1")
     a = wl.Simulation(n, (1, 0, 0), 12, ν=12 / 400, body=wl.Sphere(c, 6), T=np.float32)
-    monkeypatch.setenv("WL_LROWS", "0")
+    monkeypatch.setenv(var, "0")
     b = wl.Simulation(n, (1, 0, 0), 12, ν=12 / 400, body=wl.Sphere(c, 6), T=np.float32)
     for k in range(4):
         wl.sim_step(a, remeasure=False)

@@ -900,6 +900,69 @@ __global__ void __launch_bounds__(256) k_gs_half(Geo g, ZT t, LRows<T> LR, const
   for (int kz = z0; kz <= z1; ++kz) gs_cell(g, LR, r, e, it, c0, c1, kz);
 }
 
+// Two GS-RB half sweeps in one pass (src/Poisson.jl:116-148, it = 2q−1 then 2q, 3D): colour A
+// (cells whose 1-based index sum is even, updated by odd it) from ein, then colour B from the
+// new A, both written to eout (ein and eout are distinct buffers with equal ghosts).  Blocks of
+// 64×GBY threads overlap by one cell in x and y: the ring computes the A values the tile's B
+// cells read.  z is marched, the A values of three planes kept in LDS; a z tile recomputes the
+// A planes just outside it.  For levels without periodic directions and with an even last
+// dimension (half_rangek then covers every inside cell).
+constexpr int GBY = 16;
+template <class T>
+__global__ void __launch_bounds__(64 * GBY) k_gs_pair(Geo g, ZT t, LRows<T> LR, const T* __restrict__ r,
+                                                      const T* __restrict__ ein, T* __restrict__ eout) {
+  __shared__ T sA[3][GBY][64];
+  int cx, cy, z0, z1;
+  zt_cell(t, cx, cy, z0, z1);
+  const int tx = threadIdx.x, ty = threadIdx.y;
+  const int N0 = g.N[0], N1 = g.N[1], N2 = g.N[2];
+  const int64_t s1 = g.s[1], s2 = g.s[2];
+  const bool ingrid = cx <= N0 - 1 && cy <= N1 - 1;
+  const bool inxy = cx >= 1 && cx <= N0 - 2 && cy >= 1 && cy <= N1 - 2;
+  const bool outxy = inxy && tx >= 1 && tx <= 62 && ty >= 1 && ty <= GBY - 2;
+  auto isA = [&](int cz) { return ((cx + cy + cz + 3) & 1) == 0; };
+  // gauss (src/Poisson.jl:116-122) at (cx, cy, cz) with the six neighbour values nb[0..5] =
+  // x−, x+, y−, y+, z−, z+, in the reference's order of operations
+  auto gauss = [&](int cz, const T* nb) -> T {
+    const int64_t I = at(g, cx, cy, cz);
+    const LC<T> L = lcell(g, LR, cx, cy, cz);
+    const T Dg = diag_at<T>(g, L, I);
+    T s = r[I];
+    s -= (nb[0] * L.lo(I, 0) + nb[1] * L.hi(I, 0, 1));
+    s -= (nb[2] * L.lo(I, 1) + nb[3] * L.hi(I, 1, s1));
+    s -= (nb[4] * L.lo(I, 2) + nb[5] * L.hi(I, 2, s2));
+    return s * inv_diag(Dg);
+  };
+  for (int kz = z0 - 1; kz <= z1 + 1; ++kz) {
+    T a = 0;
+    if (ingrid && isA(kz)) {
+      const int64_t I = at(g, cx, cy, kz);
+      if (inxy && kz >= 1 && kz <= N2 - 2) {
+        const T nb[6] = {ein[I - 1], ein[I + 1], ein[I - s1], ein[I + s1], ein[I - s2], ein[I + s2]};
+        a = gauss(kz, nb);
+      } else {
+        a = ein[I];
+      }
+    }
+    sA[kz % 3][ty][tx] = a;
+    __syncthreads();
+    const int bz = kz - 1;
+    if (bz >= z0 && outxy) {
+      const int q = bz % 3;
+      T v;
+      if (isA(bz)) {
+        v = sA[q][ty][tx];
+      } else {
+        const T nb[6] = {sA[q][ty][tx - 1], sA[q][ty][tx + 1], sA[q][ty - 1][tx], sA[q][ty + 1][tx],
+                         sA[(bz + 2) % 3][ty][tx], sA[kz % 3][ty][tx]};
+        v = gauss(bz, nb);
+      }
+      eout[at(g, cx, cy, bz)] = v;
+    }
+    __syncthreads();
+  }
+}
+
 // prolongate! + increment! on the fine level, then ϵ := r·iD (the fine level's GS-RB init)
 template <class T>
 __global__ void __launch_bounds__(256) k_prolong_inc(Geo gf, Geo gc, ZT t, int cm, int perdir,

@@ -186,6 +186,7 @@ template <class T>
 struct Level {
   Geo g;
   T *L = nullptr, *x = nullptr, *r = nullptr, *e = nullptr, *z = nullptr;
+  T* e2 = nullptr;  // second ϵ buffer of the fused GS-RB pairs (ghosts stay 0, as ϵ's do)
   bool ownL = true, ownx = true, ownz = true;
   int cmask = 0;  // coarsening mask towards the next level
   // uniform-row flags of L: lrow[0, N1·N2) row bits, lrow[N1·N2, 2·N1·N2) the flags
@@ -224,6 +225,7 @@ struct MG {
       if (l.ownz) dfree(l.z);
       dfree(l.r);
       dfree(l.e);
+      dfree(l.e2);
       dfree(l.lrow);
     }
   }
@@ -250,6 +252,8 @@ struct MG {
     {
       const char* e = std::getenv("WL_LROWS");
       use_lrows = !e || std::atoi(e) != 0;
+      const char* e2 = std::getenv("WL_GS_PAIR");
+      use_gs_pair = e2 && std::atoi(e2) != 0;
     }
     lv.push_back(l0);
     if (!multilevel) return;
@@ -283,6 +287,8 @@ struct MG {
       lv.push_back(lc);
     }
     if (lv.size() <= 2) throw WlErr(WL_EINVAL, "MultiLevelPoisson requires size=a2ⁿ, where n>2");
+    for (size_t l = 0; l < lv.size(); ++l)  // before any graph capture: no allocation while capturing
+      if (gs_pair_ok((int)l)) lv[l].e2 = dalloc<T>(lv[l].g.n);
     static bool attr_set = false;  // the LDS tail may use more than 64 KB of dynamic LDS
     if (!attr_set) {
       HIPCK(hipFuncSetAttribute((const void*)k_mg_tail_lds<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -415,9 +421,36 @@ struct MG {
     hipLaunchKernelGGL(k_eps_init<T>, zt_grid(b), kBlock, 0, st, p.g, b, lr(l), p.r, p.e);
     KCK();
   }
+  // fused GS-RB pairs (k_gs_pair) where they apply, with WL_GS_PAIR=1; by default the four half
+  // sweeps run (on the 256x128x128 sphere a pair took 61 µs against 2×20 µs for two half sweeps:
+  // its plane-by-plane barriers and one-colour-per-phase lanes cost more than the traffic saved)
+  bool use_gs_pair = false;
+  bool gs_pair_ok(int l) const {
+    const Geo& g = lv[l].g;
+    return use_gs_pair && perdir == 0 && g.D == 3 && g.N[2] % 2 == 0 && g.N[0] >= 3 && g.N[1] >= 3 && g.N[2] >= 4;
+  }
   void gs_sweeps(int l) {
     Level<T>& p = lv[l];
     perbc(p.g, p.e);
+    if (gs_pair_ok(l) && p.e2) {
+      ZT t;
+      t.b.lo[0] = 0; t.b.lo[1] = 0; t.b.lo[2] = 1;
+      t.b.hi[0] = p.g.N[0] - 1; t.b.hi[1] = p.g.N[1] - 1; t.b.hi[2] = p.g.N[2] - 2;
+      t.sx = 62; t.sy = GBY - 2;
+      t.nbx = (p.g.N[0] - 2 + 61) / 62;
+      t.nby = (p.g.N[1] - 2 + GBY - 3) / (GBY - 2);
+      const int64_t per = (int64_t)t.nbx * t.nby;
+      const int nz = p.g.N[2] - 2;
+      t.zc = (int)std::min<int64_t>(32, std::max<int64_t>(4, (int64_t)nz * per / 2048));
+      const double bytes = (double)sizeof(T) * (p.g.D + 3) * ncells_inside(p.g);
+      for (int q = 0; q < 2; ++q) {
+        ProfScope ps(P, st, "gs_pair", l, bytes);
+        hipLaunchKernelGGL(k_gs_pair<T>, zt_grid(t), dim3(64, GBY), 0, st, p.g, t, lr(l), (const T*)p.r,
+                           (const T*)(q == 0 ? p.e : p.e2), q == 0 ? p.e2 : p.e);
+        KCK();
+      }
+      return;
+    }
     Box bh = box_half(p.g);
     if (box_empty(bh)) return;
     ZT b = make_zt(bh);
