@@ -28,10 +28,12 @@ def sphere_sim(wl, n, R, center, Re, L):
     return wl.Simulation(n, (1, 0, 0), L, ν=L / Re, body=wl.Sphere(center, R), T=np.float32)
 
 
-def run_gpu(wl, n, R, center, Re, L, steps, warmup, profile_steps=None):
+def run_gpu(wl, n, R, center, Re, L, steps, warmup, profile_steps=None, mode=None):
     """Warm up, time `steps` sim_step!s, then run `profile_steps` more with per-kernel HIP events
     recorded on the library's stream (the events add launch gaps, so they stay out of the timed steps)."""
     sim = sphere_sim(wl, n, R, center, Re, L)
+    if mode is not None:
+        sim.pois.set_solver(mode=mode)
     for _ in range(warmup):
         wl.sim_step(sim, remeasure=False)
     n0 = len(sim.pois.n)
@@ -70,7 +72,7 @@ def roofline(stats, traffic_file):
     rows.sort(key=lambda r: -r["total_ms"])
     dom = next(r for r in rows if r["GBs"] is not None)
     traffic = None
-    if os.path.exists(traffic_file):
+    if traffic_file and os.path.exists(traffic_file):
         with open(traffic_file) as fh:
             tt = json.load(fh)
         traffic = tt.get(dom["kernel"])
@@ -221,11 +223,18 @@ def main():
     if not a.no_c3:
         n3, R3, c3, Re3, L3 = (512, 256, 256), 32, (128, 128, 128), 1e4, 64
         r3 = run_gpu(wl, n3, R3, c3, Re3, L3, a.c3_steps, 3)
-        roof3, _ = roofline(r3["stats"], os.path.join(ROOT, "profiles", "hbm_traffic_c3.json"))
+        roof3, rows3 = roofline(r3["stats"], os.path.join(ROOT, "profiles", "hbm_traffic_c3.json"))
         out["c3"] = dict(workload="3D sphere 512x256x256, r=32, Re=1e4, plain FP32 MG", steps=a.c3_steps,
                          mcells=round(r3["mcells"], 2), ms_per_step=round(r3["ms_per_step"], 3),
-                         vcycles_per_step=round(r3["vcycles_per_step"], 3), roofline=roof3)
+                         vcycles_per_step=round(r3["vcycles_per_step"], 3), roofline=roof3,
+                         kernels_L0=[k for k in rows3 if k["kernel"].endswith("@L0")][:10])
         del r3
+        r3m = run_gpu(wl, n3, R3, c3, Re3, L3, a.c3_steps, 3, profile_steps=0, mode=wl.WL_SOLVER_MIXED)
+        out["c3_mixed"] = dict(workload="the same 512x256x256 sphere with FP32 V-cycles inside an FP64 "
+                                        "defect-correction loop (coarsest-level smoothing in FP64)", steps=a.c3_steps,
+                               mcells=round(r3m["mcells"], 2), ms_per_step=round(r3m["ms_per_step"], 3),
+                               vcycles_per_step=round(r3m["vcycles_per_step"], 3))
+        del r3m
     if not a.no_cpu:
         out["cpu_baseline"] = run_cpu(n, R, center, Re, L)
     else:

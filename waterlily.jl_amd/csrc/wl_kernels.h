@@ -997,8 +997,10 @@ template <class T>
 struct TailDesc {
   int n;
   int perdir;
+  int exact;  // mixed precision: the coarsest level's smooth! in FP64
   TailLevel<T> lv[TAIL_MAX_LEVELS];
 };
+constexpr int TAIL_CG_MAX = 128;  // cells (ghosts included) of a coarsest level smoothed in FP64
 __device__ __forceinline__ void box_decode(const Box& b, int idx, int* c) {
   const int nx = b.hi[0] - b.lo[0] + 1, ny = b.hi[1] - b.lo[1] + 1;
   c[0] = b.lo[0] + idx % nx;
@@ -1252,6 +1254,7 @@ __global__ void __launch_bounds__(1024) k_mg_tail_lds(TailDesc<T> d, const T* __
     });
     __syncthreads();
   }
+  __shared__ double cg[2][TAIL_CG_MAX];
   for (int l = d.n - 1; l >= 0; --l) {  // up-leg: smooth!(l), then prolongate! + increment!(l-1)
     const SLev<T> p = lev(l);
     perbc(p, p.e);
@@ -1260,6 +1263,49 @@ __global__ void __launch_bounds__(1024) k_mg_tail_lds(TailDesc<T> d, const T* __
     int hh[3] = {hi[0], hi[1], hi[2]};
     const int Ld = p.D - 1;
     hh[Ld] = (Ld == 2 ? p.N2 : p.N1) / 2 - 1;  // half_rangek
+    if (d.exact && l == d.n - 1 && p.n <= TAIL_CG_MAX && !d.perdir) {
+      // Mixed precision: smooth! of the coarsest level in double: ϵ = r·iD and the four GS-RB
+      // half sweeps (src/Poisson.jl:141-148) with ϵ, D and iD held in double, then ϵ rounded to T
+      // for increment!.  Same algorithm and order as the T sweeps below.
+      if (threadIdx.x == 0) {
+        double *E = cg[0], *ID = cg[1];
+        for (int I = 0; I < p.n; ++I) { E[I] = 0; ID[I] = 0; }
+        for (int k = lo[2]; k <= hi[2]; ++k)
+          for (int j = lo[1]; j <= hi[1]; ++j)
+            for (int i = lo[0]; i <= hi[0]; ++i) {
+              const int I = i + p.s1 * j + p.s2 * k;
+              double sd = 0;
+              sd -= ((double)p.L[I] + (double)p.L[I + 1]);
+              sd -= ((double)p.L[p.n + I] + (double)p.L[p.n + I + p.s1]);
+              if (p.D == 3) sd -= ((double)p.L[2 * p.n + I] + (double)p.L[2 * p.n + I + p.s2]);
+              ID[I] = sd == 0.0 ? 0.0 : 1.0 / sd;
+              E[I] = (double)p.r[I] * ID[I];
+            }
+        for (int it = 1; it <= 4; ++it)
+          for (int k = lo[2]; k <= hh[2]; ++k)
+            for (int j = lo[1]; j <= hh[1]; ++j)
+              for (int i = lo[0]; i <= hh[0]; ++i) {
+                int q[3] = {i, j, k};
+                int front = 0;
+                for (int dd = 0; dd < Ld; ++dd) front += q[dd] + 1;
+                q[Ld] = 2 * (q[Ld] + 1) - 1 - (front + it) % 2 - 1;
+                const int I = q[0] + p.s1 * q[1] + p.s2 * q[2];
+                double sv = (double)p.r[I];
+                sv -= (E[I - 1] * (double)p.L[I] + E[I + 1] * (double)p.L[I + 1]);
+                sv -= (E[I - p.s1] * (double)p.L[p.n + I] + E[I + p.s1] * (double)p.L[p.n + I + p.s1]);
+                if (p.D == 3)
+                  sv -= (E[I - p.s2] * (double)p.L[2 * p.n + I] + E[I + p.s2] * (double)p.L[2 * p.n + I + p.s2]);
+                E[I] = sv * ID[I];
+              }
+        for (int k = lo[2]; k <= hi[2]; ++k)
+          for (int j = lo[1]; j <= hi[1]; ++j)
+            for (int i = lo[0]; i <= hi[0]; ++i) {
+              const int I = i + p.s1 * j + p.s2 * k;
+              p.e[I] = (T)E[I];
+            }
+      }
+      __syncthreads();
+    } else
     for (int it = 1; it <= 4; ++it) {
       tile_loop(lo, hh, bx, by, bz, [&](int i, int j, int k) {
         int q[3] = {i, j, k};

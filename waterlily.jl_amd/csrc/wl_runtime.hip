@@ -499,10 +499,12 @@ struct MG {
       if (tail_bytes((int)l) <= TAIL_LDS_BYTES) { *in_lds = true; return (int)l; }
     return -1;
   }
+  bool exact_coarsest = false;  // set by the mixed-precision loop around its V-cycles
   void launch_tail(int Lt, bool in_lds, const T* w) {
     TailDesc<T> d;
     d.n = (int)lv.size() - Lt;
     d.perdir = perdir;
+    d.exact = exact_coarsest && in_lds;
     int64_t off = 0;
     for (int k = 0; k < d.n; ++k) {
       const Level<T>& q = lv[Lt + k];
@@ -668,7 +670,9 @@ struct MG {
                            (const T*)B->tsc);
         KCK();
       }
+      exact_coarsest = true;
       vcycle(w);
+      exact_coarsest = false;
       gs_sweeps(0);
       increment(0, w, false);
       perbc(p.g, p.x);
