@@ -145,6 +145,22 @@ def summarize(stats_csv, fetch_csv, write_csv, out_md, out_json, cmd, calib_byte
         lines.append(f"| {nm} | {r['Calls']} | {float(r['AverageNs']) / 1e3:.2f} | "
                      f"{float(r['TotalDurationNs']) / 1e6:.3f} | {r['Percentage']} |")
 
+    trace_csv = stats_csv.replace("_kernel_stats.csv", "_kernel_trace.csv")
+    if os.path.exists(trace_csv):
+        grids = collections.defaultdict(list)
+        for r in csv.DictReader(open(trace_csv)):
+            k = _key_of(r["Kernel_Name"])
+            if k:
+                grids[(k, r["Kernel_Name"].split("(")[0].replace("void ", ""))].append(
+                    (int(r["Grid_Size_X"]) * int(r["Grid_Size_Y"]) * int(r["Grid_Size_Z"]),
+                     (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3))
+        lines += ["", "## Level-0 dispatches (the largest grid of each kernel, from the kernel trace)", "",
+                  "| profiler key | kernel | dispatches | avg us |", "|---|---|---|---|"]
+        for (k, kn), v in sorted(grids.items(), key=lambda kv: -sum(d for g, d in kv[1])):
+            gmax = max(g for g, _ in v)
+            d0 = [d for g, d in v if g == gmax]
+            lines.append(f"| {k}@L0 | {kn} | {len(d0)} | {np.mean(d0):.2f} |")
+
     def counters(fn, cname):
         per = collections.defaultdict(list)
         copies = []

@@ -139,8 +139,9 @@ struct Prof {
   struct Rec {
     const char* key;
     int level;
-    double bytes;
+    double bytes;  // per launch
     hipEvent_t a, b;
+    int count;     // back-to-back launches of the kernel between the two events
   };
   std::vector<Rec> recs;
   std::vector<hipEvent_t> pool;
@@ -161,9 +162,9 @@ struct ProfScope {
   Prof* P;
   hipStream_t st;
   size_t idx = 0;
-  ProfScope(Prof* P_, hipStream_t st_, const char* key, int level, double bytes) : P(P_), st(st_) {
+  ProfScope(Prof* P_, hipStream_t st_, const char* key, int level, double bytes, int count = 1) : P(P_), st(st_) {
     if (P && P->on) {
-      Prof::Rec r{key, level, bytes, P->ev(), P->ev()};
+      Prof::Rec r{key, level, bytes, P->ev(), P->ev(), count};
       HIPCK(hipEventRecord(r.a, st));
       P->recs.push_back(r);
       idx = P->recs.size() - 1;
@@ -454,8 +455,9 @@ struct MG {
     Box bh = box_half(p.g);
     if (box_empty(bh)) return;
     ZT b = make_zt(bh);
+    // the four half sweeps are timed as one back-to-back group (no event between them)
+    ProfScope ps(P, st, "gs_half", l, (double)sizeof(T) * (p.g.D + 1.5) * ncells_inside(p.g), 4);
     for (int it = 1; it <= 4; ++it) {
-      ProfScope ps(P, st, "gs_half", l, (double)sizeof(T) * (p.g.D + 1.5) * ncells_inside(p.g));
       hipLaunchKernelGGL(k_gs_half<T>, zt_grid(b), kBlock, 0, st, p.g, b, lr(l), p.r, p.e, it);
       KCK();
     }
@@ -1505,7 +1507,7 @@ int wl_flow_profile_read(wl_flow_t h, wl_kernel_stat* out, int cap, int* count) 
       agg.push_back(z);
       s = &agg.back();
     }
-    s->launches += 1;
+    s->launches += r.count;
     s->total_ms += ms;
   }
   P.recs.clear();
