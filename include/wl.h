@@ -132,6 +132,19 @@ int wl_meanflow_reset(wl_meanflow_t m);                                  /* rese
 int wl_meanflow_download(wl_meanflow_t m, int field, void* dst, size_t nbytes); /* P, U, UU; τ = uu(m), :243-252 */
 int wl_meanflow_copy_to_flow(wl_meanflow_t m, wl_flow_t flow);           /* copy!(flow, meanflow), :254-257 */
 
+/* Batches: many flows of different sizes stepped together, one kernel launch per stage of the step
+ * for all of them.  No counterpart in the reference, which steps one Simulation at a time
+ * (src/WaterLily.jl:127-140 is the loop each member follows).  Members are ordinary flow handles
+ * (created, measured and pois-updated as usual, still owned by the caller); they share D, dtype and
+ * scheme, use the plain solver, and have no periodic directions and no exitBC.  One wl_batch_step
+ * is one mom_step! of every member (dt, t0, t1, dt_next: one entry per member; n_iter: two per
+ * member) with the same results as wl_mom_step on each. */
+typedef struct wl_batch_s* wl_batch_t;
+int wl_batch_create(wl_batch_t* out, const wl_flow_t* flows, int nflows);
+int wl_batch_destroy(wl_batch_t b);
+int wl_batch_step(wl_batch_t b, const double* dt, const double* t0, const double* t1, double* dt_next,
+                  int16_t* n_iter);
+
 /* Σr² residual history of the last mom_step! (both projections); returns the count written. */
 int wl_flow_last_residuals(wl_flow_t h, double* r2, int cap, int* count);
 

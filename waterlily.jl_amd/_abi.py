@@ -64,6 +64,9 @@ SIGNATURES = {
     "wl_meanflow_reset": (_i, [_vp]),
     "wl_meanflow_download": (_i, [_vp, _i, _vp, _sz]),
     "wl_meanflow_copy_to_flow": (_i, [_vp, _vp]),
+    "wl_batch_create": (_i, [_P(_vp), _P(_vp), _i]),
+    "wl_batch_destroy": (_i, [_vp]),
+    "wl_batch_step": (_i, [_vp, _P(_d), _P(_d), _P(_d), _P(_d), _P(C.c_int16)]),
     "wl_flow_last_residuals": (_i, [_vp, _P(_d), _i, _P(_i)]),
     "wl_flow_profile": (_i, [_vp, _i]),
     "wl_flow_profile_read": (_i, [_vp, _P(WlKernelStat), _i, _P(_i)]),

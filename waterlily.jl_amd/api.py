@@ -432,6 +432,48 @@ def sim_step(sim, t_end=None, remeasure=True, max_steps=None):
     return sim
 
 
+class Batch:
+    """Many Simulations of different sizes stepped together: one kernel launch per stage of the
+    time step for all members.  Members keep their own Δt, pois.n and fields; `step()` is one
+    sim_step!(sim; remeasure=false) (src/WaterLily.jl:127-140) of every member, with the same
+    results as stepping each alone.  Members share D, T and λ, use the plain solver, and have no
+    periodic directions and no exitBC."""
+
+    def __init__(self, sims):
+        self.sims = list(sims)
+        hs = (C.c_void_p * len(self.sims))(*[s.flow.h.value for s in self.sims])
+        h = C.c_void_p()
+        A.check(A.load().wl_batch_create(C.byref(h), hs, len(self.sims)))
+        self.h = h
+
+    def step(self):
+        n = len(self.sims)
+        dt, t0, t1 = (C.c_double * n)(), (C.c_double * n)(), (C.c_double * n)()
+        for k, s in enumerate(self.sims):
+            f = s.flow
+            dt[k] = f.Δt[-1]
+            t1[k] = float(np.sum(np.asarray(f.Δt, dtype=f.T), dtype=f.T))
+            t0[k] = float(f.T(t1[k]) - f.T(dt[k]))
+        out = (C.c_double * n)()
+        nit = (C.c_int16 * (2 * n))()
+        rc = A.load().wl_batch_step(self.h, dt, t0, t1, out, nit)
+        if rc == A.WL_ENAN:
+            raise FloatingPointError(A.load().wl_last_error().decode())
+        A.check(rc)
+        for k, s in enumerate(self.sims):
+            s.flow.Δt.append(float(s.flow.T(out[k])))
+            s.pois.n.extend([int(nit[2 * k]), int(nit[2 * k + 1])])
+        return self
+
+    def __del__(self):
+        try:
+            if getattr(self, "h", None):
+                A.load().wl_batch_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+
 def pressure_force(sim):
     """pressure_force(sim) (src/Metrics.jl:126-133): Σ p·nds(body) in Float64."""
     F = (C.c_double * 3)()

@@ -66,8 +66,8 @@ __device__ __forceinline__ int xcd_tile_id() {
   const int q = nb >> 3, rr = nb & 7, x = bid & 7, k = bid >> 3;
   return x * q + (x < rr ? x : rr) + k;
 }
-__device__ __forceinline__ void zt_cell(const ZT& t, int& i, int& j, int& z0, int& z1) {
-  const int lid = xcd_tile_id();
+// (x, y) cell of this thread and the z range of tile `lid` (a tile index of this launch's box)
+__device__ __forceinline__ void zt_cell_id(const ZT& t, int lid, int& i, int& j, int& z0, int& z1) {
   const int tx = lid % t.nbx, r1 = lid / t.nbx;
   const int ty = r1 % t.nby, tz = r1 / t.nby;
   i = t.b.lo[0] + tx * t.sx + (int)threadIdx.x;
@@ -75,6 +75,14 @@ __device__ __forceinline__ void zt_cell(const ZT& t, int& i, int& j, int& z0, in
   z0 = t.b.lo[2] + tz * t.zc;
   z1 = z0 + t.zc - 1 < t.b.hi[2] ? z0 + t.zc - 1 : t.b.hi[2];
 }
+__device__ __forceinline__ void zt_cell(const ZT& t, int& i, int& j, int& z0, int& z1) {
+  zt_cell_id(t, xcd_tile_id(), i, j, z0, z1);
+}
+// One launch for many simulations: block b of a batched launch works on tile blk[b].lid of
+// simulation blk[b].sim (the block table is built on the host, sims back to back).
+struct BBlk {
+  int sim, lid, pidx;  // simulation, tile, and the partial-sum slot the single launch would use
+};
 
 // ---------------------------------------------------------------------------------------------
 // wave64 + block reductions (deterministic for a fixed launch shape)

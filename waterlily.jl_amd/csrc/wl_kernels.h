@@ -246,17 +246,17 @@ __global__ void __launch_bounds__(64 * CBY) k_conv_bdim1(Geo g, ZT t, const T* _
 // velocity components at its own (x,y); plane z of the tile (+2/-1 halo in x and y) is staged in
 // LDS, so the 9 face fluxes of a cell cost no global address arithmetic.
 template <class T, int S, bool PRED, int D, bool PF>
-__global__ void __launch_bounds__(64 * CBY, sizeof(T) == 4 ? 4 : 2) k_conv_lds(Geo g, ZT t, const T* __restrict__ u, const T* __restrict__ u0,
+__device__ __forceinline__ void k_conv_lds_body(const Geo& g, const ZT& t, const T* __restrict__ u, const T* __restrict__ u0,
                                                        T* __restrict__ f, T* __restrict__ sigma,
                                                        const T* __restrict__ V, const uint8_t* __restrict__ vrows,
-                                                       T nu, T dt, Accel acc) {
+                                                       T nu, T dt, const Accel& acc, int lid) {
   constexpr int RX = 64 + 3, RY = CBY + 3;  // staged x range x0-2..x0+64, y range y0-2..y0+CBY
   constexpr int NH = 3 * RY + 3 * 64;       // halo cells per component and plane
   constexpr int NHT = (D * NH + 64 * CBY - 1) / (64 * CBY);  // halo loads per thread
   __shared__ T sU[3][RY][RX];
   __shared__ T sFy[3][CBY][64];
   int cx, cy, z0, z1;
-  zt_cell(t, cx, cy, z0, z1);
+  zt_cell_id(t, lid, cx, cy, z0, z1);
   const int tx = threadIdx.x, ty = threadIdx.y, tid = tx + 64 * ty;
   const int x0 = cx - tx, y0 = cy - ty;
   const int64_t n = g.n;
@@ -416,6 +416,13 @@ __global__ void __launch_bounds__(64 * CBY, sizeof(T) == 4 ? 4 : 2) k_conv_lds(G
     finish(z1, T3{{rpart[0], rpart[1], rpart[2]}}, T3{{W[0][2], W[1][2], W[2][2]}}, T3{{Vp[0], Vp[1], Vp[2]}});
   }
 }
+template <class T, int S, bool PRED, int D, bool PF>
+__global__ void __launch_bounds__(64 * CBY, sizeof(T) == 4 ? 4 : 2) k_conv_lds(Geo g, ZT t, const T* __restrict__ u, const T* __restrict__ u0,
+                                                       T* __restrict__ f, T* __restrict__ sigma,
+                                                       const T* __restrict__ V, const uint8_t* __restrict__ vrows,
+                                                       T nu, T dt, Accel acc) {
+  k_conv_lds_body<T, S, PRED, D, PF>(g, t, u, u0, f, sigma, V, vrows, nu, dt, acc, xcd_tile_id());
+}
 
 // BDIM! second half + scale_u!: u += μddn(μ₁,f) + V + μ₀·f on inside (src/Flow.jl:179,211-214).
 // PRED: u is u⁰·0 (mom_step!, :157) with u⁰ read from uold, a distinct buffer; CORR: u is updated
@@ -425,12 +432,12 @@ __global__ void __launch_bounds__(64 * CBY, sizeof(T) == 4 ? 4 : 2) k_conv_lds(G
 // (identical results for finite f). A wave covers 64 cells of one row, so the branch is uniform.
 // μ₀ is read through the finest level's LRows (μ₀ is its L) and V only in rows where it is not +0.
 template <class T, bool PRED>
-__global__ void __launch_bounds__(256) k_bdim2(Geo g, ZT t, T* __restrict__ u, const T* __restrict__ uold,
+__device__ __forceinline__ void k_bdim2_body(const Geo& g, const ZT& t, T* __restrict__ u, const T* __restrict__ uold,
                                                const T* __restrict__ f, const T* __restrict__ V,
-                                               LRows<T> mu0, const T* __restrict__ mu1,
-                                               const uint8_t* __restrict__ rows, const uint8_t* __restrict__ vrows) {
+                                               const LRows<T>& mu0, const T* __restrict__ mu1,
+                                               const uint8_t* __restrict__ rows, const uint8_t* __restrict__ vrows, int lid) {
   int c0, c1, z0, z1;
-  zt_cell(t, c0, c1, z0, z1);
+  zt_cell_id(t, lid, c0, c1, z0, z1);
   if (c0 > t.b.hi[0] || c1 > t.b.hi[1]) return;
   for (int c2 = z0; c2 <= z1; ++c2) {
   const int D = g.D;
@@ -449,6 +456,13 @@ __global__ void __launch_bounds__(256) k_bdim2(Geo g, ZT t, T* __restrict__ u, c
     else u[k] = (u[k] + inc) * T(0.5);
   }
   }
+}
+template <class T, bool PRED>
+__global__ void __launch_bounds__(256) k_bdim2(Geo g, ZT t, T* __restrict__ u, const T* __restrict__ uold,
+                                               const T* __restrict__ f, const T* __restrict__ V,
+                                               LRows<T> mu0, const T* __restrict__ mu1,
+                                               const uint8_t* __restrict__ rows, const uint8_t* __restrict__ vrows) {
+  k_bdim2_body<T, PRED>(g, t, u, uold, f, V, mu0, mu1, rows, vrows, xcd_tile_id());
 }
 
 // Uniform-row flags of a level's face coefficients (LRows). Pass 1, one wavefront per x-row
@@ -519,12 +533,12 @@ __device__ __forceinline__ int bc_map(int c, int N, bool per) {
   return c;
 }
 template <class T>
-__global__ void __launch_bounds__(256) k_bc(Geo g, T* __restrict__ a, int ncomp_vec, double U0, double U1,
-                                            double U2, int saveexit, int perdir) {
-  // blockIdx.y: component (vector field) ; blockIdx.z: face id 0..2D (2D+0..: normal plane 2)
+__device__ __forceinline__ void k_bc_body(const Geo& g, T* __restrict__ a, int ncomp_vec, double U0, double U1, double U2,
+                                          int saveexit, int perdir, int bx, int by, int bz) {
+  // by: component (vector field) ; bz: face id 0..2D (2D+0..: normal plane 2); bx: 256-cell chunk
   const int D = g.D;
-  const int i = ncomp_vec ? (int)blockIdx.y : -1;  // -1: scalar field (perBC!)
-  const int face = blockIdx.z;
+  const int i = ncomp_vec ? by : -1;  // -1: scalar field (perBC!)
+  const int face = bz;
   int d, cfix;
   if (face < 2 * D) { d = face >> 1; cfix = (face & 1) ? g.N[d] - 1 : 0; }
   else { if (i < 0) return; d = i; cfix = 1; }
@@ -532,7 +546,7 @@ __global__ void __launch_bounds__(256) k_bc(Geo g, T* __restrict__ a, int ncomp_
   int p0 = d == 0 ? 1 : 0, p1 = d == 2 ? 1 : 2;
   if (D == 2) p1 = -1;
   const int n0 = g.N[p0], n1 = p1 >= 0 ? g.N[p1] : 1;
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t t = (int64_t)bx * 256 + threadIdx.x;
   if (t >= (int64_t)n0 * n1) return;
   int c[3] = {0, 0, 0};
   c[d] = cfix;
@@ -556,6 +570,11 @@ __global__ void __launch_bounds__(256) k_bc(Geo g, T* __restrict__ a, int ncomp_
   }
   if (m[0] == c[0] && m[1] == c[1] && m[2] == c[2]) return;
   ai[at(g, c[0], c[1], c[2])] = ai[at(g, m[0], m[1], m[2])];
+}
+template <class T>
+__global__ void __launch_bounds__(256) k_bc(Geo g, T* __restrict__ a, int ncomp_vec, double U0, double U1,
+                                            double U2, int saveexit, int perdir) {
+  k_bc_body<T>(g, a, ncomp_vec, U0, U1, U2, saveexit, perdir, (int)blockIdx.x, (int)blockIdx.y, (int)blockIdx.z);
 }
 
 // exitBC! (src/core.jl:226-233) on the x-component of u: U = mean of u over the inflow plane
@@ -738,12 +757,12 @@ __device__ __forceinline__ void prolong_cell(const Geo& gf, const Geo& gc, int c
 // ---- residual! (src/Poisson.jl:92-98) fused with mom_project!'s z = div(u) and x .*= dt
 // (src/Flow.jl:225).  xin·scale is written to xout; with FLOW the source z is div(u).
 template <class T, bool FLOW>
-__global__ void __launch_bounds__(256) k_residual(Geo g, ZT t, LRows<T> LR, const T* __restrict__ xin,
+__device__ __forceinline__ void k_residual_body(const Geo& g, const ZT& t, const LRows<T>& LR, const T* __restrict__ xin,
                                                   T* __restrict__ xout, T scale, int write_x, T* __restrict__ z,
                                                   const T* __restrict__ u, T* __restrict__ r,
-                                                  double* __restrict__ partial) {
+                                                  double* __restrict__ partial, int lid, int pidx) {
   int c0, c1, z0, z1;
-  zt_cell(t, c0, c1, z0, z1);
+  zt_cell_id(t, lid, c0, c1, z0, z1);
   double acc = 0;
   if (c0 <= t.b.hi[0] && c1 <= t.b.hi[1]) {
     for (int c2 = z0; c2 <= z1; ++c2) {
@@ -768,16 +787,23 @@ __global__ void __launch_bounds__(256) k_residual(Geo g, ZT t, LRows<T> LR, cons
     }
   }
   const double s = block_sum(acc);
-  if (threadIdx.x == 0 && threadIdx.y == 0) partial[blockIdx.x] = s;
+  if (threadIdx.x == 0 && threadIdx.y == 0) partial[pidx] = s;
+}
+template <class T, bool FLOW>
+__global__ void __launch_bounds__(256) k_residual(Geo g, ZT t, LRows<T> LR, const T* __restrict__ xin,
+                                                  T* __restrict__ xout, T scale, int write_x, T* __restrict__ z,
+                                                  const T* __restrict__ u, T* __restrict__ r,
+                                                  double* __restrict__ partial) {
+  k_residual_body<T, FLOW>(g, t, LR, xin, xout, scale, write_x, z, u, r, partial, xcd_tile_id(), (int)blockIdx.x);
 }
 
 // r -= s (when |s| > 2eps) ; Σr² partials ; ϵ = r·iD (Jacobi!/GS init of the first V-cycle)
 template <class T>
-__global__ void __launch_bounds__(256) k_submean_l2(Geo g, ZT t, LRows<T> LR, T* __restrict__ r,
+__device__ __forceinline__ void k_submean_l2_body(const Geo& g, const ZT& t, const LRows<T>& LR, T* __restrict__ r,
                                                     T* __restrict__ e, const T* __restrict__ sdev,
-                                                    double* __restrict__ partial) {
+                                                    double* __restrict__ partial, int lid, int pidx) {
   int c0, c1, z0, z1;
-  zt_cell(t, c0, c1, z0, z1);
+  zt_cell_id(t, lid, c0, c1, z0, z1);
   double acc = 0;
   if (c0 <= t.b.hi[0] && c1 <= t.b.hi[1]) {
     const T s = sdev[0];
@@ -791,31 +817,42 @@ __global__ void __launch_bounds__(256) k_submean_l2(Geo g, ZT t, LRows<T> LR, T*
     }
   }
   const double tt = block_sum(acc);
-  if (threadIdx.x == 0 && threadIdx.y == 0) partial[blockIdx.x] = tt;
+  if (threadIdx.x == 0 && threadIdx.y == 0) partial[pidx] = tt;
+}
+template <class T>
+__global__ void __launch_bounds__(256) k_submean_l2(Geo g, ZT t, LRows<T> LR, T* __restrict__ r,
+                                                    T* __restrict__ e, const T* __restrict__ sdev,
+                                                    double* __restrict__ partial) {
+  k_submean_l2_body<T>(g, t, LR, r, e, sdev, partial, xcd_tile_id(), (int)blockIdx.x);
 }
 
 // ϵ = r·iD on inside (Jacobi!/GS-RB init, src/Poisson.jl:112,142)
 template <class T>
-__global__ void __launch_bounds__(256) k_eps_init(Geo g, ZT t, LRows<T> LR, const T* __restrict__ r,
-                                                  T* __restrict__ e) {
+__device__ __forceinline__ void k_eps_init_body(const Geo& g, const ZT& t, const LRows<T>& LR, const T* __restrict__ r,
+                                                  T* __restrict__ e, int lid) {
   int c0, c1, z0, z1;
-  zt_cell(t, c0, c1, z0, z1);
+  zt_cell_id(t, lid, c0, c1, z0, z1);
   if (c0 > t.b.hi[0] || c1 > t.b.hi[1]) return;
   for (int c2 = z0; c2 <= z1; ++c2) {
     const int64_t I = at(g, c0, c1, c2);
     e[I] = r[I] * inv_diag(diag_at<T>(g, lcell(g, LR, c0, c1, c2), I));
   }
 }
+template <class T>
+__global__ void __launch_bounds__(256) k_eps_init(Geo g, ZT t, LRows<T> LR, const T* __restrict__ r,
+                                                  T* __restrict__ e) {
+  k_eps_init_body<T>(g, t, LR, r, e, xcd_tile_id());
+}
 
 // increment! (src/Poisson.jl:100-104): r -= ω·Aϵ; x += ω·ϵ.  Optionally the Σr² partials (L₂ of
 // the finest level after a V-cycle, src/MultiLevelPoisson.jl:115).
 template <class T, bool L2>
-__global__ void __launch_bounds__(256) k_increment(Geo g, ZT t, LRows<T> LR, T* __restrict__ r,
+__device__ __forceinline__ void k_increment_body(const Geo& g, const ZT& t, const LRows<T>& LR, T* __restrict__ r,
                                                    T* __restrict__ x, const T* __restrict__ e, const T* __restrict__ wp,
-                                                   double* __restrict__ partial) {
+                                                   double* __restrict__ partial, int lid, int pidx) {
   const T w = *wp;
   int c0, c1, z0, z1;
-  zt_cell(t, c0, c1, z0, z1);
+  zt_cell_id(t, lid, c0, c1, z0, z1);
   double acc = 0;
   if (c0 <= t.b.hi[0] && c1 <= t.b.hi[1]) {
     for (int c2 = z0; c2 <= z1; ++c2) {
@@ -825,21 +862,34 @@ __global__ void __launch_bounds__(256) k_increment(Geo g, ZT t, LRows<T> LR, T* 
   }
   if (L2) {
     const double tt = block_sum(acc);
-    if (threadIdx.x == 0 && threadIdx.y == 0) partial[blockIdx.x] = tt;
+    if (threadIdx.x == 0 && threadIdx.y == 0) partial[pidx] = tt;
   }
+}
+template <class T, bool L2>
+__global__ void __launch_bounds__(256) k_increment(Geo g, ZT t, LRows<T> LR, T* __restrict__ r,
+                                                   T* __restrict__ x, const T* __restrict__ e, const T* __restrict__ wp,
+                                                   double* __restrict__ partial) {
+  k_increment_body<T, L2>(g, t, LR, r, x, e, wp, partial, xcd_tile_id(), (int)blockIdx.x);
 }
 
 // Jacobi!'s increment! (ω=1) on the fine level fused with restrict! and fill!(coarse.x,0)
 // (src/MultiLevelPoisson.jl:92-94,13-19,49): one thread per coarse inside cell (any coarsening mask).
 template <class T>
+__device__ __forceinline__ void k_inc_restrict_body(const Geo& gf, const Geo& gc, const ZT& t, int cm, const LRows<T>& Lf,
+                                                      T* __restrict__ rf, T* __restrict__ xf,
+                                                      const T* __restrict__ ef, const LRows<T>& Lc,
+                                                      T* __restrict__ rc, T* __restrict__ xc, T* __restrict__ ec, int lid) {
+  int c[3], z0, z1;
+  zt_cell_id(t, lid, c[0], c[1], z0, z1);
+  if (c[0] > t.b.hi[0] || c[1] > t.b.hi[1]) return;
+  for (c[2] = z0; c[2] <= z1; ++c[2]) inc_restrict_cell(gf, gc, cm, Lf, rf, xf, ef, Lc, rc, xc, ec, c);
+}
+template <class T>
 __global__ void __launch_bounds__(256) k_inc_restrict(Geo gf, Geo gc, ZT t, int cm, LRows<T> Lf,
                                                       T* __restrict__ rf, T* __restrict__ xf,
                                                       const T* __restrict__ ef, LRows<T> Lc,
                                                       T* __restrict__ rc, T* __restrict__ xc, T* __restrict__ ec) {
-  int c[3], z0, z1;
-  zt_cell(t, c[0], c[1], z0, z1);
-  if (c[0] > t.b.hi[0] || c[1] > t.b.hi[1]) return;
-  for (c[2] = z0; c[2] <= z1; ++c[2]) inc_restrict_cell(gf, gc, cm, Lf, rf, xf, ef, Lc, rc, xc, ec, c);
+  k_inc_restrict_body<T>(gf, gc, t, cm, Lf, rf, xf, ef, Lc, rc, xc, ec, xcd_tile_id());
 }
 
 // The same fused Jacobi! increment + restrict! + fill! when every direction is coarsened: one
@@ -848,14 +898,14 @@ __global__ void __launch_bounds__(256) k_inc_restrict(Geo gf, Geo gc, ZT t, int 
 // the reference's x-fastest order (src/MultiLevelPoisson.jl:13-19).  The tile's x and y ranges
 // are fine-cell ranges, its z range is in coarse planes.
 template <class T, int D>
-__global__ void __launch_bounds__(256) k_inc_restrict_full(Geo gf, Geo gc, ZT t, LRows<T> Lf,
+__device__ __forceinline__ void k_inc_restrict_full_body(const Geo& gf, const Geo& gc, const ZT& t, const LRows<T>& Lf,
                                                            T* __restrict__ rf, T* __restrict__ xf,
-                                                           const T* __restrict__ ef, LRows<T> Lc,
+                                                           const T* __restrict__ ef, const LRows<T>& Lc,
                                                            T* __restrict__ rc, T* __restrict__ xc,
-                                                           T* __restrict__ ec) {
+                                                           T* __restrict__ ec, int lid) {
   __shared__ T sR[2][BY][64];
   int cx, cy, z0, z1;
-  zt_cell(t, cx, cy, z0, z1);
+  zt_cell_id(t, lid, cx, cy, z0, z1);
   const int tx = threadIdx.x, ty = threadIdx.y;
   const bool in = cx <= t.b.hi[0] && cy <= t.b.hi[1];
   auto xv = [&](int64_t J) { return ef[J]; };
@@ -889,15 +939,28 @@ __global__ void __launch_bounds__(256) k_inc_restrict_full(Geo gf, Geo gc, ZT t,
     __syncthreads();
   }
 }
+template <class T, int D>
+__global__ void __launch_bounds__(256) k_inc_restrict_full(Geo gf, Geo gc, ZT t, LRows<T> Lf,
+                                                           T* __restrict__ rf, T* __restrict__ xf,
+                                                           const T* __restrict__ ef, LRows<T> Lc,
+                                                           T* __restrict__ rc, T* __restrict__ xc,
+                                                           T* __restrict__ ec) {
+  k_inc_restrict_full_body<T, D>(gf, gc, t, Lf, rf, xf, ef, Lc, rc, xc, ec, xcd_tile_id());
+}
 
 // GS-RB half sweep (src/Poisson.jl:124-128,144-146) over half_rangek
 template <class T>
-__global__ void __launch_bounds__(256) k_gs_half(Geo g, ZT t, LRows<T> LR, const T* __restrict__ r,
-                                                 T* __restrict__ e, int it) {
+__device__ __forceinline__ void k_gs_half_body(const Geo& g, const ZT& t, const LRows<T>& LR, const T* __restrict__ r,
+                                                 T* __restrict__ e, int it, int lid) {
   int c0, c1, z0, z1;
-  zt_cell(t, c0, c1, z0, z1);
+  zt_cell_id(t, lid, c0, c1, z0, z1);
   if (c0 > t.b.hi[0] || c1 > t.b.hi[1]) return;
   for (int kz = z0; kz <= z1; ++kz) gs_cell(g, LR, r, e, it, c0, c1, kz);
+}
+template <class T>
+__global__ void __launch_bounds__(256) k_gs_half(Geo g, ZT t, LRows<T> LR, const T* __restrict__ r,
+                                                 T* __restrict__ e, int it) {
+  k_gs_half_body<T>(g, t, LR, r, e, it, xcd_tile_id());
 }
 
 // Two GS-RB half sweeps in one pass (src/Poisson.jl:116-148, it = 2q−1 then 2q, 3D): colour A
@@ -965,15 +1028,22 @@ __global__ void __launch_bounds__(64 * GBY) k_gs_pair(Geo g, ZT t, LRows<T> LR, 
 
 // prolongate! + increment! on the fine level, then ϵ := r·iD (the fine level's GS-RB init)
 template <class T>
+__device__ __forceinline__ void k_prolong_inc_body(const Geo& gf, const Geo& gc, const ZT& t, int cm, int perdir,
+                                                     const LRows<T>& Lf, T* __restrict__ rf,
+                                                     T* __restrict__ xf, T* __restrict__ ef,
+                                                     const T* __restrict__ xc, const T* __restrict__ wp, int lid) {
+  const T w = *wp;
+  int c[3], z0, z1;
+  zt_cell_id(t, lid, c[0], c[1], z0, z1);
+  if (c[0] > t.b.hi[0] || c[1] > t.b.hi[1]) return;
+  for (c[2] = z0; c[2] <= z1; ++c[2]) prolong_cell(gf, gc, cm, perdir, Lf, rf, xf, ef, xc, w, c);
+}
+template <class T>
 __global__ void __launch_bounds__(256) k_prolong_inc(Geo gf, Geo gc, ZT t, int cm, int perdir,
                                                      LRows<T> Lf, T* __restrict__ rf,
                                                      T* __restrict__ xf, T* __restrict__ ef,
                                                      const T* __restrict__ xc, const T* __restrict__ wp) {
-  const T w = *wp;
-  int c[3], z0, z1;
-  zt_cell(t, c[0], c[1], z0, z1);
-  if (c[0] > t.b.hi[0] || c[1] > t.b.hi[1]) return;
-  for (c[2] = z0; c[2] <= z1; ++c[2]) prolong_cell(gf, gc, cm, perdir, Lf, rf, xf, ef, xc, w, c);
+  k_prolong_inc_body<T>(gf, gc, t, cm, perdir, Lf, rf, xf, ef, xc, wp, xcd_tile_id());
 }
 
 // ---- the coarse end of the V-cycle in one workgroup ----
@@ -1167,8 +1237,8 @@ __device__ __forceinline__ T smult(const SLev<T>& q, const T* __restrict__ v, in
 // The coarse tail with every level resident in LDS (L, x, r, ϵ copied in, D and iD formed once,
 // x, r, ϵ copied back at the end): same passes and per-cell arithmetic as k_mg_tail.
 template <class T>
-__global__ void __launch_bounds__(1024) k_mg_tail_lds(TailDesc<T> d, const T* __restrict__ wp, int bx, int by,
-                                                      int bz) {
+__device__ __forceinline__ void k_mg_tail_lds_body(const TailDesc<T>& d, const T* __restrict__ wp, int bx, int by,
+                                                   int bz) {
   const T w = *wp;
   extern __shared__ __align__(16) unsigned char tail_smem[];
   T* sb = reinterpret_cast<T*>(tail_smem);
@@ -1361,14 +1431,19 @@ __global__ void __launch_bounds__(1024) k_mg_tail_lds(TailDesc<T> d, const T* __
     }
   }
 }
+template <class T>
+__global__ void __launch_bounds__(1024) k_mg_tail_lds(TailDesc<T> d, const T* __restrict__ wp, int bx, int by,
+                                                      int bz) {
+  k_mg_tail_lds_body<T>(d, wp, bx, by, bz);
+}
 
 // mom_project! correction (src/Flow.jl:227-230): u[I,i] -= L[I,i]·(x[I]-x[I-e_i]) on inside,
 // p = x/dt on every cell.
 template <class T>
-__global__ void __launch_bounds__(256) k_correct(Geo g, ZT t, T* __restrict__ u, LRows<T> LR,
-                                                 const T* __restrict__ x, T* __restrict__ p, T dt) {
+__device__ __forceinline__ void k_correct_body(const Geo& g, const ZT& t, T* __restrict__ u, const LRows<T>& LR,
+                                                 const T* __restrict__ x, T* __restrict__ p, T dt, int lid) {
   int c[3], z0, z1;
-  zt_cell(t, c[0], c[1], z0, z1);
+  zt_cell_id(t, lid, c[0], c[1], z0, z1);
   if (c[0] > t.b.hi[0] || c[1] > t.b.hi[1]) return;
   for (c[2] = z0; c[2] <= z1; ++c[2]) {
     const int64_t I = at(g, c[0], c[1], c[2]);
@@ -1382,13 +1457,18 @@ __global__ void __launch_bounds__(256) k_correct(Geo g, ZT t, T* __restrict__ u,
     p[I] = xI / dt;
   }
 }
+template <class T>
+__global__ void __launch_bounds__(256) k_correct(Geo g, ZT t, T* __restrict__ u, LRows<T> LR,
+                                                 const T* __restrict__ x, T* __restrict__ p, T dt) {
+  k_correct_body<T>(g, t, u, LR, x, p, dt, xcd_tile_id());
+}
 
 // CFL (src/Flow.jl:234-244): σ = flux_out(u) on inside; block maxima of σ over every cell.
 template <class T>
-__global__ void __launch_bounds__(256) k_cfl(Geo g, ZT t, const T* __restrict__ u, T* __restrict__ sigma,
-                                             T* __restrict__ partial) {
+__device__ __forceinline__ void k_cfl_body(const Geo& g, const ZT& t, const T* __restrict__ u, T* __restrict__ sigma,
+                                             T* __restrict__ partial, int lid, int pidx) {
   int c[3], z0, z1;
-  zt_cell(t, c[0], c[1], z0, z1);
+  zt_cell_id(t, lid, c[0], c[1], z0, z1);
   T m = -T(INFINITY);
   if (c[0] <= t.b.hi[0] && c[1] <= t.b.hi[1]) {
     for (c[2] = z0; c[2] <= z1; ++c[2]) {
@@ -1411,7 +1491,12 @@ __global__ void __launch_bounds__(256) k_cfl(Geo g, ZT t, const T* __restrict__ 
     }
   }
   m = block_max<T>(m, -T(INFINITY));
-  if (threadIdx.x == 0 && threadIdx.y == 0) partial[blockIdx.x] = m;
+  if (threadIdx.x == 0 && threadIdx.y == 0) partial[pidx] = m;
+}
+template <class T>
+__global__ void __launch_bounds__(256) k_cfl(Geo g, ZT t, const T* __restrict__ u, T* __restrict__ sigma,
+                                             T* __restrict__ partial) {
+  k_cfl_body<T>(g, t, u, sigma, partial, xcd_tile_id(), (int)blockIdx.x);
 }
 
 // =============================================================================================
@@ -1854,3 +1939,5 @@ __global__ void __launch_bounds__(256) k_selftest_div6(unsigned long long* bad) 
 }
 
 }  // namespace wl
+
+#include "wl_batch.h"

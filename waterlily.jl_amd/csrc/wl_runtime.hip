@@ -923,28 +923,34 @@ struct FlowT : FlowBase {
     sync();
   }
 
-  template <int S, int DD>
-  void conv_bdim(bool pred, T dt, double t) {
+  // convection tiles: 63 output columns and CBY-1 output rows per block (overlapping)
+  ZT conv_zt() const {
     ZT ba = make_zt(box_all(g));
-    // overlapping tiles: 63 output columns and CBY-1 output rows per block
     ba.sx = 63;
     ba.sy = CBY - 1;
     ba.nbx = (g.N[0] + 62) / 63;
     ba.nby = (g.N[1] + CBY - 2) / (CBY - 1);
-    {
-      const int64_t per = (int64_t)ba.nbx * ba.nby;
-      ba.zc = (int)std::min<int64_t>(32, std::max<int64_t>(1, (int64_t)g.N[2] * per / (zt_target_default() / 2)));
-    }
+    const int64_t per = (int64_t)ba.nbx * ba.nby;
+    ba.zc = (int)std::min<int64_t>(32, std::max<int64_t>(1, (int64_t)g.N[2] * per / (zt_target_default() / 2)));
+    return ba;
+  }
+  // accelerate!: r += g(i,x,t) + ∂uBC/∂t (src/Flow.jl:69-73), uniform in space, formed in T
+  Accel accel(double t) const {
     Accel acc;
     acc.on = desc.has_g || desc.has_uBC_rate;
     for (int i = 0; i < 3; ++i) {
-      // accelerate!: r += g(i,x,t) + ∂uBC/∂t (src/Flow.jl:69-73), uniform in space, formed in T
       T a;
       if (desc.has_g && desc.has_uBC_rate) a = ((T)desc.g0[i] + (T)desc.g1[i] * (T)t) + (T)desc.uBC_rate[i];
       else if (desc.has_g) a = (T)desc.g0[i] + (T)desc.g1[i] * (T)t;
       else a = (T)desc.uBC_rate[i];
       acc.add[i] = (double)a;
     }
+    return acc;
+  }
+  template <int S, int DD>
+  void conv_bdim(bool pred, T dt, double t) {
+    const ZT ba = conv_zt();
+    const Accel acc = accel(t);
     const T nu = (T)desc.nu;
     const dim3 cblk(64, CBY, 1);
     {
@@ -1146,6 +1152,475 @@ struct FlowT : FlowBase {
     HIPCK(hipStreamSynchronize(st));
   }
   hipStream_t stream() const override { return st; }
+};
+
+// ---------------------------------------------------------------------------------------------
+// Batches: many uneven simulations stepped together, one launch per stage for all of them
+// (kb_* kernels, wl_batch.h).  Each member is an ordinary flow handle; the batch reads its fields
+// and multigrid levels in place, so a member gives the same bits as stepped alone.  Members share
+// D and dtype, use the plain solver, have no periodic directions and no exitBC, and have a coarse
+// tail that fits the LDS.
+// ---------------------------------------------------------------------------------------------
+static int xcd_tile_host(int bid, int nb) {  // the host copy of xcd_tile_id()
+  const int q = nb >> 3, rr = nb & 7, x = bid & 7, k = bid >> 3;
+  return x * q + (x < rr ? x : rr) + k;
+}
+struct BatchBase {
+  virtual ~BatchBase() {}
+  virtual int step(const double* dt, const double* t0, const double* t1, double* dt_next, int16_t* nit) = 0;
+};
+template <class T>
+struct BatchT : BatchBase {
+  std::vector<FlowT<T>*> F;
+  int ns = 0, D = 2, scheme = 0;
+  hipStream_t st = nullptr;
+  std::vector<void*> owned;  // device allocations of this batch
+  struct Launch {
+    const void* args = nullptr;
+    const BBlk* blk = nullptr;
+    int nblk = 0;
+  };
+  std::vector<int> Lt;  // first tail level per member
+  int Cmax = 0;         // max over members of Lt
+  // static launches per level
+  std::vector<Launch> inc_full, inc_gen, gs, incr, pro;
+  Launch incr0, eps0, sub0;
+  const ATail<T>* tails = nullptr;
+  const int* tail_sims = nullptr;
+  size_t tail_lds = 0;
+  const AFin *fin_res = nullptr, *fin_l2 = nullptr;
+  const ACtl* ctl = nullptr;
+  int *act = nullptr, *cnt = nullptr;
+  T* cflout = nullptr;
+  int *hact = nullptr, *hcnt = nullptr;
+  T* hcfl = nullptr;
+  // per-step launches: tables static, argument arrays staged each step
+  BBlk *blk_conv = nullptr, *blk_bdim = nullptr, *blk_bc = nullptr, *blk_res = nullptr, *blk_corr = nullptr,
+       *blk_cfl = nullptr;
+  int nb_conv = 0, nb_bdim = 0, nb_bc = 0, nb_res = 0, nb_corr = 0, nb_cfl = 0;
+  char* dyn = nullptr;   // device arguments of one step
+  char* hdyn = nullptr;  // pinned staging
+  size_t dyn_bytes = 0;
+
+  template <class U>
+  U* upload(const std::vector<U>& v) {
+    U* p = dalloc<U>((int64_t)std::max<size_t>(v.size(), 1));
+    owned.push_back(p);
+    if (!v.empty()) HIPCK(hipMemcpy(p, v.data(), v.size() * sizeof(U), hipMemcpyHostToDevice));
+    return p;
+  }
+  // block table: member s contributes nb blocks, mapped as its own launch maps them
+  static void add_blocks(std::vector<BBlk>& t, int s, int64_t nb, bool xcd = true) {
+    for (int b = 0; b < (int)nb; ++b) t.push_back(BBlk{s, xcd ? xcd_tile_host(b, (int)nb) : b, b});
+  }
+  template <class A>
+  Launch make_launch(const std::vector<A>& args, const std::vector<BBlk>& blk) {
+    Launch L;
+    L.nblk = (int)blk.size();
+    if (L.nblk == 0) return L;
+    L.args = upload(args);
+    L.blk = upload(blk);
+    return L;
+  }
+
+  BatchT(const std::vector<FlowT<T>*>& flows) : F(flows) {
+    ns = (int)F.size();
+    D = F[0]->D;
+    scheme = F[0]->desc.scheme;
+    HIPCK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    for (auto* f : F) {
+      if (f->D != D) throw WlErr(WL_EINVAL, "batch members must have the same dimension");
+      if (f->desc.scheme != scheme) throw WlErr(WL_EINVAL, "batch members must use the same convection scheme");
+      if (f->desc.perdir_mask || f->desc.exit_bc) throw WlErr(WL_EINVAL, "batch members: no periodic directions, no exitBC");
+      if (f->mode != WL_SOLVER_PLAIN) throw WlErr(WL_EINVAL, "batch members use the plain solver");
+      f->sync();
+      if (f->mu1_dirty) f->mu1_rows();
+      f->mg.ensure_lrows();
+      f->sync();
+      bool in_lds = false;
+      const int lt = f->mg.tail_start(&in_lds);
+      if (lt <= 0 || !in_lds) throw WlErr(WL_EINVAL, "batch members need a multigrid tail that fits the LDS");
+      Lt.push_back(lt);
+      Cmax = std::max(Cmax, lt);
+    }
+    build_static();
+  }
+  ~BatchT() override {
+    if (st) (void)hipStreamSynchronize(st);
+    for (void* p : owned) dfree(p);
+    dfree(dyn);
+    if (hdyn) (void)hipHostFree(hdyn);
+    if (hact) (void)hipHostFree(hact);
+    if (hcnt) (void)hipHostFree(hcnt);
+    if (hcfl) (void)hipHostFree(hcfl);
+    if (st) (void)hipStreamDestroy(st);
+  }
+
+  ALev<T> lev_args(FlowT<T>* f, int l, const ZT& t) {
+    MG<T>& mg = f->mg;
+    ALev<T> a;
+    std::memset(&a, 0, sizeof(a));
+    Level<T>& p = mg.lv[l];
+    a.g = p.g;
+    a.t = t;
+    a.cm = p.cmask;
+    a.perdir = 0;
+    a.L = mg.lr(l);
+    a.x = p.x; a.r = p.r; a.e = p.e;
+    if (l + 1 < (int)mg.lv.size()) {
+      Level<T>& c = mg.lv[l + 1];
+      a.gc = c.g;
+      a.Lc = mg.lr(l + 1);
+      a.xc = c.x; a.rc = c.r; a.ec = c.e;
+    }
+    a.wp = (const T*)f->B.tsc + 3;
+    a.partial = f->B.partial;
+    a.sdev = (const T*)f->B.tsc;
+    return a;
+  }
+
+  void build_static() {
+    inc_full.resize(Cmax); inc_gen.resize(Cmax); gs.resize(Cmax + 1); incr.resize(Cmax + 1); pro.resize(Cmax + 1);
+    for (int l = 0; l < Cmax; ++l) {  // down-leg l -> l+1
+      std::vector<ALev<T>> af, ag;
+      std::vector<BBlk> bf, bg;
+      for (int s = 0; s < ns; ++s) {
+        if (Lt[s] <= l) { af.push_back(ALev<T>{}); ag.push_back(ALev<T>{}); continue; }
+        MG<T>& mg = F[s]->mg;
+        Level<T>& f = mg.lv[l];
+        Level<T>& c = mg.lv[l + 1];
+        const int full = (1 << f.g.D) - 1;
+        if (f.cmask == full) {
+          Box b = box_inside(f.g);
+          if (f.g.D == 3) { b.lo[2] = 1; b.hi[2] = c.g.N[2] - 2; }
+          const ZT t = make_zt(b, 2 * zt_target_default());
+          af.push_back(lev_args(F[s], l, t));
+          ag.push_back(ALev<T>{});
+          add_blocks(bf, s, zt_blocks(t));
+        } else {
+          const ZT t = make_zt(box_inside(c.g), 2 * zt_target_default());
+          ag.push_back(lev_args(F[s], l, t));
+          af.push_back(ALev<T>{});
+          add_blocks(bg, s, zt_blocks(t));
+        }
+      }
+      inc_full[l] = make_launch(af, bf);
+      inc_gen[l] = make_launch(ag, bg);
+    }
+    for (int l = 0; l <= Cmax; ++l) {
+      std::vector<ALev<T>> ags, agi, agp;
+      std::vector<BBlk> bgs, bgi, bgp;
+      for (int s = 0; s < ns; ++s) {
+        const bool smooth = l == 0 || l < Lt[s];  // levels smoothed outside the tail
+        const bool prol = l < Lt[s];              // prolongation into level l from l+1
+        const Geo& g = F[s]->mg.lv[l < (int)F[s]->mg.lv.size() ? l : 0].g;
+        if (smooth) {
+          const ZT th = make_zt(box_half(g));
+          ags.push_back(lev_args(F[s], l, th));
+          add_blocks(bgs, s, zt_blocks(th));
+          const ZT ti = make_zt(box_inside(g));
+          agi.push_back(lev_args(F[s], l, ti));
+          add_blocks(bgi, s, zt_blocks(ti));
+        } else {
+          ags.push_back(ALev<T>{}); agi.push_back(ALev<T>{});
+        }
+        if (prol) {
+          const ZT tp = make_zt(box_inside(g), 2 * zt_target_default());
+          agp.push_back(lev_args(F[s], l, tp));
+          add_blocks(bgp, s, zt_blocks(tp));
+        } else {
+          agp.push_back(ALev<T>{});
+        }
+      }
+      gs[l] = make_launch(ags, bgs);
+      incr[l] = make_launch(agi, bgi);
+      pro[l] = make_launch(agp, bgp);
+    }
+    incr0 = incr[0];
+    {
+      std::vector<ALev<T>> a;
+      std::vector<BBlk> b;
+      for (int s = 0; s < ns; ++s) {
+        const ZT t = make_zt(box_inside(F[s]->g));
+        a.push_back(lev_args(F[s], 0, t));
+        add_blocks(b, s, zt_blocks(t));
+      }
+      eps0 = make_launch(a, b);
+      sub0 = eps0;
+    }
+    {  // coarse tails (MG::launch_tail)
+      std::vector<ATail<T>> a;
+      std::vector<int> sims;
+      for (int s = 0; s < ns; ++s) {
+        MG<T>& mg = F[s]->mg;
+        ATail<T> t;
+        std::memset(&t, 0, sizeof(t));
+        t.d.n = (int)mg.lv.size() - Lt[s];
+        t.d.perdir = 0;
+        t.d.exact = 0;
+        int64_t off = 0;
+        for (int k = 0; k < t.d.n; ++k) {
+          const Level<T>& q = mg.lv[Lt[s] + k];
+          t.d.lv[k].g = q.g; t.d.lv[k].L = q.L; t.d.lv[k].x = q.x; t.d.lv[k].r = q.r; t.d.lv[k].e = q.e;
+          t.d.lv[k].cmask = q.cmask;
+          t.d.lv[k].off = off;
+          off += (int64_t)(q.g.D + 5) * q.g.n;
+        }
+        tail_lds = std::max(tail_lds, (size_t)(off * sizeof(T)));
+        t.wp = (const T*)F[s]->B.tsc + 3;
+        const bool d3 = mg.lv[Lt[s]].g.D == 3;
+        t.bx = d3 ? 16 : 32; t.by = d3 ? 8 : 32; t.bz = d3 ? 8 : 1;
+        a.push_back(t);
+        sims.push_back(s);
+      }
+      tails = upload(a);
+      tail_sims = upload(sims);
+      static bool attr = false;
+      if (!attr) {
+        HIPCK(hipFuncSetAttribute((const void*)kb_mg_tail<T>, hipFuncAttributeMaxDynamicSharedMemorySize, TAIL_LDS_BYTES));
+        attr = true;
+      }
+    }
+    {  // reduction finishes and solver control
+      std::vector<AFin> fr, fl;
+      std::vector<ACtl> c;
+      for (int s = 0; s < ns; ++s) {
+        FlowT<T>* f = F[s];
+        const int64_t nb = zt_blocks(make_zt(box_inside(f->g)));
+        fr.push_back(AFin{f->B.partial, nb, f->B.dsc + 0, f->B.tsc, 1, (double)ncells_inside(f->g)});
+        fl.push_back(AFin{f->B.partial, nb, f->B.dsc + 1, f->B.tsc, 0, 1.0});
+        c.push_back(ACtl{f->B.dsc, f->B.tsc, f->tol, f->itmx});
+      }
+      fin_res = upload(fr);
+      fin_l2 = upload(fl);
+      ctl = upload(c);
+      act = upload(std::vector<int>(ns, 0));
+      cnt = upload(std::vector<int>(ns, 0));
+      cflout = upload(std::vector<T>(ns, T(0)));
+      HIPCK(hipHostMalloc((void**)&hact, ns * sizeof(int), hipHostMallocDefault));
+      HIPCK(hipHostMalloc((void**)&hcnt, ns * sizeof(int), hipHostMallocDefault));
+      HIPCK(hipHostMalloc((void**)&hcfl, ns * sizeof(T), hipHostMallocDefault));
+    }
+    {  // per-step launch tables
+      std::vector<BBlk> bc_, bb, bbc, br, bco, bcf;
+      for (int s = 0; s < ns; ++s) {
+        FlowT<T>* f = F[s];
+        add_blocks(bc_, s, zt_blocks(f->conv_zt()));
+        add_blocks(bb, s, zt_blocks(make_zt(box_inside(f->g), 2 * zt_target_default())));
+        add_blocks(bbc, s, (int64_t)bc_gx(f->g) * f->g.D * (2 * f->g.D + 1), false);
+        add_blocks(br, s, zt_blocks(make_zt(box_inside(f->g))));
+        add_blocks(bco, s, zt_blocks(make_zt(box_all(f->g), 2 * zt_target_default())));
+        add_blocks(bcf, s, zt_blocks(make_zt(box_all(f->g))));
+      }
+      blk_conv = upload(bc_); nb_conv = (int)bc_.size();
+      blk_bdim = upload(bb); nb_bdim = (int)bb.size();
+      blk_bc = upload(bbc); nb_bc = (int)bbc.size();
+      blk_res = upload(br); nb_res = (int)br.size();
+      blk_corr = upload(bco); nb_corr = (int)bco.size();
+      blk_cfl = upload(bcf); nb_cfl = (int)bcf.size();
+    }
+    // per-step argument arrays: conv ×2, bdim ×2, bc ×2 (t1), residual ×2, correct ×2, cfl
+    dyn_bytes = ns * (2 * sizeof(AConv<T>) + 2 * sizeof(ABdim<T>) + sizeof(ABc<T>) + 2 * sizeof(ARes<T>) +
+                      2 * sizeof(ACorr<T>) + sizeof(ACfl<T>)) + 16 * 64;
+    dyn = dalloc<char>((int64_t)dyn_bytes);
+    HIPCK(hipHostMalloc((void**)&hdyn, dyn_bytes, hipHostMallocDefault));
+  }
+  static int bc_gx(const Geo& g) {  // MG::bc_vec grid x
+    int maxplane = 0;
+    for (int d = 0; d < g.D; ++d) {
+      int p = 1;
+      for (int q = 0; q < g.D; ++q) if (q != d) p *= g.N[q];
+      maxplane = std::max(maxplane, p);
+    }
+    return (maxplane + 255) / 256;
+  }
+
+  int step(const double* dt_d, const double* t0, const double* t1, double* dt_next, int16_t* nit) override {
+    for (auto* f : F) {
+      f->sync();
+      if (f->mu1_dirty) f->mu1_rows();
+      f->mg.ensure_lrows();
+      f->sync();
+    }
+    // stage the step's arguments; u/u⁰ follow the predictor's buffer swap
+    size_t off = 0;
+    auto take = [&](size_t bytes) { char* p = hdyn + off; off += (bytes + 63) & ~size_t(63); return p; };
+    AConv<T>* cp = (AConv<T>*)take(ns * sizeof(AConv<T>));
+    AConv<T>* cc = (AConv<T>*)take(ns * sizeof(AConv<T>));
+    ABdim<T>* bp = (ABdim<T>*)take(ns * sizeof(ABdim<T>));
+    ABdim<T>* bcr = (ABdim<T>*)take(ns * sizeof(ABdim<T>));
+    ABc<T>* bcv = (ABc<T>*)take(ns * sizeof(ABc<T>));
+    ARes<T>* r1 = (ARes<T>*)take(ns * sizeof(ARes<T>));
+    ARes<T>* r2 = (ARes<T>*)take(ns * sizeof(ARes<T>));
+    ACorr<T>* k1 = (ACorr<T>*)take(ns * sizeof(ACorr<T>));
+    ACorr<T>* k2 = (ACorr<T>*)take(ns * sizeof(ACorr<T>));
+    ACfl<T>* cf = (ACfl<T>*)take(ns * sizeof(ACfl<T>));
+    if (off > dyn_bytes) throw WlErr(WL_EINVAL, "batch argument staging overflow");
+    for (int s = 0; s < ns; ++s) {
+      FlowT<T>* f = F[s];
+      const T dt = (T)dt_d[s];
+      T* uA = f->u;   // u at the start of the step: convected by the predictor, then u⁰
+      T* uB = f->u0;  // becomes u after the predictor's swap
+      const T nu = (T)f->desc.nu;
+      const ZT tc = f->conv_zt();
+      cp[s] = AConv<T>{f->g, tc, uA, uB, f->f, f->sigma, f->V, f->vrows(), nu, dt, f->accel(t0[s])};
+      cc[s] = AConv<T>{f->g, tc, uB, uA, f->f, f->sigma, f->V, f->vrows(), nu, dt, f->accel(t1[s])};
+      const ZT tb = make_zt(box_inside(f->g), 2 * zt_target_default());
+      bp[s] = ABdim<T>{f->g, tb, uB, uA, f->f, f->V, f->mg.lr(0), f->mu1, f->rows, f->vrows()};
+      bcr[s] = ABdim<T>{f->g, tb, uB, uA, f->f, f->V, f->mg.lr(0), f->mu1, f->rows, f->vrows()};
+      double Ub[3];
+      f->Ut(t1[s], Ub);
+      bcv[s] = ABc<T>{f->g, uB, {Ub[0], Ub[1], f->g.D > 2 ? Ub[2] : 0.0}, 0, 0, bc_gx(f->g), f->g.D};
+      const ZT ti = make_zt(box_inside(f->g));
+      r1[s] = ARes<T>{f->g, ti, f->mg.lr(0), f->p, f->xw, T(1) * dt, f->sigma, uB, f->mg.lv[0].r, f->B.partial};
+      r2[s] = ARes<T>{f->g, ti, f->mg.lr(0), f->p, f->xw, T(0.5) * dt, f->sigma, uB, f->mg.lv[0].r, f->B.partial};
+      const ZT ta = make_zt(box_all(f->g), 2 * zt_target_default());
+      k1[s] = ACorr<T>{f->g, ta, uB, f->mg.lr(0), f->xw, f->p, T(1) * dt};
+      k2[s] = ACorr<T>{f->g, ta, uB, f->mg.lr(0), f->xw, f->p, T(0.5) * dt};
+      const ZT tf = make_zt(box_all(f->g));
+      cf[s] = ACfl<T>{f->g, tf, uB, f->sigma, (T*)f->B.partial, zt_blocks(tf)};
+    }
+    HIPCK(hipMemcpyAsync(dyn, hdyn, off, hipMemcpyHostToDevice, st));
+    auto dv = [&](const void* hp) { return (const void*)(dyn + ((const char*)hp - hdyn)); };
+    // predictor (src/Flow.jl:190-196)
+    conv(true, (const AConv<T>*)dv(cp));
+    for (auto* f : F) std::swap(f->u, f->u0);
+    hipLaunchKernelGGL((kb_bdim2<T, true>), dim3(nb_bdim), kBlock, 0, st, (const ABdim<T>*)dv(bp), blk_bdim);
+    KCK();
+    hipLaunchKernelGGL(kb_bc<T>, dim3(nb_bc), dim3(256), 0, st, (const ABc<T>*)dv(bcv), blk_bc);
+    KCK();
+    std::vector<int> n1 = project((const ARes<T>*)dv(r1), (const ACorr<T>*)dv(k1), (const ABc<T>*)dv(bcv));
+    // corrector (src/Flow.jl:205-210)
+    conv(false, (const AConv<T>*)dv(cc));
+    hipLaunchKernelGGL((kb_bdim2<T, false>), dim3(nb_bdim), kBlock, 0, st, (const ABdim<T>*)dv(bcr), blk_bdim);
+    KCK();
+    hipLaunchKernelGGL(kb_bc<T>, dim3(nb_bc), dim3(256), 0, st, (const ABc<T>*)dv(bcv), blk_bc);
+    KCK();
+    std::vector<int> n2 = project((const ARes<T>*)dv(r2), (const ACorr<T>*)dv(k2), (const ABc<T>*)dv(bcv));
+    // CFL (src/Flow.jl:234-237)
+    hipLaunchKernelGGL(kb_cfl<T>, dim3(nb_cfl), kBlock, 0, st, (const ACfl<T>*)dv(cf), blk_cfl);
+    KCK();
+    hipLaunchKernelGGL(kb_finish_max<T>, dim3(ns), dim3(256), 0, st, (const ACfl<T>*)dv(cf), cflout);
+    KCK();
+    HIPCK(hipMemcpyAsync(hcfl, cflout, ns * sizeof(T), hipMemcpyDeviceToHost, st));
+    HIPCK(hipStreamSynchronize(st));
+    int rc = 0;
+    for (int s = 0; s < ns; ++s) {
+      const T m = hcfl[s];
+      const T nu = (T)F[s]->desc.nu;
+      const T inv = T(1) / (m + T(5) * nu);
+      const T dtn = std::isnan(inv) ? inv : std::min(T(10), inv);
+      dt_next[s] = (double)dtn;
+      nit[2 * s] = (int16_t)n1[s];
+      nit[2 * s + 1] = (int16_t)n2[s];
+      if (!std::isfinite(dtn)) rc = set_err(WL_ENAN, "non-finite CFL time step (NaN/Inf in u)");
+    }
+    return rc;
+  }
+
+  void conv(bool pred, const AConv<T>* a) {
+    const dim3 cblk(64, CBY, 1);
+#define WL_BCONV(SS)                                                                                        \
+  if (D == 3) {                                                                                             \
+    if (pred) hipLaunchKernelGGL((kb_conv<T, SS, true, 3>), dim3(nb_conv), cblk, 0, st, a, blk_conv);       \
+    else hipLaunchKernelGGL((kb_conv<T, SS, false, 3>), dim3(nb_conv), cblk, 0, st, a, blk_conv);           \
+  } else {                                                                                                  \
+    if (pred) hipLaunchKernelGGL((kb_conv<T, SS, true, 2>), dim3(nb_conv), cblk, 0, st, a, blk_conv);       \
+    else hipLaunchKernelGGL((kb_conv<T, SS, false, 2>), dim3(nb_conv), cblk, 0, st, a, blk_conv);           \
+  }
+    switch (scheme) {
+      case WL_QUICK: WL_BCONV(WL_QUICK) break;
+      case WL_VANLEER: WL_BCONV(WL_VANLEER) break;
+      default: WL_BCONV(WL_CDS) break;
+    }
+#undef WL_BCONV
+    KCK();
+  }
+
+  void gs_sweeps(int l) {
+    const Launch& L = gs[l];
+    for (int it = 1; it <= 4; ++it) {
+      hipLaunchKernelGGL(kb_gs_half<T>, dim3(L.nblk), kBlock, 0, st, (const ALev<T>*)L.args, L.blk, (const int*)act, it);
+      KCK();
+    }
+  }
+  void vcycle() {
+    for (int l = 0; l < Cmax; ++l) {
+      if (inc_full[l].nblk) {
+        if (D == 3)
+          hipLaunchKernelGGL((kb_inc_restrict_full<T, 3>), dim3(inc_full[l].nblk), kBlock, 0, st,
+                             (const ALev<T>*)inc_full[l].args, inc_full[l].blk, (const int*)act);
+        else
+          hipLaunchKernelGGL((kb_inc_restrict_full<T, 2>), dim3(inc_full[l].nblk), kBlock, 0, st,
+                             (const ALev<T>*)inc_full[l].args, inc_full[l].blk, (const int*)act);
+        KCK();
+      }
+      if (inc_gen[l].nblk) {
+        hipLaunchKernelGGL(kb_inc_restrict<T>, dim3(inc_gen[l].nblk), kBlock, 0, st, (const ALev<T>*)inc_gen[l].args,
+                           inc_gen[l].blk, (const int*)act);
+        KCK();
+      }
+    }
+    hipLaunchKernelGGL(kb_mg_tail<T>, dim3(ns), dim3(1024), tail_lds, st, tails, tail_sims, (const int*)act);
+    KCK();
+    for (int c = Cmax; c >= 1; --c) {  // members with c < Lt: smooth!(c); members with c <= Lt: prolongate!
+      if (gs[c].nblk) {
+        gs_sweeps(c);
+        hipLaunchKernelGGL((kb_increment<T, false>), dim3(incr[c].nblk), kBlock, 0, st, (const ALev<T>*)incr[c].args,
+                           incr[c].blk, (const int*)act);
+        KCK();
+      }
+      const Launch& P = pro[c - 1];
+      if (P.nblk) {
+        hipLaunchKernelGGL(kb_prolong<T>, dim3(P.nblk), kBlock, 0, st, (const ALev<T>*)P.args, P.blk, (const int*)act);
+        KCK();
+      }
+    }
+  }
+  // mom_project! (src/Flow.jl:223-232) for every member; returns the V-cycle counts
+  std::vector<int> project(const ARes<T>* ra, const ACorr<T>* ka, const ABc<T>* bca) {
+    hipLaunchKernelGGL(kb_residual<T>, dim3(nb_res), kBlock, 0, st, ra, blk_res);
+    KCK();
+    hipLaunchKernelGGL(kb_finish_sum<T>, dim3(ns), dim3(256), 0, st, fin_res, (const int*)nullptr);
+    KCK();
+    hipLaunchKernelGGL(kb_submean<T>, dim3(sub0.nblk), kBlock, 0, st, (const ALev<T>*)sub0.args, sub0.blk);
+    KCK();
+    hipLaunchKernelGGL(kb_finish_sum<T>, dim3(ns), dim3(256), 0, st, fin_l2, (const int*)nullptr);
+    KCK();
+    // solver! (src/MultiLevelPoisson.jl:108-127) for all members at once
+    hipLaunchKernelGGL(kb_mg_init<T>, dim3((ns + 63) / 64), dim3(64), 0, st, ctl, ns, act, cnt);
+    KCK();
+    int itmax = 0;
+    for (auto* f : F) itmax = std::max(itmax, f->itmx);
+    for (int it = 0; it < itmax; ++it) {
+      if (it > 0) {
+        hipLaunchKernelGGL(kb_eps_init<T>, dim3(eps0.nblk), kBlock, 0, st, (const ALev<T>*)eps0.args, eps0.blk,
+                           (const int*)act);
+        KCK();
+      }
+      vcycle();
+      gs_sweeps(0);
+      hipLaunchKernelGGL((kb_increment<T, true>), dim3(incr0.nblk), kBlock, 0, st, (const ALev<T>*)incr0.args,
+                         incr0.blk, (const int*)act);
+      KCK();
+      hipLaunchKernelGGL(kb_finish_sum<T>, dim3(ns), dim3(256), 0, st, fin_l2, (const int*)act);
+      KCK();
+      hipLaunchKernelGGL(kb_mg_control<T>, dim3((ns + 63) / 64), dim3(64), 0, st, ctl, ns, act, cnt);
+      KCK();
+      HIPCK(hipMemcpyAsync(hact, act, ns * sizeof(int), hipMemcpyDeviceToHost, st));
+      HIPCK(hipStreamSynchronize(st));
+      bool any = false;
+      for (int s = 0; s < ns; ++s) any |= hact[s] != 0;
+      if (!any) break;
+    }
+    HIPCK(hipMemcpyAsync(hcnt, cnt, ns * sizeof(int), hipMemcpyDeviceToHost, st));
+    hipLaunchKernelGGL(kb_correct<T>, dim3(nb_corr), kBlock, 0, st, ka, blk_corr);
+    KCK();
+    hipLaunchKernelGGL(kb_bc<T>, dim3(nb_bc), dim3(256), 0, st, bca, blk_bc);
+    KCK();
+    HIPCK(hipStreamSynchronize(st));
+    return std::vector<int>(hcnt, hcnt + ns);
+  }
 };
 
 struct wl_flow_s {
@@ -1478,6 +1953,45 @@ int wl_meanflow_copy_to_flow(wl_meanflow_t m, wl_flow_t flow) {
   HIPCK(hipMemcpyAsync(flow->impl->field_ptr(WL_U), m->U, m->n * m->D * m->es(), hipMemcpyDeviceToDevice, st));
   HIPCK(hipMemcpyAsync(flow->impl->field_ptr(WL_P), m->P, m->n * m->es(), hipMemcpyDeviceToDevice, st));
   HIPCK(hipStreamSynchronize(st));
+  API_END
+}
+struct wl_batch_s {
+  std::unique_ptr<BatchBase> impl;
+};
+int wl_batch_create(wl_batch_t* out, const wl_flow_t* flows, int nflows) {
+  API_BEGIN
+  if (!out || !flows || nflows < 1) throw WlErr(WL_EINVAL, "a batch needs at least one flow");
+  const int dt = flows[0]->impl->dtype;
+  auto b = std::make_unique<wl_batch_s>();
+  if (dt == WL_F64) {
+    std::vector<FlowT<double>*> v;
+    for (int k = 0; k < nflows; ++k) {
+      if (!flows[k] || flows[k]->impl->dtype != dt) throw WlErr(WL_EINVAL, "batch members must share the dtype");
+      v.push_back(static_cast<FlowT<double>*>(flows[k]->impl.get()));
+    }
+    b->impl = std::make_unique<BatchT<double>>(v);
+  } else {
+    std::vector<FlowT<float>*> v;
+    for (int k = 0; k < nflows; ++k) {
+      if (!flows[k] || flows[k]->impl->dtype != dt) throw WlErr(WL_EINVAL, "batch members must share the dtype");
+      v.push_back(static_cast<FlowT<float>*>(flows[k]->impl.get()));
+    }
+    b->impl = std::make_unique<BatchT<float>>(v);
+  }
+  *out = b.release();
+  API_END
+}
+int wl_batch_destroy(wl_batch_t b) {
+  API_BEGIN
+  delete b;
+  API_END
+}
+int wl_batch_step(wl_batch_t b, const double* dt, const double* t0, const double* t1, double* dt_next,
+                  int16_t* n_iter) {
+  API_BEGIN
+  if (!b || !dt || !t0 || !t1 || !dt_next || !n_iter) throw WlErr(WL_EINVAL, "null argument");
+  const int rc = b->impl->step(dt, t0, t1, dt_next, n_iter);
+  if (rc) return rc;
   API_END
 }
 int wl_flow_profile(wl_flow_t h, int enable) {
