@@ -59,6 +59,60 @@ def run_gpu(wl, n, R, center, Re, L, steps, warmup, profile_steps=None, mode=Non
                 profiled_ms_per_step=None if psec is None else 1e3 * psec)
 
 
+def c5_members(wl, n=64, seed0=0):
+    """64 uneven 2D flows (seeds 0..63): grid from {64x64, 128x64, 256x128, 512x256}, one circle or
+    ellipse (radius U[m/16, m/6], m the smaller side, centred at (N0/4, N1/2)), Re U[100, 1000] on
+    the diameter (nu = 2R/Re), U = (1, 0), Float32."""
+    sizes = [(64, 64), (128, 64), (256, 128), (512, 256)]
+    sims = []
+    for k in range(n):
+        rng = np.random.default_rng(seed0 + k)
+        N = sizes[int(rng.integers(0, 4))]
+        m = min(N)
+        R = float(rng.uniform(m / 16, m / 6))
+        c = (N[0] / 4, N[1] / 2)
+        body = wl.Sphere(c, R) if rng.uniform() < 0.5 else wl.Ellipsoid(c, (R, float(rng.uniform(0.4, 1.0)) * R))
+        Re = float(rng.uniform(100, 1000))
+        sims.append(wl.Simulation(N, (1, 0), 2 * R, ν=2 * R / Re, body=body, T=np.float32))
+    return sims
+
+
+def run_c5(wl, steps, warmup):
+    """Aggregate throughput of the 64 uneven flows stepped as one batch, and stepped one at a time."""
+    sims = c5_members(wl)
+    cells = sum(int(np.prod(s.flow.N)) for s in sims)
+    b = wl.Batch(sims)
+    for _ in range(warmup):
+        b.step()
+    lat = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        a = time.perf_counter()
+        b.step()
+        lat.append(time.perf_counter() - a)
+    sec = time.perf_counter() - t0
+    nv = np.mean([np.mean(s.pois.n[-2 * steps:]) for s in sims])
+    del b
+    solo = c5_members(wl)
+    for s in solo:
+        for _ in range(warmup):
+            wl.sim_step(s, remeasure=False)
+    s0 = time.perf_counter()
+    for s in solo:
+        for _ in range(steps):
+            wl.sim_step(s, remeasure=False)
+    ssec = time.perf_counter() - s0
+    lat_ms = np.array(lat) * 1e3
+    return dict(workload="64 uneven 2D flows (64x64..512x256, circles/ellipses, Re 100-1000), Float32, one batch",
+                sims=len(sims), cells=cells, steps=steps, mcells=round(cells * steps / sec / 1e6, 2),
+                ms_per_batch_step=round(1e3 * sec / steps, 3),
+                step_latency_ms_p50=round(float(np.percentile(lat_ms, 50)), 3),
+                step_latency_ms_p99=round(float(np.percentile(lat_ms, 99)), 3),
+                vcycles_per_projection=round(float(nv), 3),
+                one_at_a_time_mcells=round(cells * steps / ssec / 1e6, 2),
+                batch_speedup=round(ssec / sec, 2))
+
+
 def roofline(stats, traffic_file):
     rows = []
     for s in stats:
@@ -205,6 +259,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-c3", action="store_true")
     ap.add_argument("--c3-steps", type=int, default=20)
+    ap.add_argument("--no-c5", action="store_true")
+    ap.add_argument("--c5-steps", type=int, default=20)
     ap.add_argument("--copy-calib", action="store_true")
     ap.add_argument("--quiet-kernels", action="store_true")
     ap.add_argument("--summarize", nargs=6, metavar=("STATS", "FETCH", "WRITE", "OUT_MD", "OUT_JSON", "CMD"),
@@ -251,6 +307,8 @@ def main():
                                mcells=round(r3m["mcells"], 2), ms_per_step=round(r3m["ms_per_step"], 3),
                                vcycles_per_step=round(r3m["vcycles_per_step"], 3))
         del r3m
+    if not a.no_c5:
+        out["c5"] = run_c5(wl, a.c5_steps, 3)
     if not a.no_cpu:
         out["cpu_baseline"] = run_cpu(n, R, center, Re, L)
     else:
