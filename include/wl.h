@@ -132,6 +132,20 @@ int wl_meanflow_reset(wl_meanflow_t m);                                  /* rese
 int wl_meanflow_download(wl_meanflow_t m, int field, void* dst, size_t nbytes); /* P, U, UU; τ = uu(m), :243-252 */
 int wl_meanflow_copy_to_flow(wl_meanflow_t m, wl_flow_t flow);           /* copy!(flow, meanflow), :254-257 */
 
+/* Space- and time-dependent boundary values and forcing (function uBC, g), evaluated by the caller
+ * on every cell in the layout of u (Ng…×D) and valid for the next wl_mom_step:
+ *   WL_FN_UBC:  uBC(i, loc(i,I), t₁); BC! then runs the reference's loops on it (src/core.jl:200-219);
+ *   WL_FN_ACC0: g(i, loc(i,I), t₀) + ∂uBC/∂t(i, loc(i,I), t₀), added by the predictor's
+ *               accelerate! (src/Flow.jl:69-73,193);
+ *   WL_FN_ACC1: the same at t₁ for the corrector (:208).
+ * A field replaces the descriptor's uniform values; src = NULL clears it.  This is how a Julia
+ * closure `uBC`/`g` reaches the device: the shim evaluates it with `apply!` on the host. */
+enum wl_fn_field { WL_FN_UBC = 0, WL_FN_ACC0 = 1, WL_FN_ACC1 = 2 };
+int wl_flow_set_field_fn(wl_flow_t h, int which, const void* src, size_t nbytes);
+/* After the caller uploaded u (the initial condition uBC(i,x,0), src/Flow.jl:138): BC!(u, uBC,
+ * exitBC, perdir) at t = 0 with the WL_FN_UBC field if set, exitBC!(u,u,0) and u⁰ = u (:140-141). */
+int wl_flow_apply_ic(wl_flow_t h);
+
 /* Batches: many flows of different sizes stepped together, one kernel launch per stage of the step
  * for all of them.  No counterpart in the reference, which steps one Simulation at a time
  * (src/WaterLily.jl:127-140 is the loop each member follows).  Members are ordinary flow handles

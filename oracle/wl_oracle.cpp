@@ -37,6 +37,31 @@ void BC(const Grid& g, T* a, const T* U, bool saveexit, int perdir) {
     }
 }
 
+// BC!(a, uBC::Function, saveexit, perdir, t): src/core.jl:201-219 with uBC given as the field
+// UB[I,i] = uBC(i, loc(i,I), t) on every cell.  The loops run in the reference's order (i outer,
+// j inner, whole slices), and the tangential write is (uBC(I) + a[I±δ]) − uBC(I±δ) as written.
+template <class T>
+void BC_fn(const Grid& g, T* a, const T* UB, bool saveexit, int perdir) {
+  const int64_t n = g.n;
+  for (int i = 0; i < g.D; ++i)
+    for (int j = 0; j < g.D; ++j) {
+      T* ai = a + i * n;
+      const T* Ui = UB + i * n;
+      const int64_t sj = g.st(j);
+      const int Nj = g.N[j];
+      if (perdir & (1 << j)) {
+        loop(g, r_slice(g, 1, j), [&](int64_t I, int, int, int) { ai[I] = ai[I + (int64_t)(Nj - 2) * sj]; });
+        loop(g, r_slice(g, Nj, j), [&](int64_t I, int, int, int) { ai[I] = ai[I - (int64_t)(Nj - 2) * sj]; });
+      } else if (i == j) {
+        for (int s = 1; s <= 2; ++s) loop(g, r_slice(g, s, j), [&](int64_t I, int, int, int) { ai[I] = Ui[I]; });
+        if (!saveexit || i > 0) loop(g, r_slice(g, Nj, j), [&](int64_t I, int, int, int) { ai[I] = Ui[I]; });
+      } else {
+        loop(g, r_slice(g, 1, j), [&](int64_t I, int, int, int) { ai[I] = (Ui[I] + ai[I + sj]) - Ui[I + sj]; });
+        loop(g, r_slice(g, Nj, j), [&](int64_t I, int, int, int) { ai[I] = (Ui[I] + ai[I - sj]) - Ui[I - sj]; });
+      }
+    }
+}
+
 // perBC!: src/core.jl:239-243
 template <class T>
 void perBC(const Grid& g, T* a, int perdir) {
@@ -417,8 +442,17 @@ struct Flow {
   MLP<T> pois;
   double tol = 1e-4;
   int itmx = 32;
+  // function-valued uBC and forcing, evaluated by the caller on every cell (empty: not used):
+  // ubc = uBC(i,loc(i,I),t₁); acc0/acc1 = g + ∂uBC/∂t at t₀ (predictor) and t₁ (corrector)
+  std::vector<T> ubc, acc0, acc1;
   void Ut(T t, T* out) const {  // uBC(i,x,t), uniform in x
     for (int i = 0; i < 3; ++i) out[i] = has_rate ? U[i] + rate[i] * t : U[i];
+  }
+  void applyBC(T t) {
+    if (!ubc.empty()) { BC_fn(g, u.data(), ubc.data(), exit_bc, perdir); return; }
+    T Ub[3];
+    Ut(t, Ub);
+    BC(g, u.data(), Ub, exit_bc, perdir);
   }
 };
 
@@ -487,7 +521,13 @@ void conv_diff(Flow<T>& a, T* r, const T* u, T* Phi) {
 
 // accelerate!: src/Flow.jl:69-73 (g and ∂uBC/∂t uniform in space; loop over all of r)
 template <class T>
-void accelerate(Flow<T>& a, T* r, T t) {
+void accelerate(Flow<T>& a, T* r, T t, const std::vector<T>& field) {
+  if (!field.empty()) {  // r += g(i,x,t) + ∂uBC/∂t(i,x,t) on every cell (src/Flow.jl:69-73)
+    loop(a.g, r_all(a.g), [&](int64_t I, int, int, int) {
+      for (int i = 0; i < a.g.D; ++i) r[I + i * a.g.n] += field[I + i * a.g.n];
+    });
+    return;
+  }
   if (!a.has_g && !a.has_rate) return;
   const Grid& g = a.g;
   for (int i = 0; i < g.D; ++i) {
@@ -553,9 +593,7 @@ void mom_project(Flow<T>& a, T w, T dt_end, T t) {
     loop(g, r_inside(g), [&](int64_t I, int, int, int) { u[I + i * n] -= L[I + i * n] * (x[I] - x[I - si]); });
   }
   for (int64_t k = 0; k < n; ++k) x[k] /= dt;
-  T Ub[3];
-  a.Ut(t, Ub);
-  BC(g, u, Ub, a.exit_bc, a.perdir);
+  a.applyBC(t);
 }
 
 // CFL with flux_out: src/Flow.jl:234-244 (maximum over the whole σ, ghosts included)
@@ -582,21 +620,19 @@ T mom_step(Flow<T>& a, T dt, T t0, T t1, int16_t* nit) {
   scale_u(a, T(0));
   a.pois.n.clear();
   a.pois.r2hist.clear();
-  T Ub[3];
-  a.Ut(t1, Ub);
   // predictor
   conv_diff(a, a.f.data(), a.u0.data(), a.sigma.data());
-  accelerate(a, a.f.data(), t0);
+  accelerate(a, a.f.data(), t0, a.acc0);
   BDIM(a, dt);
-  BC(a.g, a.u.data(), Ub, a.exit_bc, a.perdir);
+  a.applyBC(t1);
   if (a.exit_bc) exitBC(a.g, a.u.data(), a.u0.data(), dt);
   mom_project(a, T(1), dt, t1);
   // corrector
   conv_diff(a, a.f.data(), a.u.data(), a.sigma.data());
-  accelerate(a, a.f.data(), t1);
+  accelerate(a, a.f.data(), t1, a.acc1);
   BDIM(a, dt);
   scale_u(a, T(0.5));
-  BC(a.g, a.u.data(), Ub, a.exit_bc, a.perdir);
+  a.applyBC(t1);
   mom_project(a, T(0.5), dt, t1);
   if (nit) { nit[0] = a.pois.n[0]; nit[1] = a.pois.n[1]; }
   return CFL(a);
@@ -896,6 +932,14 @@ int wlo_bc(int dtype, int D, const int32_t* Ng, void* a, const double* U, int sa
   else { float Ub[3] = {(float)U[0], (float)U[1], D > 2 ? (float)U[2] : 0.f}; BC<float>(g, (float*)a, Ub, saveexit, perdir); }
   return 0;
 }
+// BC! with a function uBC given as the field UB (src/core.jl:201-219), on a host vector field
+int wlo_bc_fn(int dtype, int D, const int32_t* Ng, void* a, const void* UB, int saveexit, int perdir) {
+  Grid g;
+  g.set(D, Ng);
+  if (dtype == WL_F64) BC_fn<double>(g, (double*)a, (const double*)UB, saveexit, perdir);
+  else BC_fn<float>(g, (float*)a, (const float*)UB, saveexit, perdir);
+  return 0;
+}
 int wlo_perbc(int dtype, int D, const int32_t* Ng, void* a, int perdir) {
   Grid g;
   g.set(D, Ng);
@@ -1000,6 +1044,31 @@ int wlo_flow_last_residuals(wlo_flow* h, double* r2, int cap, int* count) {
   *count = (int)v.size();
   return 0;
 }
+// function-valued uBC / forcing fields (which: 0 uBC at t₁, 1 forcing at t₀, 2 forcing at t₁);
+// src == nullptr clears
+int wlo_flow_set_field_fn(wlo_flow* h, int which, const void* src, size_t nbytes) {
+  auto run = [&](auto& a) {
+    using T = typename std::remove_reference_t<decltype(a.u)>::value_type;
+    std::vector<T>& v = which == 0 ? a.ubc : (which == 1 ? a.acc0 : a.acc1);
+    if (!src) { v.clear(); return 0; }
+    if (nbytes != a.u.size() * sizeof(T)) return -1;
+    v.assign((const T*)src, (const T*)src + a.u.size());
+    return 0;
+  };
+  return h->dtype == WL_F64 ? run(*(Flow<double>*)h->impl) : run(*(Flow<float>*)h->impl);
+}
+// after the caller set u: BC!(u, uBC, exitBC, perdir) at t = 0, exitBC!(u,u,0), u⁰ = u
+// (src/Flow.jl:138-141)
+int wlo_flow_apply_ic(wlo_flow* h) {
+  auto run = [&](auto& a) {
+    using T = typename std::remove_reference_t<decltype(a.u)>::value_type;
+    a.applyBC(T(0));
+    exitBC(a.g, a.u.data(), a.u.data(), T(0));
+    a.u0 = a.u;
+    return 0;
+  };
+  return h->dtype == WL_F64 ? run(*(Flow<double>*)h->impl) : run(*(Flow<float>*)h->impl);
+}
 // sub-steps of mom_step! for phase-by-phase parity tests
 int wlo_flow_phase(wlo_flow* h, int phase, double dt, double w, double t) {
   auto run = [&](auto& a, auto T0) {
@@ -1007,7 +1076,7 @@ int wlo_flow_phase(wlo_flow* h, int phase, double dt, double w, double t) {
     switch (phase) {
       case 0: conv_diff(a, a.f.data(), a.u.data(), a.sigma.data()); break;
       case 1: BDIM(a, (T)dt); break;
-      case 2: { T Ub[3]; a.Ut((T)t, Ub); BC(a.g, a.u.data(), Ub, a.exit_bc, a.perdir); } break;
+      case 2: a.applyBC((T)t); break;
       case 3: mom_project(a, (T)w, (T)dt, (T)t); break;
       case 4: return (int)0;
     }

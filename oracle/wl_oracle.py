@@ -84,6 +84,9 @@ def lib():
             "wlo_metrics": (i, [i, i, P(C.c_int32), vp, vp, d, P(WlBody), i, P(d), P(d), P(d), P(d), P(d)]),
             "wlo_strain": (i, [i, i, P(C.c_int32), vp, P(C.c_int32), P(d)]),
             "wlo_meanflow_update": (i, [i, C.c_int64, i, d, vp, vp, vp, vp, vp]),
+            "wlo_flow_set_field_fn": (i, [vp, i, vp, sz]),
+            "wlo_bc_fn": (i, [i, i, P(C.c_int32), vp, vp, i, i]),
+            "wlo_flow_apply_ic": (i, [vp]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -189,6 +192,18 @@ class OFlow:
 
     def time(self):
         return float(sum(self.dt[:-1]))
+
+    def set_field_fn(self, which, a):
+        """Function-valued uBC (which 0, at t₁) or forcing g + ∂uBC/∂t (1 at t₀, 2 at t₁) on every
+        cell, in the layout of u; None clears."""
+        if a is None:
+            _ck(lib().wlo_flow_set_field_fn(self.h, which, None, 0), "set_field_fn")
+            return
+        a = np.asfortranarray(a, dtype=_np_dtype(self.dtype))
+        _ck(lib().wlo_flow_set_field_fn(self.h, which, a.ctypes.data, a.nbytes), "set_field_fn")
+
+    def apply_ic(self):
+        _ck(lib().wlo_flow_apply_ic(self.h), "apply_ic")
 
     def pressure_force(self, bodies, t=None):
         arr = make_bodies(bodies)
@@ -300,6 +315,17 @@ def bc(a, U, saveexit=False, perdir=()):
     Ng = (C.c_int32 * 3)(*(list(a.shape[:-1]) + [1] * (3 - D)))
     Ub = (C.c_double * 3)(*(list(U) + [0.0] * (3 - len(U))))
     lib().wlo_bc(dtype, D, Ng, a.ctypes.data, Ub, int(saveexit), sum(1 << (j - 1) for j in perdir))
+    return a
+
+
+def bc_fn(a, UB, saveexit=False, perdir=()):
+    """BC!(a, uBC::Function, saveexit, perdir, t) with UB[I,i] = uBC(i, loc(i,I), t) (src/core.jl:201-219)."""
+    assert a.flags.f_contiguous
+    D = a.ndim - 1
+    dtype = WL_F64 if a.dtype == np.float64 else WL_F32
+    UB = np.asfortranarray(UB, dtype=a.dtype)
+    Ng = (C.c_int32 * 3)(*(list(a.shape[:-1]) + [1] * (3 - D)))
+    lib().wlo_bc_fn(dtype, D, Ng, a.ctypes.data, UB.ctypes.data, int(saveexit), sum(1 << (j - 1) for j in perdir))
     return a
 
 

@@ -1,0 +1,107 @@
+"""Function-valued boundary values and forcing on the device (F4) against the oracle driven with
+the same host-evaluated fields, and against the reference's own criteria: the periodic
+Taylor-Green vortex (test/test_flow.jl:100-109), the boundary-layer profile (:133-140) and the
+rotating reference frame (:142-159)."""
+import math
+import sys
+
+import numpy as np
+import pytest
+
+import wl_oracle as O
+from __graft_entry__ import load_package
+from fn_fields import field, step_times
+from test_oracle_fnbc import L2, bl_profile, rotating, tgv, tgv_dt
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def wl():
+    m = load_package()
+    m.load_library()
+    return m
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def oracle_run(n, T, nu, perdir, ufn, accfns, steps=None, until=None, L=None):
+    """The oracle flow driven like the device flow: IC and uBC/forcing fields from the functions."""
+    Ng = tuple(v + 2 for v in n)
+    fl = O.OFlow(O.make_desc(n, (0,) * len(n), nu=float(nu), dtype=O.WL_F64 if T == np.float64 else O.WL_F32,
+                             perdir=perdir))
+    ic = field(ufn, Ng, T, 0.0)
+    fl.set("u", ic)
+    fl.set_field_fn(0, ic)
+    fl.apply_ic()
+    fl.pois_update()
+    k = 0
+    while (steps is not None and k < steps) or (until is not None and fl.time() / L < until):
+        t0, t1 = step_times(fl.dt, T)
+        fl.set_field_fn(0, field(ufn, Ng, T, t1))
+        if accfns:
+            acc = lambda t: field(lambda i, x, tt: sum_t(T, [f(i, x, tt) for f in accfns]), Ng, T, t)
+            fl.set_field_fn(1, acc(t0))
+            fl.set_field_fn(2, acc(t1))
+        fl.mom_step()
+        k += 1
+    return fl
+
+
+def sum_t(T, vals):
+    acc = None
+    for v in vals:
+        v = np.asarray(v, dtype=T)
+        acc = v if acc is None else (acc + v).astype(T)
+    return acc
+
+
+def test_tgv_on_device(wl):
+    say("case: periodic Taylor-Green vortex 64², function uBC and ∂uBC/∂t, to sim_time π/100")
+    T = np.float32
+    L = 64
+    k = T(2 * math.pi / L)
+    nu = T(1 / (k * T(1e8)))
+    f = lambda i, x, t: tgv(i, x, t, k, nu)
+    fd = lambda i, x, t: tgv_dt(i, x, t, k, nu)
+    sim = wl.Simulation((L, L), f, L, U=1, ν=nu, T=T, perdir=(1, 2), uBC_dt=fd)
+    wl.sim_step(sim, math.pi / 100, remeasure=False)
+    fl = oracle_run((L, L), T, nu, (1, 2), f, [fd], until=math.pi / 100, L=L)
+    u = sim.flow.u
+    assert np.array_equal(u, fl.get("u")) and np.array_equal(sim.flow.p, fl.get("p"))
+    ue = field(f, (L + 2, L + 2), T, T(np.sum(np.asarray(sim.flow.Δt[:-1], dtype=T), dtype=T)))
+    e1, e2 = L2(u[..., 0] - ue[..., 0]), L2(u[..., 1] - ue[..., 1])
+    say(f"  L2 error vs exact: {e1:.3e} {e2:.3e}; steps {len(sim.flow.Δt) - 1}")
+    assert e1 < 1e-4 and e2 < 1e-4
+
+
+def test_boundary_layer_on_device(wl):
+    say("case: boundary-layer profile 32², function uBC, 40 steps")
+    T = np.float32
+    L = 32
+    f = bl_profile(L)
+    sim = wl.Simulation((L, L), f, L, U=1, ν=0.001, T=T)
+    for _ in range(40):
+        wl.sim_step(sim, remeasure=False)
+    fl = oracle_run((L, L), T, 0.001, (), f, [], steps=40)
+    u = sim.flow.u
+    assert np.array_equal(u, fl.get("u")) and np.array_equal(sim.flow.p, fl.get("p"))
+    assert np.allclose(u[0, :, 0], u[-1, :, 0], rtol=math.sqrt(np.finfo(T).eps), atol=0)
+
+
+def test_rotating_frame_on_device(wl):
+    say("case: rotating reference frame 8², Float64, function uBC + Coriolis/centrifugal g")
+    T = np.float64
+    velocity, dvel, g = rotating(4, T)
+    sim = wl.Simulation((8, 8), velocity, 8, U=1, ν=0.0, T=T, g=g, uBC_dt=dvel)
+    wl.sim_step(sim, remeasure=False)
+    fl = oracle_run((8, 8), T, 0.0, (), velocity, [g, dvel], steps=1)
+    # Float64 runs agree with the oracle to rounding (the other Float64 parity tests use the same bar)
+    ru = np.linalg.norm(sim.flow.u - fl.get("u")) / np.linalg.norm(fl.get("u"))
+    dp = np.max(np.abs(sim.flow.p - fl.get("p")))
+    say(f"  L2(p) = {L2(sim.flow.p):.3e}; rel-L2 u vs oracle {ru:.3e}, max |Δp| {dp:.3e}")
+    assert ru < 1e-12 and dp < 1e-12
+    assert L2(sim.flow.p) < 3e-3

@@ -17,6 +17,7 @@ WL_SOLVER_PLAIN, WL_SOLVER_MIXED = 0, 1
 WL_BODY_SPHERE, WL_BODY_ELLIPSOID = 1, 2
 WL_ENAN = -3
 MFIELDS = {"P": 0, "U": 1, "UU": 2, "tau": 3}
+WL_FN_UBC, WL_FN_ACC0, WL_FN_ACC1 = 0, 1, 2
 
 
 class WlBody(C.Structure):
@@ -64,6 +65,8 @@ SIGNATURES = {
     "wl_meanflow_reset": (_i, [_vp]),
     "wl_meanflow_download": (_i, [_vp, _i, _vp, _sz]),
     "wl_meanflow_copy_to_flow": (_i, [_vp, _vp]),
+    "wl_flow_set_field_fn": (_i, [_vp, _i, _vp, _sz]),
+    "wl_flow_apply_ic": (_i, [_vp]),
     "wl_batch_create": (_i, [_P(_vp), _P(_vp), _i]),
     "wl_batch_destroy": (_i, [_vp]),
     "wl_batch_step": (_i, [_vp, _P(_d), _P(_d), _P(_d), _P(_d), _P(C.c_int16)]),

@@ -107,16 +107,40 @@ def _body_array(body):
 # ---------------------------------------------------------------------------------------------
 # Flow (src/Flow.jl:114-148)
 # ---------------------------------------------------------------------------------------------
+def _loc_coords(Ng, i, T):
+    """Coordinates of loc(i,I) = I − 1.5 − ½eᵢ (src/core.jl:177, 1-based I) on every cell, in T."""
+    idx = np.meshgrid(*[np.arange(1, n + 1, dtype=np.float64) for n in Ng], indexing="ij")
+    return [np.asarray(idx[d] - 1.5 - (0.5 if d == i - 1 else 0.0), dtype=T) for d in range(len(Ng))]
+
+
+def _fn_field(fns, Ng, T, t):
+    """Σ of the functions fns(i, x, t) at loc(i,I) on every cell, in the layout of u (apply!,
+    src/util.jl); each function takes coordinate arrays and returns an array (or scalar) of T."""
+    D = len(Ng)
+    out = np.zeros(tuple(Ng) + (D,), dtype=T, order="F")
+    for i in range(1, D + 1):
+        x = _loc_coords(Ng, i, T)
+        acc = None
+        for fn in fns:
+            v = np.broadcast_to(np.asarray(fn(i, x, T(t)), dtype=T), Ng)
+            acc = v if acc is None else (acc + v).astype(T)
+        out[..., i - 1] = acc
+    return out
+
+
 class Flow:
     """Flow(N, uBC; Δt=0.25, ν=0, g=None, λ=quick, T=float32, perdir=(), exitBC=False).
 
-    uBC: a tuple (constant), or a pair (U, dU/dt) for a boundary velocity uniform in space that
-    varies linearly in time.  g: None, or a pair (g0, g1) for g(i,x,t) = g0[i] + g1[i]*t.
-    The flow owns its MultiLevelPoisson (x≡p, L≡μ₀, z≡σ), exposed as `flow.pois`.
+    uBC: a tuple (constant); a pair (U, dU/dt) for a boundary velocity uniform in space that varies
+    linearly in time; or a function uBC(i, x, t) (x: the coordinate arrays of the cells' loc(i,I))
+    evaluated on the host every step, with its time derivative `uBC_dt(i, x, t)` when it depends
+    on t (∂uBC/∂t is taken as 0 without it).  g: None, a pair (g0, g1) for g(i,x,t) = g0[i] +
+    g1[i]*t, or a function g(i, x, t).  The flow owns its MultiLevelPoisson (x≡p, L≡μ₀, z≡σ),
+    exposed as `flow.pois`.
     """
 
     def __init__(self, N, uBC, Δt=0.25, ν=0.0, g=None, λ=quick, T=np.float32, perdir=(), exitBC=False,
-                 dt=None, nu=None):
+                 dt=None, nu=None, uBC_dt=None):
         if dt is not None:
             Δt = dt
         if nu is not None:
@@ -133,6 +157,13 @@ class Flow:
         d.scheme = int(λ)
         d.perdir_mask = _perdir_mask(perdir)
         d.exit_bc = int(bool(exitBC))
+        self.uBC_fn = uBC if callable(uBC) else None
+        self.uBC_dt = uBC_dt
+        self.g_fn = g if callable(g) else None
+        if self.uBC_fn is not None:
+            uBC = (0.0,) * self.D
+        if self.g_fn is not None:
+            g = None
         if len(uBC) == 2 and hasattr(uBC[0], "__len__"):
             U, rate = uBC
             d.has_uBC_rate = 1
@@ -156,6 +187,28 @@ class Flow:
         self.h = C.c_void_p()
         A.check(A.load().wl_flow_create(C.byref(self.h), C.byref(d)))
         self.pois = FlowPoisson(self)
+        if self.uBC_fn is not None:  # u = uBC(i,x,0) everywhere, then BC! and exitBC! (src/Flow.jl:138-141)
+            ic = _fn_field([self.uBC_fn], self.Ng, self.T, 0.0)
+            self.set("u", ic)
+            self.set_field_fn(A.WL_FN_UBC, ic)
+            A.check(A.load().wl_flow_apply_ic(self.h))
+
+    def set_field_fn(self, which, a):
+        """Upload a host-evaluated function field (WL_FN_UBC, WL_FN_ACC0, WL_FN_ACC1); None clears."""
+        if a is None:
+            A.check(A.load().wl_flow_set_field_fn(self.h, which, None, 0))
+            return
+        a = np.asfortranarray(a, dtype=self.T)
+        A.check(A.load().wl_flow_set_field_fn(self.h, which, a.ctypes.data, a.nbytes))
+
+    def _functions_for_step(self, t0, t1):
+        """The function fields of the coming step: uBC at t₁; g + ∂uBC/∂t at t₀ and t₁."""
+        if self.uBC_fn is not None:
+            self.set_field_fn(A.WL_FN_UBC, _fn_field([self.uBC_fn], self.Ng, self.T, t1))
+        fns = [f for f in (self.g_fn, self.uBC_dt if self.uBC_fn is not None else None) if f is not None]
+        if fns:
+            self.set_field_fn(A.WL_FN_ACC0, _fn_field(fns, self.Ng, self.T, t0))
+            self.set_field_fn(A.WL_FN_ACC1, _fn_field(fns, self.Ng, self.T, t1))
 
     # fields, downloaded in Julia layout
     def _shape(self, name):
@@ -244,6 +297,8 @@ def mom_step(flow, pois=None):
     dt = flow.Δt[-1]
     t1 = float(np.sum(np.asarray(flow.Δt, dtype=flow.T), dtype=flow.T))
     t0 = float(flow.T(t1) - flow.T(dt))
+    if flow.uBC_fn is not None or flow.g_fn is not None:
+        flow._functions_for_step(t0, t1)
     out = C.c_double()
     nit = (C.c_int16 * 2)()
     rc = A.load().wl_mom_step(flow.h, dt, t0, t1, C.byref(out), nit)
@@ -376,21 +431,21 @@ class Simulation:
     """Simulation(dims, uBC, L; U, Δt, ν, ϵ, g, perdir, exitBC, λ, body, T)."""
 
     def __init__(self, dims, uBC, L, U=None, Δt=0.25, ν=0.0, ϵ=1, g=None, perdir=(), exitBC=False, λ=quick,
-                 body=None, T=np.float32, dt=None, nu=None, eps=None):
+                 body=None, T=np.float32, dt=None, nu=None, eps=None, uBC_dt=None):
         if dt is not None:
             Δt = dt
         if nu is not None:
             ν = nu
         if eps is not None:
             ϵ = eps
-        funcBC = len(uBC) == 2 and hasattr(uBC[0], "__len__")
+        funcBC = callable(uBC) or (len(uBC) == 2 and hasattr(uBC[0], "__len__"))
         if U is None:
             if funcBC:
                 raise AssertionError("`U` (velocity scale) must be specified if boundary conditions `uBC` is a `Function`")
             U = math.sqrt(sum(float(v) ** 2 for v in uBC))
         self.U, self.L, self.ϵ = U, L, ϵ
         self.body = body if body is not None else NoBody()
-        self.flow = Flow(dims, uBC, Δt=Δt, ν=ν, g=g, λ=λ, T=T, perdir=perdir, exitBC=exitBC)
+        self.flow = Flow(dims, uBC, Δt=Δt, ν=ν, g=g, λ=λ, T=T, perdir=perdir, exitBC=exitBC, uBC_dt=uBC_dt)
         measure_flow(self.flow, self.body, t=0.0, ϵ=ϵ)
         self.pois = self.flow.pois
         self.pois.update()

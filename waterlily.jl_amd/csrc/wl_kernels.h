@@ -73,7 +73,8 @@ __device__ __forceinline__ bool same_bits(T a, T b) {
 // =============================================================================================
 struct Accel {
   int on;
-  double add[3];
+  double add[3];       // uniform g(t) + ∂uBC/∂t, formed in T by the host
+  const void* field;   // or, when set, g(i,x,t) + ∂uBC/∂t(i,x,t) on every cell (T, layout of u)
 };
 
 // Face flux F_j^i at the lower j-face of cell I (src/Flow.jl:8-11,47,56-57,61): variant 0 inner
@@ -147,6 +148,7 @@ __global__ void __launch_bounds__(64 * CBY) k_conv_bdim1(Geo g, ZT t, const T* _
     for (int i = 0; i < D; ++i) {
       T r = rr[i];
       if (acc.on) r += (T)acc.add[i];
+      if (acc.field) r += ((const T*)acc.field)[I + i * n];
       const T uv = u[I + i * n];
       const T u0v = PRED ? uv : u0[I + i * n];
       const T vv = vrows[cy + (int64_t)g.N[1] * cz] ? V[I + i * n] : T(0);  // V is +0 in unflagged rows
@@ -305,6 +307,7 @@ __device__ __forceinline__ void k_conv_lds_body(const Geo& g, const ZT& t, const
     for (int i = 0; i < D; ++i) {
       T r = rr.v[i];
       if (acc.on) r += (T)acc.add[i];
+      if (acc.field) r += ((const T*)acc.field)[I + i * n];
       const T uv = uvals.v[i];
       const T u0v = PRED ? uv : u0[I + i * n];
       f[I + i * n] = u0v + dt * r - Vv.v[i];
@@ -575,6 +578,42 @@ template <class T>
 __global__ void __launch_bounds__(256) k_bc(Geo g, T* __restrict__ a, int ncomp_vec, double U0, double U1,
                                             double U2, int saveexit, int perdir) {
   k_bc_body<T>(g, a, ncomp_vec, U0, U1, U2, saveexit, perdir, (int)blockIdx.x, (int)blockIdx.y, (int)blockIdx.z);
+}
+
+// BC!(a, uBC::Function) (src/core.jl:201-219) for one (component i, direction j) pair, with uBC
+// given as the field UB (uBC(i, loc(i,I), t) on every cell).  The host launches the pairs in the
+// reference's order (i outer, j inner); within a pair the written slices and the slices read are
+// disjoint.  One thread per position of the j-plane.
+template <class T>
+__global__ void __launch_bounds__(256) k_bc_fn(Geo g, T* __restrict__ a, const T* __restrict__ UB, int i, int j,
+                                               int saveexit, int perdir) {
+  const int D = g.D;
+  int p0 = j == 0 ? 1 : 0, p1 = j == 2 ? 1 : 2;
+  if (D == 2) p1 = -1;
+  const int n0 = g.N[p0], n1 = p1 >= 0 ? g.N[p1] : 1;
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (int64_t)n0 * n1) return;
+  int c[3] = {0, 0, 0};
+  c[p0] = (int)(t % n0);
+  if (p1 >= 0) c[p1] = (int)(t / n0);
+  const int Nj = g.N[j];
+  const int64_t sj = g.s[j];
+  T* ai = a + (int64_t)i * g.n;
+  const T* Ui = UB + (int64_t)i * g.n;
+  c[j] = 0;
+  const int64_t I0 = at(g, c[0], c[1], c[2]);
+  const int64_t IN = I0 + (int64_t)(Nj - 1) * sj;
+  if ((perdir >> j) & 1) {
+    ai[I0] = ai[I0 + (int64_t)(Nj - 2) * sj];
+    ai[IN] = ai[IN - (int64_t)(Nj - 2) * sj];
+  } else if (i == j) {
+    ai[I0] = Ui[I0];
+    ai[I0 + sj] = Ui[I0 + sj];
+    if (!saveexit || i > 0) ai[IN] = Ui[IN];
+  } else {
+    ai[I0] = (Ui[I0] + ai[I0 + sj]) - Ui[I0 + sj];
+    ai[IN] = (Ui[IN] + ai[IN - sj]) - Ui[IN - sj];
+  }
 }
 
 // exitBC! (src/core.jl:226-233) on the x-component of u: U = mean of u over the inflow plane

@@ -786,6 +786,8 @@ struct FlowBase {
   int mode = WL_SOLVER_PLAIN;
   bool mu1_dirty = true;  // μ₁ changed since its row flags were built
   virtual void field_uploaded(int field) = 0;
+  virtual void set_field(int which, const void* src, size_t nbytes) = 0;
+  virtual void apply_ic() = 0;
 };
 
 static DBodies to_dbodies(const wl_body* b, int nb) {
@@ -848,7 +850,7 @@ struct FlowT : FlowBase {
   ~FlowT() override {
     (void)hipStreamSynchronize(st);
     dfree(u); dfree(u0); dfree(f); dfree(V); dfree(mu0); dfree(mu1); dfree(p); dfree(sigma); dfree(xw); dfree(rows);
-    dfree(mpart);
+    dfree(mpart); dfree(fn_ubc); dfree(fn_acc[0]); dfree(fn_acc[1]);
     free_bufs(B);
     mg.lv[0].ownL = mg.lv[0].ownx = mg.lv[0].ownz = false;
     (void)hipStreamDestroy(st);
@@ -934,10 +936,54 @@ struct FlowT : FlowBase {
     ba.zc = (int)std::min<int64_t>(32, std::max<int64_t>(1, (int64_t)g.N[2] * per / (zt_target_default() / 2)));
     return ba;
   }
-  // accelerate!: r += g(i,x,t) + ∂uBC/∂t (src/Flow.jl:69-73), uniform in space, formed in T
-  Accel accel(double t) const {
+  // function-valued uBC and forcing (host-evaluated fields, wl_flow_set_field_fn): uBC at t₁,
+  // and g + ∂uBC/∂t at t₀ (predictor) and t₁ (corrector)
+  T* fn_ubc = nullptr;
+  T* fn_acc[2] = {nullptr, nullptr};
+  bool use_ubc = false, use_acc[2] = {false, false};
+  bool has_fields() const { return use_ubc || use_acc[0] || use_acc[1]; }
+  void set_field(int which, const void* src, size_t nbytes) override {
+    if (which < 0 || which > 2) throw WlErr(WL_EINVAL, "unknown function field id");
+    T*& buf = which == 0 ? fn_ubc : fn_acc[which - 1];
+    bool& use = which == 0 ? use_ubc : use_acc[which - 1];
+    if (!src) { use = false; return; }
+    if (nbytes != (size_t)g.n * D * sizeof(T)) throw WlErr(WL_EINVAL, "a function field has the size of u");
+    if (!buf) buf = dalloc<T>(g.n * D);
+    sync();
+    HIPCK(hipMemcpy(buf, src, nbytes, hipMemcpyHostToDevice));
+    use = true;
+  }
+  // BC! of u at time t: the uniform closed form, or the reference's sequential loops on the
+  // function field (src/core.jl:200-219)
+  void apply_bc_u(double t) {
+    if (!use_ubc) {
+      double Ub[3];
+      Ut(t, Ub);
+      mg.bc_vec(g, u, Ub, desc.exit_bc);
+      return;
+    }
+    ProfScope ps(&prof, st, "bc", 0, 0.0);
+    for (int i = 0; i < D; ++i)
+      for (int j = 0; j < D; ++j) {
+        int np = 1;
+        for (int q = 0; q < D; ++q) if (q != j) np *= g.N[q];
+        hipLaunchKernelGGL(k_bc_fn<T>, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, st, g, u,
+                           (const T*)fn_ubc, i, j, desc.exit_bc, desc.perdir_mask);
+        KCK();
+      }
+  }
+  void apply_ic() override {  // after the caller set u: the rest of Flow's constructor (src/Flow.jl:140-141)
+    apply_bc_u(0.0);
+    exit_bc(u, T(0));
+    HIPCK(hipMemcpyAsync(u0, u, g.n * D * sizeof(T), hipMemcpyDeviceToDevice, st));
+    sync();
+  }
+  // accelerate!: r += g(i,x,t) + ∂uBC/∂t (src/Flow.jl:69-73): uniform in space and formed in T,
+  // or the caller's field for this phase (0 predictor, 1 corrector)
+  Accel accel(double t, int phase = 0) const {
     Accel acc;
-    acc.on = desc.has_g || desc.has_uBC_rate;
+    acc.field = use_acc[phase] ? (const void*)fn_acc[phase] : nullptr;
+    acc.on = acc.field ? 0 : (desc.has_g || desc.has_uBC_rate);
     for (int i = 0; i < 3; ++i) {
       T a;
       if (desc.has_g && desc.has_uBC_rate) a = ((T)desc.g0[i] + (T)desc.g1[i] * (T)t) + (T)desc.uBC_rate[i];
@@ -950,7 +996,7 @@ struct FlowT : FlowBase {
   template <int S, int DD>
   void conv_bdim(bool pred, T dt, double t) {
     const ZT ba = conv_zt();
-    const Accel acc = accel(t);
+    const Accel acc = accel(t, pred ? 0 : 1);
     const T nu = (T)desc.nu;
     const dim3 cblk(64, CBY, 1);
     {
@@ -1006,9 +1052,7 @@ struct FlowT : FlowBase {
         default: conv_bdim<WL_CDS, 2>(pred, dt, t); break;
       }
     }
-    double Ub[3];
-    Ut(t1, Ub);
-    mg.bc_vec(g, u, Ub, desc.exit_bc);
+    apply_bc_u(t1);
     if (pred && desc.exit_bc) exit_bc(u0, dt);  // src/Flow.jl:195, predictor only
   }
   // exitBC!(u, src, Δt) (src/core.jl:226-233)
@@ -1045,9 +1089,7 @@ struct FlowT : FlowBase {
     hipLaunchKernelGGL(k_correct<T>, zt_grid(ba), kBlock, 0, st, g, ba, u, mg.lr(0), (const T*)xw, p, dt);
     KCK();
     }
-    double Ub[3];
-    Ut(t1, Ub);
-    mg.bc_vec(g, u, Ub, desc.exit_bc);
+    apply_bc_u(t1);
     return nit;
   }
 
@@ -1074,9 +1116,7 @@ struct FlowT : FlowBase {
     hipLaunchKernelGGL(k_correct<T>, zt_grid(ba), kBlock, 0, st, g, ba, u, mg.lr(0), (const T*)xw, p, dt);
     KCK();
     }
-    double Ub[3];
-    Ut(t1, Ub);
-    mg.bc_vec(g, u, Ub, desc.exit_bc);
+    apply_bc_u(t1);
     return nit;
   }
 
@@ -1233,6 +1273,7 @@ struct BatchT : BatchBase {
       if (f->desc.scheme != scheme) throw WlErr(WL_EINVAL, "batch members must use the same convection scheme");
       if (f->desc.perdir_mask || f->desc.exit_bc) throw WlErr(WL_EINVAL, "batch members: no periodic directions, no exitBC");
       if (f->mode != WL_SOLVER_PLAIN) throw WlErr(WL_EINVAL, "batch members use the plain solver");
+      if (f->has_fields()) throw WlErr(WL_EINVAL, "batch members use uniform boundary values and forcing");
       f->sync();
       if (f->mu1_dirty) f->mu1_rows();
       f->mg.ensure_lrows();
@@ -1464,7 +1505,7 @@ struct BatchT : BatchBase {
       const T nu = (T)f->desc.nu;
       const ZT tc = f->conv_zt();
       cp[s] = AConv<T>{f->g, tc, uA, uB, f->f, f->sigma, f->V, f->vrows(), nu, dt, f->accel(t0[s])};
-      cc[s] = AConv<T>{f->g, tc, uB, uA, f->f, f->sigma, f->V, f->vrows(), nu, dt, f->accel(t1[s])};
+      cc[s] = AConv<T>{f->g, tc, uB, uA, f->f, f->sigma, f->V, f->vrows(), nu, dt, f->accel(t1[s], 1)};
       const ZT tb = make_zt(box_inside(f->g), 2 * zt_target_default());
       bp[s] = ABdim<T>{f->g, tb, uB, uA, f->f, f->V, f->mg.lr(0), f->mu1, f->rows, f->vrows()};
       bcr[s] = ABdim<T>{f->g, tb, uB, uA, f->f, f->V, f->mg.lr(0), f->mu1, f->rows, f->vrows()};
@@ -1953,6 +1994,18 @@ int wl_meanflow_copy_to_flow(wl_meanflow_t m, wl_flow_t flow) {
   HIPCK(hipMemcpyAsync(flow->impl->field_ptr(WL_U), m->U, m->n * m->D * m->es(), hipMemcpyDeviceToDevice, st));
   HIPCK(hipMemcpyAsync(flow->impl->field_ptr(WL_P), m->P, m->n * m->es(), hipMemcpyDeviceToDevice, st));
   HIPCK(hipStreamSynchronize(st));
+  API_END
+}
+int wl_flow_set_field_fn(wl_flow_t h, int which, const void* src, size_t nbytes) {
+  API_BEGIN
+  if (!h) throw WlErr(WL_EINVAL, "null handle");
+  h->impl->set_field(which, src, nbytes);
+  API_END
+}
+int wl_flow_apply_ic(wl_flow_t h) {
+  API_BEGIN
+  if (!h) throw WlErr(WL_EINVAL, "null handle");
+  h->impl->apply_ic();
   API_END
 }
 struct wl_batch_s {
