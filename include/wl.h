@@ -131,6 +131,8 @@ int wl_meanflow_update(wl_meanflow_t m, wl_flow_t flow, double eps);     /* upda
 int wl_meanflow_reset(wl_meanflow_t m);                                  /* reset!(meanflow), :222-227 (fields) */
 int wl_meanflow_download(wl_meanflow_t m, int field, void* dst, size_t nbytes); /* P, U, UU; τ = uu(m), :243-252 */
 int wl_meanflow_copy_to_flow(wl_meanflow_t m, wl_flow_t flow);           /* copy!(flow, meanflow), :254-257 */
+/* Put stored averages (P, U, UU) back on the device: load!(meanflow), ext/WaterLilyJLD2Ext.jl:57-70 */
+int wl_meanflow_upload(wl_meanflow_t m, int field, const void* src, size_t nbytes);
 
 /* Space- and time-dependent boundary values and forcing (function uBC, g), evaluated by the caller
  * on every cell in the layout of u (Ng…×D) and valid for the next wl_mom_step:
@@ -145,6 +147,10 @@ int wl_flow_set_field_fn(wl_flow_t h, int which, const void* src, size_t nbytes)
 /* After the caller uploaded u (the initial condition uBC(i,x,0), src/Flow.jl:138): BC!(u, uBC,
  * exitBC, perdir) at t = 0 with the WL_FN_UBC field if set, exitBC!(u,u,0) and u⁰ = u (:140-141). */
 int wl_flow_apply_ic(wl_flow_t h);
+
+/* CFL(a) (src/Flow.jl:234-237) of the flow's current u, as mom_step! ends with it: used when a
+ * restart replaces u (load!, ext/WaterLilyReadVTKExt.jl:38). */
+int wl_flow_cfl(wl_flow_t h, double* dt);
 
 /* Batches: many flows of different sizes stepped together, one kernel launch per stage of the step
  * for all of them.  No counterpart in the reference, which steps one Simulation at a time

@@ -11,3 +11,7 @@ from .api import (  # noqa: F401
     pressure_force, pressure_moment, quick, reset, sim_step, sim_time, solver, time, total_force, total_moment,
     update, uu, vanLeer, viscous_force, viscous_moment,
 )
+from .io import (  # noqa: F401
+    VTKWriter, close, default_attrib, load_flow, load_meanflow, load_pvd, read_vti, save, save_flow, save_meanflow,
+    vtk_writer, write_vti,
+)

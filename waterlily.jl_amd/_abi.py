@@ -65,8 +65,10 @@ SIGNATURES = {
     "wl_meanflow_reset": (_i, [_vp]),
     "wl_meanflow_download": (_i, [_vp, _i, _vp, _sz]),
     "wl_meanflow_copy_to_flow": (_i, [_vp, _vp]),
+    "wl_meanflow_upload": (_i, [_vp, _i, _vp, _sz]),
     "wl_flow_set_field_fn": (_i, [_vp, _i, _vp, _sz]),
     "wl_flow_apply_ic": (_i, [_vp]),
+    "wl_flow_cfl": (_i, [_vp, _P(_d)]),
     "wl_batch_create": (_i, [_P(_vp), _P(_vp), _i]),
     "wl_batch_destroy": (_i, [_vp]),
     "wl_batch_step": (_i, [_vp, _P(_d), _P(_d), _P(_d), _P(_d), _P(C.c_int16)]),

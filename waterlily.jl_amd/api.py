@@ -602,6 +602,13 @@ class MeanFlow:
         A.check(A.load().wl_meanflow_download(self.h, A.MFIELDS[name], a.ctypes.data, a.nbytes))
         return a
 
+    def _put(self, name, a):
+        shp = {"P": self.Ng, "U": self.Ng + (self.D,), "UU": self.Ng + (self.D, self.D)}[name]
+        a = np.asfortranarray(a, dtype=self.T)
+        if a.shape != shp:
+            raise AssertionError(f"{name} must have shape {shp}")
+        A.check(A.load().wl_meanflow_upload(self.h, A.MFIELDS[name], a.ctypes.data, a.nbytes))
+
     P = property(lambda s: s._get("P"))
     U = property(lambda s: s._get("U"))
     UU = property(lambda s: s._get("UU") if s.uu_stats else None)

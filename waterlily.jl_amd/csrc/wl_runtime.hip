@@ -788,6 +788,7 @@ struct FlowBase {
   virtual void field_uploaded(int field) = 0;
   virtual void set_field(int which, const void* src, size_t nbytes) = 0;
   virtual void apply_ic() = 0;
+  virtual double cfl() = 0;
 };
 
 static DBodies to_dbodies(const wl_body* b, int nb) {
@@ -1120,6 +1121,25 @@ struct FlowT : FlowBase {
     return nit;
   }
 
+  // CFL(a) (src/Flow.jl:234-237) of the current u: min(10, 1/(max flux out of a cell + 5ν)),
+  // formed in T; σ is its scratch (its stale ghosts included, as in the reference)
+  double cfl() override {
+    ZT ba = make_zt(box_all(g));
+    const int64_t nb = zt_blocks(ba);
+    T* tpart = (T*)B.partial;
+    {
+      ProfScope ps(&prof, st, "cfl", 0, (double)sizeof(T) * (D + 1) * ncells_inside(g));
+      hipLaunchKernelGGL(k_cfl<T>, zt_grid(ba), kBlock, 0, st, g, ba, (const T*)u, sigma, tpart);
+      KCK();
+    }
+    hipLaunchKernelGGL(k_finish_max<T>, dim3(1), dim3(256), 0, st, (const T*)tpart, nb, (T*)B.tsc + 2);
+    KCK();
+    HIPCK(hipMemcpyAsync(B.htsc, (T*)B.tsc + 2, sizeof(T), hipMemcpyDeviceToHost, st));
+    HIPCK(hipStreamSynchronize(st));
+    const T m = ((T*)B.htsc)[0];
+    const T inv = T(1) / (m + T(5) * (T)desc.nu);
+    return (double)(std::isnan(inv) ? inv : std::min(T(10), inv));
+  }
   int step(double dt_d, double t0, double t1, double* dt_next, int16_t* nit) override {
     if (mu1_dirty) mu1_rows();
     mg.ensure_lrows();
@@ -1129,23 +1149,7 @@ struct FlowT : FlowBase {
     const int n1 = project(T(1), dt, t1);
     momentum(false, dt, t1, t1);
     const int n2 = project(T(0.5), dt, t1);
-    // CFL (src/Flow.jl:234-237)
-    ZT ba = make_zt(box_all(g));
-    const int64_t nb = zt_blocks(ba);
-    T* tpart = (T*)B.partial;
-    {
-    ProfScope ps(&prof, st, "cfl", 0, (double)sizeof(T) * (D + 1) * ncells_inside(g));
-    hipLaunchKernelGGL(k_cfl<T>, zt_grid(ba), kBlock, 0, st, g, ba, (const T*)u, sigma, tpart);
-    KCK();
-    }
-    hipLaunchKernelGGL(k_finish_max<T>, dim3(1), dim3(256), 0, st, (const T*)tpart, nb, (T*)B.tsc + 2);
-    KCK();
-    HIPCK(hipMemcpyAsync(B.htsc, (T*)B.tsc + 2, sizeof(T), hipMemcpyDeviceToHost, st));
-    HIPCK(hipStreamSynchronize(st));
-    const T m = ((T*)B.htsc)[0];
-    const T nu = (T)desc.nu;
-    const T inv = T(1) / (m + T(5) * nu);
-    const T dtn = std::isnan(inv) ? inv : std::min(T(10), inv);
+    const T dtn = (T)cfl();
     *dt_next = (double)dtn;
     if (nit) { nit[0] = (int16_t)n1; nit[1] = (int16_t)n2; }
     if (!std::isfinite(dtn)) return set_err(WL_ENAN, "non-finite CFL time step (NaN/Inf in u)");
@@ -1985,6 +1989,27 @@ int wl_meanflow_download(wl_meanflow_t m, int field, void* dst, size_t nbytes) {
   HIPCK(e);
   API_END
 }
+int wl_meanflow_upload(wl_meanflow_t m, int field, const void* src, size_t nbytes) {
+  API_BEGIN
+  if (!m || !src) throw WlErr(WL_EINVAL, "null argument");
+  const int64_t D = m->D;
+  size_t cnt = 0;
+  void* dst = nullptr;
+  switch (field) {
+    case WL_MF_P: cnt = m->n; dst = m->P; break;
+    case WL_MF_U: cnt = m->n * D; dst = m->U; break;
+    case WL_MF_UU:
+      if (!m->UU) throw WlErr(WL_EINVAL, "MeanFlow was created without uu_stats");
+      cnt = m->n * D * D;
+      dst = m->UU;
+      break;
+    default: throw WlErr(WL_EINVAL, "MeanFlow field id cannot be uploaded");
+  }
+  if (nbytes != cnt * m->es()) throw WlErr(WL_EINVAL, "buffer size does not match the field");
+  HIPCK(hipStreamSynchronize(m->st));
+  HIPCK(hipMemcpy(dst, src, nbytes, hipMemcpyHostToDevice));
+  API_END
+}
 int wl_meanflow_copy_to_flow(wl_meanflow_t m, wl_flow_t flow) {
   API_BEGIN
   if (!m || !flow) throw WlErr(WL_EINVAL, "null handle");
@@ -2000,6 +2025,13 @@ int wl_flow_set_field_fn(wl_flow_t h, int which, const void* src, size_t nbytes)
   API_BEGIN
   if (!h) throw WlErr(WL_EINVAL, "null handle");
   h->impl->set_field(which, src, nbytes);
+  API_END
+}
+int wl_flow_cfl(wl_flow_t h, double* dt) {
+  API_BEGIN
+  if (!h || !dt) throw WlErr(WL_EINVAL, "null argument");
+  h->impl->sync();
+  *dt = h->impl->cfl();
   API_END
 }
 int wl_flow_apply_ic(wl_flow_t h) {
