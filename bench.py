@@ -251,6 +251,80 @@ def summarize(stats_csv, fetch_csv, write_csv, out_md, out_json, cmd, calib_byte
         json.dump(dict(tj, _fetch_scale=f_scale, _write_scale=w_scale), fh, indent=1)
 
 
+WARMNET = os.path.join(ROOT, "waterlily.jl_amd", "data", "warmnet_c4.npz")
+# algorithmic flops of one network forward per level-1 voxel: 2·27·Σ ci·co over the real channels
+# of each layer, weighted by the layer's voxel share (1, 1/8, 1/64, 1/512)
+NET_FLOPS_PER_VOXEL = 2 * 27 * (3 * 16 + 16 * 16 + 16 * 16 + 16 * 1 + (16 * 32 + 32 * 16) / 8 + (32 * 64 + 64 * 32) / 64
+                                + 64 * 64 / 512)
+
+
+def train_warmstart(wl, out_path, nseeds, steps):
+    """Train the warm-start network on pairs generated by random-ellipsoid flows (seeds 100..)."""
+    n = (256, 128, 128)
+    net = wl.WarmNet((128, 64, 64), replay_pairs=512)
+    t0 = time.perf_counter()
+    losses = wl.learned.train_on_flows(net, list(range(100, 100 + nseeds)), n=n, steps=steps,
+                                       log=lambda m: print(m, flush=True))
+    os.makedirs(os.path.dirname(out_path), exist_ok=True)
+    net.save(out_path)
+    print(json.dumps(dict(trained_pairs=2 * steps * nseeds, seconds=round(time.perf_counter() - t0, 1),
+                          first_loss=losses[0], last_loss=losses[-1], weights=out_path)), flush=True)
+
+
+def run_c4(wl, steps, warmup, seeds=(7, 8, 9)):
+    """Learned warm start against the plain method at equal residual on held-out random-ellipsoid
+    flows (256x128x128): both runs step the same flow from the same state; every projection of
+    both iterates to Σr² < 1e-4."""
+    if not os.path.exists(WARMNET):
+        return dict(skipped=f"no trained weights at {os.path.relpath(WARMNET, ROOT)}")
+    net = wl.WarmNet.load(WARMNET, grid=(128, 64, 64), replay_pairs=1)
+    res = {"plain": [], "learned": [], "learned_predictor": []}
+    fdiff, vc = [], {k: [] for k in res}
+    fwd_ms = None
+    for seed in seeds:
+        for mode in res:
+            sim = wl.learned.random_ellipsoid_simulation(seed)
+            for _ in range(warmup):
+                wl.sim_step(sim, remeasure=False)
+            if mode != "plain":
+                wl.set_warmstart(sim, net, guess=True, predictor_only=(mode == "learned_predictor"))
+            n0 = len(sim.pois.n)
+            t0 = time.perf_counter()
+            for _ in range(steps):
+                wl.sim_step(sim, remeasure=False)
+            res[mode].append((time.perf_counter() - t0) / steps)
+            vc[mode].append(sum(sim.pois.n[n0:]) / steps)
+            F = wl.pressure_force(sim)
+            if mode == "plain":
+                F0 = F
+            else:
+                fdiff.append(float(np.linalg.norm(F - F0) / max(np.linalg.norm(F0), 1e-30)))
+            if mode == "learned" and seed == seeds[0]:
+                sim.flow.profile(True)
+                for _ in range(5):
+                    wl.sim_step(sim, remeasure=False)
+                st = {k["name"]: k for k in sim.flow.profile_read()}
+                sim.flow.profile(False)
+                fw = st.get("nn_forward")
+                fwd_ms = fw["total_ms"] / fw["launches"] if fw else None
+            wl.set_warmstart(sim, None)
+            del sim
+    cells = 256 * 128 * 128
+    ms = {k: 1e3 * float(np.mean(v)) for k, v in res.items()}
+    flops = NET_FLOPS_PER_VOXEL * 128 * 64 * 64
+    return dict(workload="held-out random-ellipsoid flows 256x128x128 (seeds %s), Float32, tol Σr²<1e-4" % list(seeds),
+                steps=steps, ms_per_step={k: round(v, 3) for k, v in ms.items()},
+                mcells={k: round(cells / v / 1e3, 2) for k, v in ms.items()},
+                vcycles_per_step={k: round(float(np.mean(v)), 3) for k, v in vc.items()},
+                speedup_learned=round(ms["plain"] / ms["learned"], 3),
+                speedup_learned_predictor=round(ms["plain"] / ms["learned_predictor"], 3),
+                pressure_force_rel_diff_max=round(max(fdiff), 6),
+                net_forward_ms=None if fwd_ms is None else round(fwd_ms, 4),
+                net_roofline=None if not fwd_ms else dict(bound="mfma", achieved=round(flops / fwd_ms / 1e9, 2),
+                                                          peak=2500.0, unit="TFLOP/s",
+                                                          frac=round(flops / fwd_ms / 1e9 / 2500.0, 5)))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -261,6 +335,11 @@ def main():
     ap.add_argument("--c3-steps", type=int, default=20)
     ap.add_argument("--no-c5", action="store_true")
     ap.add_argument("--c5-steps", type=int, default=20)
+    ap.add_argument("--no-c4", action="store_true")
+    ap.add_argument("--c4-steps", type=int, default=30)
+    ap.add_argument("--train-warmstart", metavar="OUT", help="train the warm-start network and save its weights")
+    ap.add_argument("--train-seeds", type=int, default=64)
+    ap.add_argument("--train-steps", type=int, default=160)
     ap.add_argument("--copy-calib", action="store_true")
     ap.add_argument("--quiet-kernels", action="store_true")
     ap.add_argument("--summarize", nargs=6, metavar=("STATS", "FETCH", "WRITE", "OUT_MD", "OUT_JSON", "CMD"),
@@ -277,6 +356,9 @@ def main():
         copy_calibration()
     wl = load_package()
     wl.load_library()
+    if a.train_warmstart:
+        train_warmstart(wl, a.train_warmstart, a.train_seeds, a.train_steps)
+        return
     n, R, center, Re, L = (256, 128, 128), 16, (64, 64, 64), 3700.0, 32
     r = run_gpu(wl, n, R, center, Re, L, a.steps, a.warmup)
     roof, rows = roofline(r["stats"], os.path.join(ROOT, "profiles", "hbm_traffic.json"))
@@ -307,6 +389,8 @@ def main():
                                mcells=round(r3m["mcells"], 2), ms_per_step=round(r3m["ms_per_step"], 3),
                                vcycles_per_step=round(r3m["vcycles_per_step"], 3))
         del r3m
+    if not a.no_c4:
+        out["c4"] = run_c4(wl, a.c4_steps, 20)
     if not a.no_c5:
         out["c5"] = run_c5(wl, a.c5_steps, 3)
     if not a.no_cpu:

@@ -165,6 +165,45 @@ int wl_batch_destroy(wl_batch_t b);
 int wl_batch_step(wl_batch_t b, const double* dt, const double* t0, const double* t1, double* dt_next,
                   int16_t* n_iter);
 
+/* Learned warm start of the pressure solves.  No counterpart in the reference, whose warm start is
+ * the previous pressure, x = p·Δt (mom_project!, src/Flow.jl:225).  A network (a bias-free 3D
+ * U-Net of 3×3×3 convolutions, 16/32/64/64 channels, bf16 weights and activations, FP32
+ * accumulation on the MFMA units) maps the warm start's defect, the warm start and the geometry,
+ * on the multigrid's level-1 grid, to a guess of the increment that is added to x before the solver
+ * loop (src/MultiLevelPoisson.jl:108-127), which still iterates to the same Σr² tolerance.  The
+ * network trains on the device (Adam) from pairs the solver records: (input, solved x − warm
+ * start), both on the level-1 grid.
+ *   grid: the network grid, the flow's interior halved in each direction (multiples of 8).
+ *   weights: canonical FP32, the 9 layers concatenated, each w[tap][ci][co] with
+ *     tap = (dz+1)·9 + (dy+1)·3 + (dx+1); (ci, co) = (8,16) (16,16) (16,32) (32,64) (64,64)
+ *     (64,32) (32,16) (16,16) (16,8); input channels 3..7 and output channels 1..7 are padding.
+ *   wl_net_forward / wl_net_gradient: the network on a given input [V][8] (rounded to bf16), V the
+ *     grid's voxel count, index (z·ny + y)·nx + x; out / target: [V]; grad: canonical layout.
+ *   wl_net_train: `steps` Adam steps, each on a pair drawn from the recorded ones; returns the mean
+ *     loss Σ(e−ē)²/Σ(t−t̄)² before each update.
+ *   wl_flow_set_warmstart: attach (net != NULL) or detach a network; flags WL_WARM_*.  A network
+ *     must stay alive while attached; the flow must be 3D with the plain solver. */
+typedef struct wl_net_s* wl_net_t;
+enum wl_warm_flags {
+  WL_WARM_GUESS = 1,  /* add the network's guess before the solver loop */
+  WL_WARM_RECORD = 2, /* record (input, target) pairs of every projection */
+  WL_WARM_STATS = 4,  /* keep Σr² before and after the guess (wl_flow_warm_stats) */
+  WL_WARM_PREDICTOR = 8 /* only in the predictor's projection (w = 1, src/Flow.jl:162) */
+};
+int wl_net_create(wl_net_t* out, const int32_t* grid, int replay_pairs);
+int wl_net_destroy(wl_net_t n);
+int wl_net_num_weights(wl_net_t n, int64_t* count);
+int wl_net_set_weights(wl_net_t n, const float* w, int64_t count);
+int wl_net_get_weights(wl_net_t n, float* w, int64_t count);
+int wl_net_forward(wl_net_t n, const float* in, float* out);
+int wl_net_gradient(wl_net_t n, const float* in, const float* target, double* loss, float* grad);
+int wl_net_adam(wl_net_t n, double lr); /* one Adam step with the last gradient */
+int wl_net_train(wl_net_t n, int steps, double lr, uint64_t seed, double* mean_loss);
+int wl_net_pairs(wl_net_t n, int* count);
+int wl_net_get_pair(wl_net_t n, int index, float* in, float* target); /* 0 = oldest kept */
+int wl_flow_set_warmstart(wl_flow_t h, wl_net_t net, int flags);
+int wl_flow_warm_stats(wl_flow_t h, double* r2, int cap, int* count); /* (before, after) pairs */
+
 /* Σr² residual history of the last mom_step! (both projections); returns the count written. */
 int wl_flow_last_residuals(wl_flow_t h, double* r2, int cap, int* count);
 

@@ -87,6 +87,12 @@ def lib():
             "wlo_flow_set_field_fn": (i, [vp, i, vp, sz]),
             "wlo_bc_fn": (i, [i, i, P(C.c_int32), vp, vp, i, i]),
             "wlo_flow_apply_ic": (i, [vp]),
+            "wlo_nn_num_weights": (C.c_int64, []),
+            "wlo_nn_conv": (None, [P(C.c_int32), i, i, i, i, i, i, vp, vp, vp, vp]),
+            "wlo_nn_forward": (None, [P(C.c_int32), vp, vp, i, vp]),
+            "wlo_nn_gradient": (d, [P(C.c_int32), vp, vp, vp, i, vp]),
+            "wlo_nn_adam": (None, [C.c_int64, vp, vp, vp, vp, C.c_float, C.c_float, C.c_float, C.c_float,
+                                   C.c_int64]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -420,3 +426,51 @@ class OSimulation:
 
     def pressure_force(self):
         return self.flow.pressure_force(self.bodies)
+
+
+# ---------------------------------------------------------------------------------------------
+# The warm-start network (wl_nn_oracle.cpp).  Tensors are C-ordered numpy arrays (V, C), voxel
+# index v = (z·ny + y)·nx + x; weights are the canonical float32 vector of all layers
+# (each layer w[tap][ci][co]).  round=True rounds at the device's bf16 storage points.
+# ---------------------------------------------------------------------------------------------
+NN_LAYERS = [(8, 16), (16, 16), (16, 32), (32, 64), (64, 64), (64, 32), (32, 16), (16, 16), (16, 8)]
+
+
+def _f32c(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def nn_num_weights():
+    return int(lib().wlo_nn_num_weights())
+
+
+def nn_conv(grid, w, a, b=None, relu=True, round=True, outf32=False):
+    ci, cos = w.shape[1], w.shape[2]
+    a, w = _f32c(a), _f32c(w)
+    bb = _f32c(b) if b is not None else None
+    out = np.zeros((a.shape[0], cos), np.float32)
+    g = (C.c_int32 * 3)(*grid)
+    lib().wlo_nn_conv(g, ci, cos, int(b is not None), int(relu), int(round), int(outf32), a.ctypes.data,
+                      bb.ctypes.data if bb is not None else None, w.ctypes.data, out.ctypes.data)
+    return out
+
+
+def nn_forward(grid, w, in0, round=True):
+    w, in0 = _f32c(w), _f32c(in0)
+    out = np.zeros(in0.shape[0], np.float32)
+    lib().wlo_nn_forward((C.c_int32 * 3)(*grid), w.ctypes.data, in0.ctypes.data, int(round), out.ctypes.data)
+    return out
+
+
+def nn_gradient(grid, w, in0, target, round=True, grad=True):
+    w, in0, target = _f32c(w), _f32c(in0), _f32c(target)
+    g = np.zeros(w.size, np.float64) if grad else None
+    L = lib().wlo_nn_gradient((C.c_int32 * 3)(*grid), w.ctypes.data, in0.ctypes.data, target.ctypes.data,
+                              int(round), g.ctypes.data if grad else None)
+    return (L, g) if grad else L
+
+
+def nn_adam(w, g, m, v, lr, t, b1=0.9, b2=0.999, eps=1e-8):
+    """One Adam step in place on float32 arrays w, m, v with gradient g."""
+    g = _f32c(g)
+    lib().wlo_nn_adam(w.size, w.ctypes.data, g.ctypes.data, m.ctypes.data, v.ctypes.data, lr, b1, b2, eps, t)

@@ -15,3 +15,4 @@ from .io import (  # noqa: F401
     VTKWriter, close, default_attrib, load_flow, load_meanflow, load_pvd, read_vti, save, save_flow, save_meanflow,
     vtk_writer, write_vti,
 )
+from .learned import WarmNet, init_weights, set_warmstart, warm_stats  # noqa: F401

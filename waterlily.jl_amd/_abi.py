@@ -18,6 +18,7 @@ WL_BODY_SPHERE, WL_BODY_ELLIPSOID = 1, 2
 WL_ENAN = -3
 MFIELDS = {"P": 0, "U": 1, "UU": 2, "tau": 3}
 WL_FN_UBC, WL_FN_ACC0, WL_FN_ACC1 = 0, 1, 2
+WL_WARM_GUESS, WL_WARM_RECORD, WL_WARM_STATS, WL_WARM_PREDICTOR = 1, 2, 4, 8
 
 
 class WlBody(C.Structure):
@@ -85,6 +86,19 @@ SIGNATURES = {
     "wl_pois_mult": (_i, [_vp, _vp, _vp, _sz]),
     "wl_pois_update": (_i, [_vp]),
     "wl_pois_solve": (_i, [_vp, _d, _i, _P(C.c_int32)]),
+    "wl_net_create": (_i, [_P(_vp), _P(C.c_int32), _i]),
+    "wl_net_destroy": (_i, [_vp]),
+    "wl_net_num_weights": (_i, [_vp, _P(C.c_int64)]),
+    "wl_net_set_weights": (_i, [_vp, _vp, C.c_int64]),
+    "wl_net_get_weights": (_i, [_vp, _vp, C.c_int64]),
+    "wl_net_forward": (_i, [_vp, _vp, _vp]),
+    "wl_net_gradient": (_i, [_vp, _vp, _vp, _P(_d), _vp]),
+    "wl_net_adam": (_i, [_vp, _d]),
+    "wl_net_train": (_i, [_vp, _i, _d, C.c_uint64, _P(_d)]),
+    "wl_net_pairs": (_i, [_vp, _P(_i)]),
+    "wl_net_get_pair": (_i, [_vp, _i, _vp, _vp]),
+    "wl_flow_set_warmstart": (_i, [_vp, _vp, _i]),
+    "wl_flow_warm_stats": (_i, [_vp, _P(_d), _i, _P(_i)]),
 }
 
 _lib = None

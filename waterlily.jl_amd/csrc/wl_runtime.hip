@@ -132,6 +132,8 @@ static void dfree(void* p) {
   if (p) (void)hipFree(p);
 }
 
+#include "wl_net_rt.h"
+
 // ---------------------------------------------------------------------------------------------
 // Per-kernel timing with HIP events on the handle's stream (enabled by wl_flow_profile).  Each
 // record carries the kernel's algorithmic bytes (distinct elements read + written) per launch.
@@ -789,6 +791,12 @@ struct FlowBase {
   virtual void set_field(int which, const void* src, size_t nbytes) = 0;
   virtual void apply_ic() = 0;
   virtual double cfl() = 0;
+  // learned warm start of the pressure solves (wl_flow_set_warmstart)
+  nn::WarmNet* net = nullptr;
+  void* net_owner = nullptr;  // the wl_net_s handle, which lists the flows attached to it
+  int net_flags = 0;
+  std::vector<double> warm_hist;  // Σr² before and after the guess, per projection (WL_WARM_STATS)
+  virtual void check_net(const nn::WarmNet& n) = 0;
 };
 
 static DBodies to_dbodies(const wl_body* b, int nb) {
@@ -1094,6 +1102,36 @@ struct FlowT : FlowBase {
     return nit;
   }
 
+  // The learned warm start: after residual! of x = p·Δt (src/Flow.jl:225), the network's guess
+  // of the increment is added to x and residual! runs again; the solver loop that follows is
+  // unchanged (src/MultiLevelPoisson.jl:108-127), so it iterates to the same tolerance.
+  void check_net(const nn::WarmNet& n) override {
+    if (D != 3) throw WlErr(WL_EINVAL, "the warm-start network is three-dimensional");
+    for (int d = 0; d < 3; ++d)
+      if ((g.N[d] - 2) % 2 != 0 || (g.N[d] - 2) / 2 != (d == 0 ? n.R[0].nx : d == 1 ? n.R[0].ny : n.R[0].nz))
+        throw WlErr(WL_EINVAL, "network grid must be the flow's interior halved in every direction");
+  }
+  void warm_guess() {
+    Level<T>& l0 = mg.lv[0];
+    if (net_flags & WL_WARM_STATS) warm_hist.push_back((double)(T)mg.read_dsc(1));
+    {
+      ProfScope ps(&prof, st, "nn_prep", 0, 0.0);
+      net->prep<T>(g, (const T*)l0.r, (const T*)xw, (const T*)mu0, st);
+    }
+    if (net_flags & WL_WARM_GUESS) {
+      {
+        ProfScope ps(&prof, st, "nn_forward", 0, 0.0);
+        net->forward(st);
+      }
+      {
+        ProfScope ps(&prof, st, "nn_apply", 0, 0.0);
+        net->apply<T>(g, xw, st);
+        mg.residual_generic();
+      }
+    }
+    if (net_flags & WL_WARM_STATS) warm_hist.push_back((double)(T)mg.read_dsc(1));
+  }
+
   // mom_project! (src/Flow.jl:223-232)
   int project(T w, T dt_end, double t1) {
     if (mode == WL_SOLVER_MIXED) return project_mixed(w, dt_end, t1);
@@ -1110,7 +1148,10 @@ struct FlowT : FlowBase {
     mg.perbc(g, xw);
     mg.finish_sum(nb, 0, 1, (double)MG<T>::ninside(g));
     mg.submean_l2();
+    const bool use_net = net && (!(net_flags & WL_WARM_PREDICTOR) || w == T(1));
+    if (use_net) warm_guess();
     const int nit = mg.mg_loop(tol, itmx, &r2hist);
+    if (use_net && (net_flags & WL_WARM_RECORD)) net->record<T>(g, (const T*)xw, st);
     ZT ba = make_zt(box_all(g), 2 * zt_target_default());
     {
     ProfScope ps(&prof, st, "correct", 0, (double)sizeof(T) * (2 * g.n + 3 * D * ncells_inside(g)));
@@ -1811,8 +1852,10 @@ int wl_flow_create(wl_flow_t* out, const wl_flow_desc* d) {
   *out = h;
   API_END
 }
+static void net_detach(FlowBase* f);
 int wl_flow_destroy(wl_flow_t h) {
   API_BEGIN
+  if (h) net_detach(h->impl.get());
   delete h;
   API_END
 }
@@ -1852,6 +1895,7 @@ int wl_flow_pois_update(wl_flow_t h) {
 int wl_flow_set_solver(wl_flow_t h, int mode, double tol, int itmx) {
   API_BEGIN
   if (mode != WL_SOLVER_PLAIN && mode != WL_SOLVER_MIXED) throw WlErr(WL_EINVAL, "unknown solver mode");
+  if (mode != WL_SOLVER_PLAIN && h->impl->net) throw WlErr(WL_EINVAL, "the warm start runs with the plain solver");
   if (itmx < 1) throw WlErr(WL_EINVAL, "itmx must be >= 1");
   h->impl->mode = mode;
   h->impl->tol = tol;
@@ -2198,6 +2242,195 @@ int wl_pois_solve(wl_pois_t h, double tol, int itmx, int32_t* n_iter) {
   API_BEGIN
   if (itmx < 0) throw WlErr(WL_EINVAL, "itmx must be >= 0");
   *n_iter = h->impl->solve(tol, itmx);
+  API_END
+}
+
+// ---- learned warm start ----
+struct wl_net_s {
+  std::unique_ptr<nn::WarmNet> impl;
+  std::vector<FlowBase*> attached;
+};
+// take the flow off its network's list (no-op when none is attached)
+static void net_detach(FlowBase* f) {
+  if (f->net_owner) {
+    auto& v = ((wl_net_s*)f->net_owner)->attached;
+    v.erase(std::remove(v.begin(), v.end(), f), v.end());
+  }
+  f->net = nullptr;
+  f->net_owner = nullptr;
+  f->net_flags = 0;
+}
+int wl_net_create(wl_net_t* out, const int32_t* grid, int replay_pairs) {
+  API_BEGIN
+  if (!out || !grid) throw WlErr(WL_EINVAL, "null argument");
+  if (replay_pairs < 1 || replay_pairs > 100000) throw WlErr(WL_EINVAL, "replay_pairs must be in 1..100000");
+  auto n = std::make_unique<wl_net_s>();
+  const int g[3] = {grid[0], grid[1], grid[2]};
+  n->impl = std::make_unique<nn::WarmNet>(g, replay_pairs);
+  *out = n.release();
+  API_END
+}
+int wl_net_destroy(wl_net_t n) {
+  API_BEGIN
+  if (n)
+    for (FlowBase* f : n->attached) {
+      f->sync();
+      f->net = nullptr;
+      f->net_owner = nullptr;
+      f->net_flags = 0;
+    }
+  delete n;
+  API_END
+}
+int wl_net_num_weights(wl_net_t n, int64_t* count) {
+  API_BEGIN
+  if (!n || !count) throw WlErr(WL_EINVAL, "null argument");
+  *count = n->impl->nweights();
+  API_END
+}
+static void net_check_count(wl_net_t n, int64_t count) {
+  if (!n) throw WlErr(WL_EINVAL, "null handle");
+  if (count != n->impl->nweights()) throw WlErr(WL_EINVAL, "weight count does not match the network");
+}
+int wl_net_set_weights(wl_net_t n, const float* w, int64_t count) {
+  API_BEGIN
+  net_check_count(n, count);
+  if (!w) throw WlErr(WL_EINVAL, "null argument");
+  n->impl->set_weights(w, n->impl->own);
+  API_END
+}
+int wl_net_get_weights(wl_net_t n, float* w, int64_t count) {
+  API_BEGIN
+  net_check_count(n, count);
+  if (!w) throw WlErr(WL_EINVAL, "null argument");
+  n->impl->get(&nn::Layer::w, w, n->impl->own);
+  API_END
+}
+// upload [V][8] floats into in0 (bf16) through a temporary FP32 buffer
+static void net_upload_input(nn::WarmNet& N, const float* in) {
+  const int64_t n = (int64_t)N.V() * 8;
+  float* tmp = dalloc<float>(n);
+  try {
+    HIPCK(hipMemcpy(tmp, in, n * sizeof(float), hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(nn::k_nn_tobf, dim3(nn::nblk256(n)), dim3(256), 0, N.own, n, (const float*)tmp, N.in0);
+    KCK();
+    HIPCK(hipStreamSynchronize(N.own));
+  } catch (...) {
+    dfree(tmp);
+    throw;
+  }
+  dfree(tmp);
+}
+static void net_download_out(nn::WarmNet& N, float* out) {
+  std::vector<float> h((size_t)N.V() * 8);
+  HIPCK(hipMemcpyAsync(h.data(), N.out, h.size() * sizeof(float), hipMemcpyDeviceToHost, N.own));
+  HIPCK(hipStreamSynchronize(N.own));
+  for (int v = 0; v < N.V(); ++v) out[v] = h[(size_t)v * 8];
+}
+int wl_net_forward(wl_net_t n, const float* in, float* out) {
+  API_BEGIN
+  if (!n || !in || !out) throw WlErr(WL_EINVAL, "null argument");
+  nn::WarmNet& N = *n->impl;
+  net_upload_input(N, in);
+  N.forward(N.own);
+  KCK();
+  net_download_out(N, out);
+  API_END
+}
+int wl_net_gradient(wl_net_t n, const float* in, const float* target, double* loss, float* grad) {
+  API_BEGIN
+  if (!n || !in || !target) throw WlErr(WL_EINVAL, "null argument");
+  nn::WarmNet& N = *n->impl;
+  net_upload_input(N, in);
+  HIPCK(hipMemcpy(N.tgt, target, (size_t)N.V() * sizeof(float), hipMemcpyHostToDevice));
+  N.forward(N.own);
+  N.loss(N.tgt, N.own);
+  N.backward(N.own);
+  double h = 0;
+  HIPCK(hipMemcpyAsync(&h, N.dsc, sizeof(double), hipMemcpyDeviceToHost, N.own));
+  HIPCK(hipStreamSynchronize(N.own));
+  if (loss) *loss = h;
+  if (grad) N.get(&nn::Layer::gw, grad, N.own);
+  API_END
+}
+int wl_net_adam(wl_net_t n, double lr) {
+  API_BEGIN
+  if (!n) throw WlErr(WL_EINVAL, "null handle");
+  n->impl->adam((float)lr, n->impl->own);
+  HIPCK(hipStreamSynchronize(n->impl->own));
+  API_END
+}
+int wl_net_train(wl_net_t n, int steps, double lr, uint64_t seed, double* mean_loss) {
+  API_BEGIN
+  if (!n) throw WlErr(WL_EINVAL, "null handle");
+  nn::WarmNet& N = *n->impl;
+  if (N.count < 1) throw WlErr(WL_EINVAL, "no recorded pairs to train on");
+  for (FlowBase* f : n->attached) f->sync();
+  uint64_t s = seed * 0x9E3779B97F4A7C15ull + 0x2545F4914F6CDD1Dull;
+  double acc = 0;
+  for (int k = 0; k < steps; ++k) {
+    s ^= s << 13;  // xorshift64
+    s ^= s >> 7;
+    s ^= s << 17;
+    const int idx = (int)(s % (uint64_t)N.count);
+    const int slot = (N.head - N.count + idx + N.cap) % N.cap;
+    acc += N.train_slot(slot, (float)lr, N.own);
+  }
+  if (mean_loss) *mean_loss = steps > 0 ? acc / steps : 0.0;
+  API_END
+}
+int wl_net_pairs(wl_net_t n, int* count) {
+  API_BEGIN
+  if (!n || !count) throw WlErr(WL_EINVAL, "null argument");
+  for (FlowBase* f : n->attached) f->sync();
+  *count = n->impl->count;
+  API_END
+}
+int wl_net_get_pair(wl_net_t n, int index, float* in, float* target) {
+  API_BEGIN
+  if (!n) throw WlErr(WL_EINVAL, "null handle");
+  nn::WarmNet& N = *n->impl;
+  for (FlowBase* f : n->attached) f->sync();
+  if (index < 0 || index >= N.count) throw WlErr(WL_EINVAL, "pair index out of range");
+  const int slot = (N.head - N.count + index + N.cap) % N.cap;
+  const int64_t V = N.V();
+  if (in) {
+    std::vector<nn::bf16_t> h((size_t)V * 8);
+    HIPCK(hipMemcpy(h.data(), N.rin + (int64_t)slot * V * 8, h.size() * sizeof(nn::bf16_t), hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < h.size(); ++i) in[i] = nn::bf2f(h[i]);
+  }
+  if (target) HIPCK(hipMemcpy(target, N.rtgt + (int64_t)slot * V, V * sizeof(float), hipMemcpyDeviceToHost));
+  API_END
+}
+int wl_flow_set_warmstart(wl_flow_t h, wl_net_t net, int flags) {
+  API_BEGIN
+  if (!h) throw WlErr(WL_EINVAL, "null handle");
+  FlowBase* f = h->impl.get();
+  f->sync();
+  if (net) {
+    if (flags & ~(WL_WARM_GUESS | WL_WARM_RECORD | WL_WARM_STATS | WL_WARM_PREDICTOR))
+      throw WlErr(WL_EINVAL, "unknown warm-start flags");
+    if (f->mode != WL_SOLVER_PLAIN) throw WlErr(WL_EINVAL, "the warm start runs with the plain solver");
+    f->check_net(*net->impl);
+  }
+  net_detach(f);
+  if (net) {
+    f->net = net->impl.get();
+    f->net_owner = net;
+    f->net_flags = flags;
+    net->attached.push_back(f);
+  }
+  f->warm_hist.clear();
+  API_END
+}
+int wl_flow_warm_stats(wl_flow_t h, double* r2, int cap, int* count) {
+  API_BEGIN
+  if (!h || !count) throw WlErr(WL_EINVAL, "null argument");
+  FlowBase* f = h->impl.get();
+  f->sync();
+  const int n = (int)f->warm_hist.size();
+  for (int i = 0; i < std::min(n, cap); ++i) r2[i] = f->warm_hist[i];
+  *count = n;
   API_END
 }
 
