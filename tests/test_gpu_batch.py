@@ -45,9 +45,13 @@ def make(wl, spec):
                          T=np.float32)
 
 
+@pytest.mark.parametrize("graphs", [True, False])
 @pytest.mark.parametrize("D", [2, 3])
-def test_batch_matches_single(wl, D):
-    say(f"case: batch of uneven {D}D simulations vs single stepping, 6 steps")
+def test_batch_matches_single(wl, D, graphs, monkeypatch):
+    """With graphs, every solver iteration after the first of its kind is a replay of the captured
+    graph, between the single flows' eager steps and the downloads of every step."""
+    say(f"case: batch of uneven {D}D simulations vs single stepping, 6 steps, graphs={graphs}")
+    monkeypatch.setenv("WL_BATCH_GRAPHS", "1" if graphs else "0")
     specs = members(wl, D, seed=D)
     batched = [make(wl, s) for s in specs]
     single = [make(wl, s) for s in specs]

@@ -1286,6 +1286,12 @@ struct BatchT : BatchBase {
   char* dyn = nullptr;   // device arguments of one step
   char* hdyn = nullptr;  // pinned staging
   size_t dyn_bytes = 0;
+  // Each kind of solver iteration (first, later) is captured once as a HIP graph and replayed:
+  // its launches read their arguments and the per-member flags from fixed device buffers, so a
+  // replay in any later step runs what a direct launch would.  WL_BATCH_GRAPHS=0 (read at
+  // creation) launches directly instead.
+  bool use_graphs = true;
+  hipGraphExec_t giter[2] = {nullptr, nullptr};
 
   template <class U>
   U* upload(const std::vector<U>& v) {
@@ -1311,6 +1317,10 @@ struct BatchT : BatchBase {
   BatchT(const std::vector<FlowT<T>*>& flows) : F(flows) {
     ns = (int)F.size();
     D = F[0]->D;
+    {
+      const char* e = std::getenv("WL_BATCH_GRAPHS");
+      use_graphs = !e || std::atoi(e) != 0;
+    }
     scheme = F[0]->desc.scheme;
     HIPCK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
     for (auto* f : F) {
@@ -1333,6 +1343,8 @@ struct BatchT : BatchBase {
   }
   ~BatchT() override {
     if (st) (void)hipStreamSynchronize(st);
+    for (auto& g : giter)
+      if (g) (void)hipGraphExecDestroy(g);
     for (void* p : owned) dfree(p);
     dfree(dyn);
     if (hdyn) (void)hipHostFree(hdyn);
@@ -1663,6 +1675,45 @@ struct BatchT : BatchBase {
       }
     }
   }
+  // one solver! iteration of every still-iterating member (src/MultiLevelPoisson.jl:113-123)
+  void iteration(bool first) {
+    if (!first) {
+      hipLaunchKernelGGL(kb_eps_init<T>, dim3(eps0.nblk), kBlock, 0, st, (const ALev<T>*)eps0.args, eps0.blk,
+                         (const int*)act);
+      KCK();
+    }
+    vcycle();
+    gs_sweeps(0);
+    hipLaunchKernelGGL((kb_increment<T, true>), dim3(incr0.nblk), kBlock, 0, st, (const ALev<T>*)incr0.args,
+                       incr0.blk, (const int*)act);
+    KCK();
+    hipLaunchKernelGGL(kb_finish_sum<T>, dim3(ns), dim3(256), 0, st, fin_l2, (const int*)act);
+    KCK();
+    hipLaunchKernelGGL(kb_mg_control<T>, dim3((ns + 63) / 64), dim3(64), 0, st, ctl, ns, act, cnt);
+    KCK();
+  }
+  void run_iteration(bool first) {
+    if (!use_graphs) {
+      iteration(first);
+      return;
+    }
+    hipGraphExec_t& g = giter[first ? 0 : 1];
+    if (!g) {
+      hipGraph_t gr;
+      HIPCK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+      try {
+        iteration(first);
+      } catch (...) {
+        (void)hipStreamEndCapture(st, &gr);
+        throw;
+      }
+      HIPCK(hipStreamEndCapture(st, &gr));
+      const hipError_t e = hipGraphInstantiate(&g, gr, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(gr);
+      HIPCK(e);
+    }
+    HIPCK(hipGraphLaunch(g, st));
+  }
   // mom_project! (src/Flow.jl:223-232) for every member; returns the V-cycle counts
   std::vector<int> project(const ARes<T>* ra, const ACorr<T>* ka, const ABc<T>* bca) {
     hipLaunchKernelGGL(kb_residual<T>, dim3(nb_res), kBlock, 0, st, ra, blk_res);
@@ -1679,20 +1730,7 @@ struct BatchT : BatchBase {
     int itmax = 0;
     for (auto* f : F) itmax = std::max(itmax, f->itmx);
     for (int it = 0; it < itmax; ++it) {
-      if (it > 0) {
-        hipLaunchKernelGGL(kb_eps_init<T>, dim3(eps0.nblk), kBlock, 0, st, (const ALev<T>*)eps0.args, eps0.blk,
-                           (const int*)act);
-        KCK();
-      }
-      vcycle();
-      gs_sweeps(0);
-      hipLaunchKernelGGL((kb_increment<T, true>), dim3(incr0.nblk), kBlock, 0, st, (const ALev<T>*)incr0.args,
-                         incr0.blk, (const int*)act);
-      KCK();
-      hipLaunchKernelGGL(kb_finish_sum<T>, dim3(ns), dim3(256), 0, st, fin_l2, (const int*)act);
-      KCK();
-      hipLaunchKernelGGL(kb_mg_control<T>, dim3((ns + 63) / 64), dim3(64), 0, st, ctl, ns, act, cnt);
-      KCK();
+      run_iteration(it == 0);
       HIPCK(hipMemcpyAsync(hact, act, ns * sizeof(int), hipMemcpyDeviceToHost, st));
       HIPCK(hipStreamSynchronize(st));
       bool any = false;
