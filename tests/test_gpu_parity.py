@@ -91,6 +91,36 @@ def test_c1_circle_trajectory(wl):
     assert sim.pois.n == osim.n
 
 
+def test_c1_circle_1000_steps_forces(wl):
+    """The 2D circle (192x128, r=16 at (63,63), Re 250, Float32) for 1000 steps: pressure_force
+    at every step within 1e-3 (relative to the run's largest force) of the Float32 oracle, every
+    V-cycle count equal; the Float64 oracle's force deviation is reported beside it."""
+    say("case: 2D circle 192x128, 1000 steps, pressure_force every step")
+    sim, osim = oracle_pair(wl, (192, 128), (1, 0), 16, (63, 63), 32 / 250, L=32)
+    o64 = O.OSimulation((192, 128), (1, 0), 32, nu=32 / 250, dtype=O.WL_F64,
+                        bodies=[{"kind": 1, "center": (63, 63), "radius": 16}])
+    Fg, F32, F64 = [], [], []
+    for k in range(1000):
+        wl.sim_step(sim, remeasure=False)
+        osim.step(remeasure=False)
+        o64.step(remeasure=False)
+        Fg.append(wl.pressure_force(sim))
+        F32.append(osim.pressure_force())
+        F64.append(o64.pressure_force())
+        if k % 250 == 249:
+            say(f"  step {k + 1}: F gpu {Fg[-1]} f32 {F32[-1]} f64 {F64[-1]}")
+    Fg, F32, F64 = np.array(Fg), np.array(F32), np.array(F64)
+    scale = np.max(np.abs(F32))
+    e32 = float(np.max(np.abs(Fg - F32)) / scale)
+    e64 = float(np.max(np.abs(Fg - F64)) / scale)
+    same = int(np.sum(np.all(Fg == F32, axis=1)))
+    say(f"  max force error vs F32 oracle {e32:.3e} ({same}/1000 steps bit-identical); vs F64 oracle "
+        f"{e64:.3e}; u rel-L2 vs F32 {rel(sim.flow.u, osim.flow.get('u')):.3e}, vs F64 "
+        f"{rel(sim.flow.u, o64.flow.get('u')):.3e}")
+    assert e32 <= 1e-3
+    assert sim.pois.n == osim.n
+
+
 def test_c2_sphere_one_step(wl):
     """3D sphere 256x128x128, Re 3700: one step against the Float32 oracle; error against the
     Float64 oracle reported."""
