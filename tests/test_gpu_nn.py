@@ -42,7 +42,8 @@ def _case(wl, grid, seed):
     return w, x, rng
 
 
-@pytest.mark.parametrize("grid", [(16, 16, 8), (24, 8, 16)])
+# grids below 2^16 voxels run the one-tile-per-wave convolution; (64, 32, 32) the 4-tile one
+@pytest.mark.parametrize("grid", [(16, 16, 8), (24, 8, 16), (64, 32, 32)])
 def test_forward_matches_oracle(wl, grid):
     say(f"case: network forward, grid {grid}")
     w, x, _ = _case(wl, grid, 1)
@@ -54,7 +55,7 @@ def test_forward_matches_oracle(wl, grid):
     assert e <= 5e-3 and np.linalg.norm(o) > 0
 
 
-@pytest.mark.parametrize("grid", [(16, 16, 8), (24, 8, 16)])
+@pytest.mark.parametrize("grid", [(16, 16, 8), (24, 8, 16), (64, 32, 32)])
 def test_gradient_matches_oracle(wl, grid):
     say(f"case: network gradient, grid {grid}")
     w, x, rng = _case(wl, grid, 2)

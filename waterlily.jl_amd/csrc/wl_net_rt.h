@@ -128,6 +128,12 @@ struct Layer {
 };
 
 static inline unsigned nblk256(int64_t n) { return (unsigned)((n + 255) / 256); }
+// voxel-chunk blocks of a weight-gradient launch: enough to fill the chip, few enough that the
+// per-block partial sums stay near 16 MB (their reduction reads them all)
+static inline int dw_blocks(int64_t nchunks, int64_t nw) {
+  const int64_t want = std::min<int64_t>(2048, std::max<int64_t>(64, (int64_t(4) << 20) / nw));
+  return (int)std::min<int64_t>(nchunks, want);
+}
 
 struct WarmNet {
   NGrid R[4];
@@ -135,6 +141,7 @@ struct WarmNet {
   hipStream_t own = nullptr;
   // activations (channels-last bf16) and the FP32 output
   bf16_t *in0, *a0, *a0b, *p1, *a1, *p2, *a2, *p3, *a3, *u2, *u1, *u0;
+  bf16_t *s6, *s7, *s8;  // skip inputs a2 + up a3, a1 + up u2, a0b + up u1
   float* out;
   // gradients
   bf16_t *gout, *g_u0, *gpre8, *g_s8, *g_u1, *gpre7, *g_s7, *g_u2, *gpre6, *g_s6, *g_a3, *gpre5, *g_p3, *gpre4,
@@ -185,7 +192,7 @@ struct WarmNet {
       y.wf = alloc<bf16_t>((int64_t)y.s.co * y.kp);
       if (l > 0) y.wb = alloc<bf16_t>((int64_t)y.s.ci * y.kpb);
       const int64_t ch = (int64_t)(R[y.s.res].v + 31) / 32;
-      dwpart_n = std::max(dwpart_n, std::min<int64_t>(ch, 256) * y.nw);
+      dwpart_n = std::max(dwpart_n, (int64_t)dw_blocks(ch, y.nw) * y.nw);
     }
     const int64_t v0 = R[0].v, v1 = R[1].v, v2 = R[2].v, v3 = R[3].v;
     in0 = alloc<bf16_t>(v0 * 8);
@@ -200,6 +207,9 @@ struct WarmNet {
     u2 = alloc<bf16_t>(v2 * 32);
     u1 = alloc<bf16_t>(v1 * 16);
     u0 = alloc<bf16_t>(v0 * 16);
+    s6 = alloc<bf16_t>(v2 * 64);
+    s7 = alloc<bf16_t>(v1 * 32);
+    s8 = alloc<bf16_t>(v0 * 16);
     out = alloc<float>(v0 * 8);
     gout = alloc<bf16_t>(v0 * 8);
     g_u0 = alloc<bf16_t>(v0 * 16);
@@ -272,26 +282,38 @@ struct WarmNet {
     HIPCK(hipStreamSynchronize(st));
   }
 
+  // grids of fewer than 2^16 voxels run one 16-voxel tile and one column tile per wave
   template <int CI, int CO, int COS, bool SKIP, bool RELU, bool F32>
   void conv(int res, const bf16_t* a, const bf16_t* b, const bf16_t* W, void* o, hipStream_t st) {
-    hipLaunchKernelGGL((k_nn_conv<CI, CO, COS, SKIP, RELU, F32>), dim3(nblk256(R[res].v)), dim3(256), 0, st, R[res], a,
-                       b, W, o);
+    if (R[res].v >= (1 << 16)) {
+      hipLaunchKernelGGL((k_nn_conv<CI, CO, COS, SKIP, RELU, F32, 4, CO / 16>), dim3(nblk256(R[res].v)), dim3(256), 0,
+                         st, R[res], a, b, W, o);
+    } else {
+      hipLaunchKernelGGL((k_nn_conv<CI, CO, COS, SKIP, RELU, F32, 1, 1>), dim3((R[res].v + 63) / 64, CO / 16), dim3(256),
+                         0, st, R[res], a, b, W, o);
+    }
     KCK();
   }
   template <int CI, int CO, int COS, bool SKIP, int TPW>
   void dw(int l, const bf16_t* a, const bf16_t* b, const bf16_t* g, hipStream_t st) {
     const Layer& y = L[l];
     const NGrid& n = R[y.s.res];
-    const int gx = (int)std::min<int64_t>((n.v + 31) / 32, 256);
+    const int gx = dw_blocks((n.v + 31) / 32, y.nw);
     hipLaunchKernelGGL((k_nn_dw<CI, CO, COS, SKIP, TPW>), dim3(gx, (27 + TPW - 1) / TPW), dim3(256), 0, st, n, a, b, g,
                        dwpart);
     KCK();
-    hipLaunchKernelGGL(k_nn_reduce, dim3(nblk256(y.nw)), dim3(256), 0, st, (const float*)dwpart, gx, (int)y.nw, y.gw);
+    hipLaunchKernelGGL(k_nn_reduce, dim3((unsigned)((y.nw + 31) / 32)), dim3(256), 0, st, (const float*)dwpart, gx,
+                       (int)y.nw, y.gw);
     KCK();
   }
   template <int C>
   void avgpool(int res, const bf16_t* in, bf16_t* o, hipStream_t st) {
     hipLaunchKernelGGL(k_nn_avgpool<C>, dim3(nblk256((int64_t)R[res].v * C)), dim3(256), 0, st, R[res], in, o);
+    KCK();
+  }
+  template <int C>
+  void skipsum(int res, const bf16_t* a, const bf16_t* b, bf16_t* o, hipStream_t st) {
+    hipLaunchKernelGGL(k_nn_skipsum<C>, dim3(nblk256((int64_t)R[res].v * (C / 8))), dim3(256), 0, st, R[res], a, b, o);
     KCK();
   }
   template <int C>
@@ -316,9 +338,12 @@ struct WarmNet {
     conv<32, 64, 64, false, true, false>(2, p2, nullptr, L[3].wf, a2, st);
     avgpool<64>(3, a2, p3, st);
     conv<64, 64, 64, false, true, false>(3, p3, nullptr, L[4].wf, a3, st);
-    conv<64, 32, 32, true, true, false>(2, a2, a3, L[5].wf, u2, st);
-    conv<32, 16, 16, true, true, false>(1, a1, u2, L[6].wf, u1, st);
-    conv<16, 16, 16, true, true, false>(0, a0b, u1, L[7].wf, u0, st);
+    skipsum<64>(2, a2, a3, s6, st);
+    conv<64, 32, 32, false, true, false>(2, s6, nullptr, L[5].wf, u2, st);
+    skipsum<32>(1, a1, u2, s7, st);
+    conv<32, 16, 16, false, true, false>(1, s7, nullptr, L[6].wf, u1, st);
+    skipsum<16>(0, a0b, u1, s8, st);
+    conv<16, 16, 16, false, true, false>(0, s8, nullptr, L[7].wf, u0, st);
     conv<16, 16, 8, false, false, true>(0, u0, nullptr, L[8].wf, out, st);
   }
   // loss of out against t (into dsc[0]) and its gradient into gout
@@ -326,7 +351,7 @@ struct WarmNet {
     const int nb = (int)std::min<int64_t>(nblk256(V()), 1024);
     hipLaunchKernelGGL(k_nn_loss_part, dim3(nb), dim3(256), 0, st, V(), (const float*)out, 8, t, part);
     KCK();
-    hipLaunchKernelGGL(k_nn_loss_finish, dim3(1), dim3(64), 0, st, (const double*)part, nb, V(), dsc);
+    hipLaunchKernelGGL(k_nn_loss_finish, dim3(1), dim3(256), 0, st, (const double*)part, nb, V(), dsc);
     KCK();
     hipLaunchKernelGGL(k_nn_loss_grad, dim3(nblk256(V())), dim3(256), 0, st, V(), (const float*)out, 8, t,
                        (const double*)dsc, gout, 8);
@@ -337,15 +362,15 @@ struct WarmNet {
     dw<16, 16, 8, false, 27>(8, u0, nullptr, gout, st);
     conv<8, 16, 16, false, false, false>(0, gout, nullptr, L[8].wb, g_u0, st);
     gcomb<16>(0, g_u0, nullptr, 0.f, u0, gpre8, st);
-    dw<16, 16, 16, true, 27>(7, a0b, u1, gpre8, st);
+    dw<16, 16, 16, false, 27>(7, s8, nullptr, gpre8, st);
     conv<16, 16, 16, false, false, false>(0, gpre8, nullptr, L[7].wb, g_s8, st);
     sumpool<16>(1, g_s8, g_u1, st);
     gcomb<16>(1, g_u1, nullptr, 0.f, u1, gpre7, st);
-    dw<32, 16, 16, true, 9>(6, a1, u2, gpre7, st);
+    dw<32, 16, 16, false, 9>(6, s7, nullptr, gpre7, st);
     conv<16, 32, 32, false, false, false>(1, gpre7, nullptr, L[6].wb, g_s7, st);
     sumpool<32>(2, g_s7, g_u2, st);
     gcomb<32>(2, g_u2, nullptr, 0.f, u2, gpre6, st);
-    dw<64, 32, 32, true, 3>(5, a2, a3, gpre6, st);
+    dw<64, 32, 32, false, 3>(5, s6, nullptr, gpre6, st);
     conv<32, 64, 64, false, false, false>(2, gpre6, nullptr, L[5].wb, g_s6, st);
     sumpool<64>(3, g_s6, g_a3, st);
     gcomb<64>(3, g_a3, nullptr, 0.f, a3, gpre5, st);

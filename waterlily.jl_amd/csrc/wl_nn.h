@@ -70,14 +70,16 @@ __device__ inline bool nn_inside(const NGrid& g, int x, int y, int z) {
 
 // 3×3×3 convolution, CI input channels → CO (a multiple of 16) MFMA columns, of which COS are
 // stored.  W is [CO][KP] bf16, k contiguous (KP = 27·CI rounded up to 32, zero padded).  Each wave
-// computes 4 tiles of 16 voxels × all CO columns; one block = 4 waves = 256 voxels.
-// OUTF32: FP32 output (the network's last layer); RELU: max(y, 0) before the store.
-template <int CI, int CO, int COS, bool SKIP, bool RELU, bool OUTF32>
+// computes MT tiles of 16 voxels × NT column tiles, the column tiles starting at blockIdx.y·NT
+// (grid.y = CO/16/NT); one block = 4 waves.  Small grids take MT = 1 and one column tile per wave
+// so that enough waves run.  OUTF32: FP32 output (the network's last layer); RELU: max(y, 0).
+template <int CI, int CO, int COS, bool SKIP, bool RELU, bool OUTF32, int MT = 4, int NT = CO / 16>
 __global__ void __launch_bounds__(256) k_nn_conv(NGrid g, const bf16_t* __restrict__ a, const bf16_t* __restrict__ b,
                                                  const bf16_t* __restrict__ W, void* __restrict__ out) {
-  constexpr int K = 27 * CI, KP = (K + 31) / 32 * 32, NKC = KP / 32, NT = CO / 16, MT = 4;
-  static_assert(CI % 8 == 0 && CO % 16 == 0 && COS <= CO, "channel counts");
+  constexpr int K = 27 * CI, KP = (K + 31) / 32 * 32, NKC = KP / 32;
+  static_assert(CI % 8 == 0 && CO % 16 == 0 && COS <= CO && (CO / 16) % NT == 0, "channel counts");
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int n0 = blockIdx.y * NT;  // first column tile of this block
   const int vbase = (blockIdx.x * 4 + wave) * (16 * MT);
   if (vbase >= g.v) return;  // wave-uniform
   const int row = lane & 15, kq = lane >> 4;
@@ -108,7 +110,7 @@ __global__ void __launch_bounds__(256) k_nn_conv(NGrid g, const bf16_t* __restri
     }
     bf16x8 Bv[NT];
 #pragma unroll
-    for (int n = 0; n < NT; ++n) Bv[n] = *(const bf16x8*)(W + (int64_t)(n * 16 + row) * KP + k);
+    for (int n = 0; n < NT; ++n) Bv[n] = *(const bf16x8*)(W + (int64_t)((n0 + n) * 16 + row) * KP + k);
 #pragma unroll
     for (int m = 0; m < MT; ++m)
 #pragma unroll
@@ -121,7 +123,7 @@ __global__ void __launch_bounds__(256) k_nn_conv(NGrid g, const bf16_t* __restri
     for (int n = 0; n < NT; ++n)
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
-        const int v = vbase + m * 16 + kq * 4 + e, co = n * 16 + row;
+        const int v = vbase + m * 16 + kq * 4 + e, co = (n0 + n) * 16 + row;
         if (v < g.v && co < COS) {
           float y = acc[m][n][e];
           if (RELU) y = y > 0.f ? y : 0.f;
@@ -150,39 +152,49 @@ __global__ void __launch_bounds__(256) k_nn_dw(NGrid g, const bf16_t* __restrict
   f32x4 acc[TPWAVE];
 #pragma unroll
   for (int t = 0; t < TPWAVE; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+  // transposes of 8 channels × 4 voxels: one 8-byte LDS store per channel
+  auto put4 = [](bf16_t* dst0, int stride, const uint4* w) {  // w[j]: 8 channels of voxel j
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      const int sh = (e & 1) * 16;
+      const uint32_t lo = ((((const uint32_t*)&w[0])[e >> 1] >> sh) & 0xFFFFu) |
+                          (((((const uint32_t*)&w[1])[e >> 1] >> sh) & 0xFFFFu) << 16);
+      const uint32_t hi = ((((const uint32_t*)&w[2])[e >> 1] >> sh) & 0xFFFFu) |
+                          (((((const uint32_t*)&w[3])[e >> 1] >> sh) & 0xFFFFu) << 16);
+      *(uint2*)(dst0 + e * stride) = make_uint2(lo, hi);
+    }
+  };
   for (int c = blockIdx.x; c < nchunks; c += gridDim.x) {
     const int v0 = c * 32;
-    for (int q = tid; q < 32 * (CO / 8); q += 256) {
-      const int vl = q / (CO / 8), cv = q % (CO / 8);
-      const int v = v0 + vl;
-      uint4 w = make_uint4(0, 0, 0, 0);
-      if (v < g.v && cv * 8 < COS) w = *(const uint4*)(gout + (int64_t)v * COS + cv * 8);
-      const uint32_t ww[4] = {w.x, w.y, w.z, w.w};
+    for (int q = tid; q < 8 * (CO / 8); q += 256) {  // q = (channel group, 4-voxel group)
+      const int vg = q & 7, cv = q >> 3;
+      uint4 w[4];
 #pragma unroll
-      for (int e = 0; e < 4; ++e) {
-        gT[cv * 8 + 2 * e][vl] = (bf16_t)(ww[e] & 0xFFFFu);
-        gT[cv * 8 + 2 * e + 1][vl] = (bf16_t)(ww[e] >> 16);
+      for (int j = 0; j < 4; ++j) {
+        const int v = v0 + 4 * vg + j;
+        w[j] = make_uint4(0, 0, 0, 0);
+        if (v < g.v && cv * 8 < COS) w[j] = *(const uint4*)(gout + (int64_t)v * COS + cv * 8);
       }
+      put4(&gT[cv * 8][4 * vg], 32, w);
     }
-    for (int q = tid; q < 32 * (NTT * 2); q += 256) {
-      const int vl = q / (NTT * 2), nv = q % (NTT * 2);
+    for (int q = tid; q < 8 * (NTT * 2); q += 256) {
+      const int vg = q & 7, nv = q >> 3;
       const int n0 = nv * 8, tl = n0 / CI, ci0 = n0 % CI, tap = tap0 + tl;
-      const int v = v0 + vl;
-      uint4 w = make_uint4(0, 0, 0, 0);
-      if (v < g.v && tl < TPW && tap < 27) {
-        int x, y, z;
-        nn_xyz(g, v, x, y, z);
-        x += tap % 3 - 1;
-        y += (tap / 3) % 3 - 1;
-        z += tap / 9 - 1;
-        if (nn_inside(g, x, y, z)) w = nn_load8<CI, SKIP>(g, a, b, x, y, z, ci0);
-      }
-      const uint32_t ww[4] = {w.x, w.y, w.z, w.w};
+      uint4 w[4];
 #pragma unroll
-      for (int e = 0; e < 4; ++e) {
-        iT[n0 + 2 * e][vl] = (bf16_t)(ww[e] & 0xFFFFu);
-        iT[n0 + 2 * e + 1][vl] = (bf16_t)(ww[e] >> 16);
+      for (int j = 0; j < 4; ++j) {
+        const int v = v0 + 4 * vg + j;
+        w[j] = make_uint4(0, 0, 0, 0);
+        if (v < g.v && tl < TPW && tap < 27) {
+          int x, y, z;
+          nn_xyz(g, v, x, y, z);
+          x += tap % 3 - 1;
+          y += (tap / 3) % 3 - 1;
+          z += tap / 9 - 1;
+          if (nn_inside(g, x, y, z)) w[j] = nn_load8<CI, SKIP>(g, a, b, x, y, z, ci0);
+        }
       }
+      put4(&iT[n0][4 * vg], 32, w);
     }
     __syncthreads();
 #pragma unroll
@@ -214,14 +226,23 @@ __global__ void __launch_bounds__(256) k_nn_dw(NGrid g, const bf16_t* __restrict
   }
 }
 
-// out[i] = Σ_p partial[p][i], p ascending
+// out[i] = Σ_p partial[p][i]: a block takes 32 entries i; its 8 thread rows sum the partials
+// p ≡ row (mod 8) in ascending order, then the 8 row sums are added in row order (deterministic)
 __global__ void __launch_bounds__(256) k_nn_reduce(const float* __restrict__ partial, int nparts, int n,
                                                    float* __restrict__ out) {
-  const int i = blockIdx.x * 256 + threadIdx.x;
-  if (i >= n) return;
+  __shared__ float sh[8][32];
+  const int c = threadIdx.x & 31, rr = threadIdx.x >> 5;
+  const int i = blockIdx.x * 32 + c;
   float s = 0.f;
-  for (int p = 0; p < nparts; ++p) s += partial[(int64_t)p * n + i];
-  out[i] = s;
+  if (i < n)
+    for (int p = rr; p < nparts; p += 8) s += partial[(int64_t)p * n + i];
+  sh[rr][c] = s;
+  __syncthreads();
+  if (rr == 0 && i < n) {
+    float t = sh[0][c];
+    for (int q = 1; q < 8; ++q) t += sh[q][c];
+    out[i] = t;
+  }
 }
 
 // 2×2×2 average (forward pooling): out[w][c] = bf16(0.125 · Σ children), children in z, y, x order
@@ -238,6 +259,18 @@ __global__ void __launch_bounds__(256) k_nn_avgpool(NGrid coarse, const bf16_t* 
     for (int dy = 0; dy < 2; ++dy)
       for (int dx = 0; dx < 2; ++dx) s += bf2f(in[((int64_t)((2 * z + dz) * fy + 2 * y + dy) * fx + 2 * x + dx) * C + c]);
   out[i] = f2bf(0.125f * s);
+}
+
+// the U-Net skip input s[v][c] = bf16(a[v][c] + b[parent(v)][c]) (FP32 sum), formed once per pass
+template <int C>
+__global__ void __launch_bounds__(256) k_nn_skipsum(NGrid g, const bf16_t* __restrict__ a, const bf16_t* __restrict__ b,
+                                                    bf16_t* __restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;  // one 8-channel vector
+  if (i >= (int64_t)g.v * (C / 8)) return;
+  const int v = (int)(i / (C / 8)), cv = (int)(i % (C / 8));
+  int x, y, z;
+  nn_xyz(g, v, x, y, z);
+  *(uint4*)(out + (int64_t)v * C + cv * 8) = nn_load8<C, true>(g, a, b, x, y, z, cv * 8);
 }
 
 // Σ of the 8 children (the adjoint of the skip input's upsampling), same order
@@ -297,12 +330,24 @@ __global__ void __launch_bounds__(256) k_nn_loss_part(int V, const float* __rest
   }
   if (threadIdx.x < 4) part[4 * blockIdx.x + threadIdx.x] = sh[threadIdx.x][0];
 }
-// sc[0] = loss = Σ(e−ē)² / Σ(t−t̄)², sc[1] = ē, sc[2] = 2 / Σ(t−t̄)² (the gradient scale)
-__global__ void k_nn_loss_finish(const double* __restrict__ part, int nblk, int V, double* __restrict__ sc) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  double s[4] = {0, 0, 0, 0};
-  for (int b = 0; b < nblk; ++b)
-    for (int k = 0; k < 4; ++k) s[k] += part[4 * b + k];
+// sc[0] = loss = Σ(e−ē)² / Σ(t−t̄)², sc[1] = ē, sc[2] = 2 / Σ(t−t̄)² (the gradient scale); one
+// block of 256 threads, each summing a strided set of block partials, then a fixed-order tree
+__global__ void __launch_bounds__(256) k_nn_loss_finish(const double* __restrict__ part, int nblk, int V,
+                                                        double* __restrict__ sc) {
+  __shared__ double sh[4][256];
+  double a[4] = {0, 0, 0, 0};
+  for (int b = threadIdx.x; b < nblk; b += 256)
+    for (int k = 0; k < 4; ++k) a[k] += part[4 * b + k];
+  for (int k = 0; k < 4; ++k) sh[k][threadIdx.x] = a[k];
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if ((int)threadIdx.x < st)
+      for (int k = 0; k < 4; ++k) sh[k][threadIdx.x] += sh[k][threadIdx.x + st];
+    __syncthreads();
+  }
+  if (threadIdx.x != 0) return;
+  double s[4];
+  for (int k = 0; k < 4; ++k) s[k] = sh[k][0];
   const double eb = s[0] / V, tb = s[2] / V;
   const double st = s[3] - V * tb * tb;
   const double se = s[1] - V * eb * eb;
