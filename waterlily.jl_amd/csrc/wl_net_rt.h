@@ -70,12 +70,24 @@ __global__ void __launch_bounds__(256) k_nn_prep(Geo g, NGrid n, const T* __rest
   }
   if (threadIdx.x < 3) part[3 * blockIdx.x + threadIdx.x] = sh[threadIdx.x][0];
 }
-// scal = (s_r, 1/s_r, mean x, 1/s_x) (0 in place of 1/0)
-__global__ void k_nn_prep_finish(const double* __restrict__ part, int nblk, int V, float* __restrict__ scal) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  double s[3] = {0, 0, 0};
-  for (int b = 0; b < nblk; ++b)
-    for (int k = 0; k < 3; ++k) s[k] += part[3 * b + k];
+// scal = (s_r, 1/s_r, mean x, 1/s_x) (0 in place of 1/0); one block of 256 threads, strided
+// partial sums, then a fixed-order tree
+__global__ void __launch_bounds__(256) k_nn_prep_finish(const double* __restrict__ part, int nblk, int V,
+                                                        float* __restrict__ scal) {
+  __shared__ double sh[3][256];
+  double a[3] = {0, 0, 0};
+  for (int b = threadIdx.x; b < nblk; b += 256)
+    for (int k = 0; k < 3; ++k) a[k] += part[3 * b + k];
+  for (int k = 0; k < 3; ++k) sh[k][threadIdx.x] = a[k];
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if ((int)threadIdx.x < st)
+      for (int k = 0; k < 3; ++k) sh[k][threadIdx.x] += sh[k][threadIdx.x + st];
+    __syncthreads();
+  }
+  if (threadIdx.x != 0) return;
+  double s[3];
+  for (int k = 0; k < 3; ++k) s[k] = sh[k][0];
   const double sr = sqrt(s[0] / V), mx = s[1] / V;
   const double vx = s[2] / V - mx * mx, sx = vx > 0 ? sqrt(vx) : 0.0;
   scal[0] = (float)sr;
@@ -418,7 +430,7 @@ struct WarmNet {
     const unsigned nb = nblk256(V());
     hipLaunchKernelGGL(k_nn_prep<T>, dim3(nb), dim3(256), 0, st, g, R[0], r, x, Lmu, X, part);
     KCK();
-    hipLaunchKernelGGL(k_nn_prep_finish, dim3(1), dim3(64), 0, st, (const double*)part, (int)nb, V(), scal);
+    hipLaunchKernelGGL(k_nn_prep_finish, dim3(1), dim3(256), 0, st, (const double*)part, (int)nb, V(), scal);
     KCK();
     hipLaunchKernelGGL(k_nn_norm, dim3(nb), dim3(256), 0, st, V(), (const float*)X, (const float*)scal, in0);
     KCK();
