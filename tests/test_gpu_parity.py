@@ -121,6 +121,30 @@ def test_c1_circle_1000_steps_forces(wl):
     assert sim.pois.n == osim.n
 
 
+def test_c2_sphere_ten_steps(wl):
+    """3D sphere 256x128x128, Re 3700: ten steps with u, p and pressure_force compared at every
+    step against the Float32 oracle (bit for bit here; the bars are rel-L2 1e-5 and force 1e-3);
+    the Float64 oracle's deviation is reported."""
+    say("case: 3D sphere 256x128x128, 10 steps")
+    n, nu = (256, 128, 128), 32 / 3700
+    sim, osim = oracle_pair(wl, n, (1, 0, 0), 16, (64, 64, 64), nu)
+    o64 = O.OSimulation(n, (1, 0, 0), 32, nu=nu, dtype=O.WL_F64,
+                        bodies=[{"kind": 1, "center": (64, 64, 64), "radius": 16}])
+    for k in range(10):
+        wl.sim_step(sim, remeasure=False)
+        osim.step(remeasure=False)
+        o64.step(remeasure=False)
+        u, p = sim.flow.u, sim.flow.p
+        eu, ep = rel(u, osim.flow.get("u")), rel(p, osim.flow.get("p"))
+        F, Fo, F64 = wl.pressure_force(sim), osim.pressure_force(), o64.pressure_force()
+        ef = float(np.max(np.abs(F - Fo)) / max(np.max(np.abs(Fo)), 1e-12))
+        ef64 = float(np.max(np.abs(F - F64)) / max(np.max(np.abs(F64)), 1e-12))
+        say(f"  step {k + 1}: vs F32 u {eu:.1e} p {ep:.1e} F {ef:.1e}; vs F64 u {rel(u, o64.flow.get('u')):.1e} "
+            f"p {rel(p, o64.flow.get('p')):.1e} F {ef64:.1e}; n {sim.pois.n[-2:]}")
+        assert eu <= 1e-5 and ep <= 1e-5 and ef <= 1e-3
+    assert sim.pois.n == osim.n
+
+
 def test_c2_sphere_one_step(wl):
     """3D sphere 256x128x128, Re 3700: one step against the Float32 oracle; error against the
     Float64 oracle reported."""
