@@ -1878,7 +1878,8 @@ __global__ void __launch_bounds__(256) k_residual64(Geo g, ZT t, LRows<T> LR, co
                                                     T* __restrict__ z, const T* __restrict__ u,
                                                     const double* __restrict__ xa, const T* __restrict__ xc,
                                                     double* __restrict__ xb, T* __restrict__ r,
-                                                    double* __restrict__ partial, int64_t nb) {
+                                                    double* __restrict__ partial, int64_t nb,
+                                                    const T* __restrict__ einc = nullptr, const T* __restrict__ wp = nullptr) {
   int c0, c1, z0, z1;
   zt_cell(t, c0, c1, z0, z1);
   double acc = 0, acc2 = 0;
@@ -1894,7 +1895,11 @@ __global__ void __launch_bounds__(256) k_residual64(Geo g, ZT t, LRows<T> LR, co
       } else {
         zv = z[I];
       }
-      auto xv = [&](int64_t J) -> double { return FLOW ? (double)(pin[J] * scale) : xa[J] + (double)xc[J]; };
+      // einc: the correction's last increment! (c + ω·ϵ in T, src/Poisson.jl:103) taken on the fly
+      auto xv = [&](int64_t J) -> double {
+        if (FLOW) return (double)(pin[J] * scale);
+        return einc ? xa[J] + (double)(xc[J] + wp[0] * einc[J]) : xa[J] + (double)xc[J];
+      };
       const LC<T> L = lcell(g, LR, c0, c1, c2);
       double Dg = 0;
       for (int i = 0; i < g.D; ++i) Dg -= ((double)L.lo(I, i) + (double)L.hi(I, i, g.s[i]));

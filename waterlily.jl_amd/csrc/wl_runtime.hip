@@ -680,14 +680,20 @@ struct MG {
       vcycle(w);
       exact_coarsest = false;
       gs_sweeps(0);
-      increment(0, w, false);
-      perbc(p.g, p.x);
+      // the finest increment! only feeds x64 here (r is formed again in FP64): without periodic
+      // directions its c += ω·ϵ is taken inside k_residual64, which skips one pass
+      const bool fuse = perdir == 0;
+      if (!fuse) {
+        increment(0, w, false);
+        perbc(p.g, p.x);
+      }
       {
         ProfScope ps(P, st, "residual64", 0,
-                     (double)ncells_inside(p.g) * (sizeof(T) * (p.g.D + 2) + 2 * sizeof(double)));
+                     (double)ncells_inside(p.g) * (sizeof(T) * (p.g.D + 2 + (fuse ? 1 : 0)) + 2 * sizeof(double)));
         hipLaunchKernelGGL((k_residual64<T, false>), zt_grid(bi), kBlock, 0, st, p.g, bi, lr(0), (const T*)nullptr,
                            T(0), p.z, (const T*)nullptr, (const double*)x64[x64cur], (const T*)p.x,
-                           x64[1 - x64cur], p.r, B->partial, nb);
+                           x64[1 - x64cur], p.r, B->partial, nb, fuse ? (const T*)p.e : (const T*)nullptr,
+                           fuse ? w : (const T*)nullptr);
         KCK();
       }
       x64cur = 1 - x64cur;
