@@ -156,9 +156,10 @@ int wl_flow_cfl(wl_flow_t h, double* dt);
  * for all of them.  No counterpart in the reference, which steps one Simulation at a time
  * (src/WaterLily.jl:127-140 is the loop each member follows).  Members are ordinary flow handles
  * (created, measured and pois-updated as usual, still owned by the caller); they share D, dtype and
- * scheme, use the plain solver, and have no periodic directions and no exitBC.  One wl_batch_step
- * is one mom_step! of every member (dt, t0, t1, dt_next: one entry per member; n_iter: two per
- * member) with the same results as wl_mom_step on each. */
+ * scheme, use the plain solver, and have no periodic directions, no exitBC and no warm-start
+ * network attached (checked when the batch is created).  One wl_batch_step is one mom_step! of
+ * every member (dt, t0, t1, dt_next: one entry per member; n_iter: two per member) with the same
+ * results as wl_mom_step on each. */
 typedef struct wl_batch_s* wl_batch_t;
 int wl_batch_create(wl_batch_t* out, const wl_flow_t* flows, int nflows);
 int wl_batch_destroy(wl_batch_t b);

@@ -1333,6 +1333,7 @@ struct BatchT : BatchBase {
       if (f->D != D) throw WlErr(WL_EINVAL, "batch members must have the same dimension");
       if (f->desc.scheme != scheme) throw WlErr(WL_EINVAL, "batch members must use the same convection scheme");
       if (f->desc.perdir_mask || f->desc.exit_bc) throw WlErr(WL_EINVAL, "batch members: no periodic directions, no exitBC");
+      if (f->net) throw WlErr(WL_EINVAL, "batch members: no warm-start network attached");
       if (f->mode != WL_SOLVER_PLAIN) throw WlErr(WL_EINVAL, "batch members use the plain solver");
       if (f->has_fields()) throw WlErr(WL_EINVAL, "batch members use uniform boundary values and forcing");
       f->sync();
