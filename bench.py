@@ -199,6 +199,15 @@ def summarize(stats_csv, fetch_csv, write_csv, out_md, out_json, cmd, calib_byte
         lines.append(f"| {nm} | {r['Calls']} | {float(r['AverageNs']) / 1e3:.2f} | "
                      f"{float(r['TotalDurationNs']) / 1e6:.3f} | {r['Percentage']} |")
 
+    for tag, title in (("wl::nn::", "warm-start network kernels"), ("wl::kb_", "batched kernels")):
+        sel = [r for r in rows if tag in r["Name"]]
+        if sel:
+            lines += ["", f"## All {title}", "", "| kernel | calls | avg us | total ms |", "|---|---|---|---|"]
+            for r in sel:
+                nm = r["Name"].split("(")[0].replace("void ", "")
+                lines.append(f"| {nm} | {r['Calls']} | {float(r['AverageNs']) / 1e3:.2f} | "
+                             f"{float(r['TotalDurationNs']) / 1e6:.3f} |")
+
     trace_csv = stats_csv.replace("_kernel_stats.csv", "_kernel_trace.csv")
     if os.path.exists(trace_csv):
         grids = collections.defaultdict(list)
@@ -239,6 +248,10 @@ def summarize(stats_csv, fetch_csv, write_csv, out_md, out_json, cmd, calib_byte
                 per[(k, r["Kernel_Name"])].append(v)
         return per, copies
 
+    if fetch_csv == "-" or write_csv == "-":  # kernel-trace summary only
+        with open(out_md, "w") as fh:
+            fh.write("\n".join(lines) + "\n")
+        return
     fetch, fc = counters(fetch_csv, "FETCH_SIZE")
     write, wc = counters(write_csv, "WRITE_SIZE")
     f_scale = calib_bytes / (float(np.median(fc)) * 1024) if fc else None
@@ -354,7 +367,8 @@ def main():
     ap.add_argument("--copy-calib", action="store_true")
     ap.add_argument("--quiet-kernels", action="store_true")
     ap.add_argument("--summarize", nargs=6, metavar=("STATS", "FETCH", "WRITE", "OUT_MD", "OUT_JSON", "CMD"),
-                    help="turn rocprofv3 CSVs into the profile summary and the traffic table")
+                    help="turn rocprofv3 CSVs into the profile summary and the traffic table ('-' for FETCH and "
+                         "WRITE: kernel times only; OUT_JSON is then not written)")
     a = ap.parse_args()
     if a.summarize:
         summarize(*a.summarize)
