@@ -28,14 +28,16 @@ def sphere_sim(wl, n, R, center, Re, L):
     return wl.Simulation(n, (1, 0, 0), L, ν=L / Re, body=wl.Sphere(center, R), T=np.float32)
 
 
-def run_gpu(wl, n, R, center, Re, L, steps, warmup, profile_steps=None, mode=None):
-    """Warm up, time `steps` sim_step!s, then run `profile_steps` more with per-kernel HIP events
-    recorded on the library's stream (the events add launch gaps, so they stay out of the timed steps)."""
+def run_gpu(wl, n, R, center, Re, L, steps, warmup, profile_steps=None, mode=None, tune=True):
+    """Warm up, choose the tile targets with the autotuner (set-up, before the timed region), time
+    `steps` sim_step!s, then run `profile_steps` more with per-kernel HIP events recorded on the
+    library's stream (the events add launch gaps, so they stay out of the timed steps)."""
     sim = sphere_sim(wl, n, R, center, Re, L)
     if mode is not None:
         sim.pois.set_solver(mode=mode)
     for _ in range(warmup):
         wl.sim_step(sim, remeasure=False)
+    tiles = sim.flow.autotune(5) if tune else None
     n0 = len(sim.pois.n)
     t0 = time.perf_counter()
     for _ in range(steps):
@@ -55,7 +57,7 @@ def run_gpu(wl, n, R, center, Re, L, steps, warmup, profile_steps=None, mode=Non
     cells = int(np.prod(n))
     sec = t1 - t0
     return dict(sec=sec, ms_per_step=1e3 * sec / steps, mcells=cells * steps / sec / 1e6, steps_per_s=steps / sec,
-                vcycles_per_step=sum(nv) / steps, stats=stats, sim=sim,
+                vcycles_per_step=sum(nv) / steps, stats=stats, sim=sim, tiles=tiles,
                 profiled_ms_per_step=None if psec is None else 1e3 * psec)
 
 
@@ -81,6 +83,8 @@ def run_c5(wl, steps, warmup):
     """Aggregate throughput of the 64 uneven flows stepped as one batch, and stepped one at a time."""
     sims = c5_members(wl)
     cells = sum(int(np.prod(s.flow.N)) for s in sims)
+    for s in sims:  # tile targets per member (set-up), used by the batch's block tables
+        s.flow.autotune(3)
     b = wl.Batch(sims)
     for _ in range(warmup):
         b.step()
@@ -95,6 +99,7 @@ def run_c5(wl, steps, warmup):
     del b
     solo = c5_members(wl)
     for s in solo:
+        s.flow.autotune(3)
         for _ in range(warmup):
             wl.sim_step(s, remeasure=False)
     s0 = time.perf_counter()
@@ -155,6 +160,34 @@ def run_cpu(n, R, center, Re, L, budget_s=15.0):
     return dict(value=round(int(np.prod(n)) * k / el / 1e6, 3), unit="Mcell-steps/s", cores=threads, kind="port",
                 sample=f"{k} sim_step! of the same 256x128x128 sphere flow after 1 warm-up step "
                        f"({el:.1f} s), oracle built -O3 -fopenmp on {threads} threads")
+
+
+def run_c1(wl, steps=200):
+    """The 2D circle the reference runs on its CPU path (192x128, radius 16 at (63,63), Re 250 on
+    the diameter, Float32, from the default initial condition u = uBC): `steps` sim_step!s on the
+    GPU and on the oracle's CPU path, which must give the same trajectory (V-cycle counts and the
+    final pressure force compared)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import wl_oracle as O
+    n, cells = (192, 128), 192 * 128
+    sim = wl.Simulation(n, (1, 0), 32, ν=32 / 250, body=wl.Sphere((63, 63), 16), T=np.float32)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        wl.sim_step(sim, remeasure=False)
+    g = time.perf_counter() - t0
+    threads = os.cpu_count() or 1
+    O.lib().wlo_threads(threads)
+    osim = O.OSimulation(n, (1, 0), 32, nu=32 / 250, bodies=[{"kind": 1, "center": (63, 63), "radius": 16}])
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        osim.step(remeasure=False)
+    c = time.perf_counter() - t0
+    F, Fo = wl.pressure_force(sim), osim.pressure_force()
+    return dict(workload="2D circle 192x128, r=16 at (63,63), Re=250, Float32, 200 sim_step! from u = uBC",
+                steps=steps, gpu_mcells=round(cells * steps / g / 1e6, 2), gpu_ms_per_step=round(1e3 * g / steps, 3),
+                cpu_mcells=round(cells * steps / c / 1e6, 2), cpu_ms_per_step=round(1e3 * c / steps, 3),
+                cpu_threads=threads, same_vcycle_counts=sim.pois.n == osim.n,
+                pressure_force_gpu=[round(float(v), 6) for v in F], pressure_force_cpu=[round(float(v), 6) for v in Fo])
 
 
 def copy_calibration(nbytes=1 << 30, reps=5):
@@ -399,6 +432,7 @@ def main():
                            sphere_radius=R, Re=Re,
                            method="plain", remeasure=False),
                sim_steps_per_s=round(r["steps_per_s"], 3), vcycles_per_step=round(r["vcycles_per_step"], 3),
+               tiles=dict(autotuned=True, gs_target=r["tiles"][0], restrict_prolong_target=r["tiles"][1]),
                roofline=roof, profiled_ms_per_step=round(r["profiled_ms_per_step"], 4),
                timing="value from the timed steps with no per-kernel events; per-kernel HIP-event times "
                       "from a second pass of the same number of steps")
@@ -409,6 +443,7 @@ def main():
         r3 = run_gpu(wl, n3, R3, c3, Re3, L3, a.c3_steps, 3)
         roof3, rows3 = roofline(r3["stats"], os.path.join(ROOT, "profiles", "hbm_traffic_c3.json"))
         out["c3"] = dict(workload="3D sphere 512x256x256, r=32, Re=1e4, plain FP32 MG", steps=a.c3_steps,
+                         tiles=list(r3["tiles"]),
                          mcells=round(r3["mcells"], 2), ms_per_step=round(r3["ms_per_step"], 3),
                          vcycles_per_step=round(r3["vcycles_per_step"], 3), roofline=roof3,
                          kernels_L0=[k for k in rows3 if k["kernel"].endswith("@L0")][:10])
@@ -424,6 +459,7 @@ def main():
     if not a.no_c5:
         out["c5"] = run_c5(wl, a.c5_steps, 3)
     if not a.no_cpu:
+        out["c1"] = run_c1(wl)
         out["cpu_baseline"] = run_cpu(n, R, center, Re, L)
     else:
         out["cpu_baseline"] = None

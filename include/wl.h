@@ -205,6 +205,14 @@ int wl_net_get_pair(wl_net_t n, int index, float* in, float* target); /* 0 = old
 int wl_flow_set_warmstart(wl_flow_t h, wl_net_t net, int flags);
 int wl_flow_warm_stats(wl_flow_t h, double* r2, int cap, int* count); /* (before, after) pairs */
 
+/* Tile autotuner, no counterpart in the reference: times `reps` level-0 launches of the GS half
+ * sweeps and of the restriction/prolongation kernels for each candidate blocks-in-flight target
+ * (1024..16384) and keeps the fastest for this flow (targets[0]: GS, targets[1]: restriction /
+ * prolongation; may be NULL).  Only kernels without reductions are tuned, so results do not
+ * change; the fields are saved and restored around the timing.  A batch created afterwards uses
+ * each member's targets. */
+int wl_flow_autotune(wl_flow_t h, int reps, int32_t* targets);
+
 /* Σr² residual history of the last mom_step! (both projections); returns the count written. */
 int wl_flow_last_residuals(wl_flow_t h, double* r2, int cap, int* count);
 

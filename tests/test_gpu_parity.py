@@ -121,6 +121,26 @@ def test_c1_circle_1000_steps_forces(wl):
     assert sim.pois.n == osim.n
 
 
+def test_autotune_keeps_results(wl):
+    """The tile autotuner times GS and restriction/prolongation launches on the live flow: the
+    fields are the same after it, and the steps that follow (with its tile shapes) are bit-identical
+    to the Float32 oracle."""
+    say("case: autotuned tiles, 3D sphere 96x64x64, 3 steps")
+    sim, osim = oracle_pair(wl, (96, 64, 64), (1, 0, 0), 8, (24, 32, 32), 16 / 1000)
+    wl.sim_step(sim, remeasure=False)
+    osim.step(remeasure=False)
+    before = {k: sim.flow.get(k) for k in ("u", "p", "sigma")}
+    targets = sim.flow.autotune(reps=3)
+    say(f"  targets (GS, restriction/prolongation): {targets}")
+    for k, v in before.items():
+        assert np.array_equal(sim.flow.get(k), v), k
+    for _ in range(3):
+        wl.sim_step(sim, remeasure=False)
+        osim.step(remeasure=False)
+    assert np.array_equal(sim.flow.u, osim.flow.get("u")) and np.array_equal(sim.flow.p, osim.flow.get("p"))
+    assert sim.pois.n == osim.n
+
+
 def test_c2_sphere_ten_steps(wl):
     """3D sphere 256x128x128, Re 3700: ten steps with u, p and pressure_force compared at every
     step against the Float32 oracle (bit for bit here; the bars are rel-L2 1e-5 and force 1e-3);

@@ -238,6 +238,13 @@ class Flow:
     mu0 = property(lambda s: s.get("mu0"), lambda s, v: s.set("mu0", v))
     mu1 = property(lambda s: s.get("mu1"), lambda s, v: s.set("mu1", v))
 
+    def autotune(self, reps=5):
+        """Choose this flow's tile targets (GS half sweeps; restriction/prolongation) by timing;
+        results do not change.  Returns the two targets."""
+        t = (C.c_int32 * 2)()
+        A.check(A.load().wl_flow_autotune(self.h, int(reps), t))
+        return int(t[0]), int(t[1])
+
     def residuals(self):
         """Σr² after residual! and after every V-cycle of the last mom_step (both projections)."""
         buf = (C.c_double * 512)()

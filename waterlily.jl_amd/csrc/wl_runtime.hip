@@ -458,7 +458,7 @@ struct MG {
     }
     Box bh = box_half(p.g);
     if (box_empty(bh)) return;
-    ZT b = make_zt(bh);
+    ZT b = make_zt(bh, tgt_gs());
     // the four half sweeps are timed as one back-to-back group (no event between them)
     ProfScope ps(P, st, "gs_half", l, (double)sizeof(T) * (p.g.D + 1.5) * ncells_inside(p.g), 4);
     for (int it = 1; it <= 4; ++it) {
@@ -538,45 +538,122 @@ struct MG {
     bool in_lds = false;
     const int Lt = tail_start(&in_lds);
     const int down_end = Lt > 0 ? Lt : Lm;
-    for (int l = 0; l < down_end; ++l) {  // down-leg: Jacobi! + restrict! + fill!
-      Level<T>& f = lv[l];
-      Level<T>& c = lv[l + 1];
-      perbc(f.g, f.e);
-      ProfScope ps(P, st, "inc_restrict", l,
-                   (double)sizeof(T) * ((f.g.D + 5) * ncells_inside(f.g) + 3 * ncells_inside(c.g)));
-      const int full = (1 << f.g.D) - 1;
-      if (f.cmask == full) {
-        // fine x,y tile ranges, coarse z planes (the fine inside extents are even here)
-        Box b = box_inside(f.g);
-        if (f.g.D == 3) { b.lo[2] = 1; b.hi[2] = c.g.N[2] - 2; }
-        ZT bc = make_zt(b, 2 * zt_target_default());
-        if (f.g.D == 3)
-          hipLaunchKernelGGL((k_inc_restrict_full<T, 3>), zt_grid(bc), kBlock, 0, st, f.g, c.g, bc, lr(l), f.r, f.x,
-                             f.e, lr(l + 1), c.r, c.x, c.e);
-        else
-          hipLaunchKernelGGL((k_inc_restrict_full<T, 2>), zt_grid(bc), kBlock, 0, st, f.g, c.g, bc, lr(l), f.r, f.x,
-                             f.e, lr(l + 1), c.r, c.x, c.e);
-      } else {
-        ZT bc = make_zt(box_inside(c.g), 2 * zt_target_default());
-        hipLaunchKernelGGL(k_inc_restrict<T>, zt_grid(bc), kBlock, 0, st, f.g, c.g, bc, f.cmask, lr(l), f.r, f.x, f.e,
-                           lr(l + 1), c.r, c.x, c.e);
-      }
-      KCK();
-    }
+    for (int l = 0; l < down_end; ++l) restrict_level(l);  // down-leg: Jacobi! + restrict! + fill!
     if (Lt > 0) launch_tail(Lt, in_lds, w);  // levels Lt..Lm, through smooth!(Lt)
     for (int c = Lt > 0 ? Lt : Lm; c >= 1; --c) {  // up-leg: smooth!(coarse), prolongate! + increment!(fine)
       if (!(Lt > 0 && c == Lt)) {
         gs_sweeps(c);
         increment(c, w, false);
       }
-      Level<T>& f = lv[c - 1];
-      ZT bf = make_zt(box_inside(f.g), 2 * zt_target_default());
-      ProfScope ps(P, st, "prolong_inc", c - 1,
-                   (double)sizeof(T) * ((f.g.D + 5) * ncells_inside(f.g) + ncells_inside(lv[c].g)));
-      hipLaunchKernelGGL(k_prolong_inc<T>, zt_grid(bf), kBlock, 0, st, f.g, lv[c].g, bf, f.cmask, perdir, lr(c - 1), f.r,
-                         f.x, f.e, (const T*)lv[c].x, w);
-      KCK();
+      prolong_level(c, w);
     }
+  }
+  // Jacobi! (ω = 1) on level l, restrict! to l+1 and fill!(x, 0) there (src/MultiLevelPoisson.jl:92-94)
+  void restrict_level(int l) {
+    Level<T>& f = lv[l];
+    Level<T>& c = lv[l + 1];
+    perbc(f.g, f.e);
+    ProfScope ps(P, st, "inc_restrict", l,
+                 (double)sizeof(T) * ((f.g.D + 5) * ncells_inside(f.g) + 3 * ncells_inside(c.g)));
+    const int full = (1 << f.g.D) - 1;
+    if (f.cmask == full) {
+      // fine x,y tile ranges, coarse z planes (the fine inside extents are even here)
+      Box b = box_inside(f.g);
+      if (f.g.D == 3) { b.lo[2] = 1; b.hi[2] = c.g.N[2] - 2; }
+      ZT bc = make_zt(b, tgt_rp());
+      if (f.g.D == 3)
+        hipLaunchKernelGGL((k_inc_restrict_full<T, 3>), zt_grid(bc), kBlock, 0, st, f.g, c.g, bc, lr(l), f.r, f.x,
+                           f.e, lr(l + 1), c.r, c.x, c.e);
+      else
+        hipLaunchKernelGGL((k_inc_restrict_full<T, 2>), zt_grid(bc), kBlock, 0, st, f.g, c.g, bc, lr(l), f.r, f.x,
+                           f.e, lr(l + 1), c.r, c.x, c.e);
+    } else {
+      ZT bc = make_zt(box_inside(c.g), tgt_rp());
+      hipLaunchKernelGGL(k_inc_restrict<T>, zt_grid(bc), kBlock, 0, st, f.g, c.g, bc, f.cmask, lr(l), f.r, f.x, f.e,
+                         lr(l + 1), c.r, c.x, c.e);
+    }
+    KCK();
+  }
+  // prolongate! from level c into c-1, with increment!(ω) and ϵ := r·iD there (src/MultiLevelPoisson.jl:98-100)
+  void prolong_level(int c, const T* w) {
+    Level<T>& f = lv[c - 1];
+    ZT bf = make_zt(box_inside(f.g), tgt_rp());
+    ProfScope ps(P, st, "prolong_inc", c - 1,
+                 (double)sizeof(T) * ((f.g.D + 5) * ncells_inside(f.g) + ncells_inside(lv[c].g)));
+    hipLaunchKernelGGL(k_prolong_inc<T>, zt_grid(bf), kBlock, 0, st, f.g, lv[c].g, bf, f.cmask, perdir, lr(c - 1), f.r,
+                       f.x, f.e, (const T*)lv[c].x, w);
+    KCK();
+  }
+
+  // Tile autotuner: the blocks-in-flight targets of the GS half sweeps and of the restriction /
+  // prolongation kernels (kernels without reductions, whose results do not depend on the tiling),
+  // chosen by timing `reps` level-0 launches of each candidate with HIP events.  The buffers the
+  // timed launches write are saved first and restored after, so the solver state is unchanged.
+  int zt_gs = 0, zt_rp = 0;  // 0: the defaults
+  int tgt_gs() const { return zt_gs ? zt_gs : zt_target_default(); }
+  int tgt_rp() const { return zt_rp ? zt_rp : 2 * zt_target_default(); }
+  void autotune(int reps, const T* w) {
+    if (lv.size() < 2 || reps < 1) return;
+    free_graphs();  // captured iterations hold the old tile shapes
+    Level<T>& f = lv[0];
+    Level<T>& c = lv[1];
+    T* bufs[6] = {f.e, f.r, f.x, c.e, c.r, c.x};
+    const int64_t ns[6] = {f.g.n, f.g.n, f.g.n, c.g.n, c.g.n, c.g.n};
+    int64_t tot = 0;
+    for (int64_t v : ns) tot += v;
+    T* save = dalloc<T>(tot);
+    auto copy_all = [&](bool out) {
+      int64_t o = 0;
+      for (int k = 0; k < 6; ++k) {
+        HIPCK(hipMemcpyAsync(out ? save + o : bufs[k], out ? bufs[k] : save + o, ns[k] * sizeof(T),
+                             hipMemcpyDeviceToDevice, st));
+        o += ns[k];
+      }
+    };
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    try {
+      copy_all(true);
+      HIPCK(hipEventCreate(&e0));
+      HIPCK(hipEventCreate(&e1));
+      auto timed = [&](auto&& launch) {
+        launch();  // warm-up
+        HIPCK(hipEventRecord(e0, st));
+        for (int k = 0; k < reps; ++k) launch();
+        HIPCK(hipEventRecord(e1, st));
+        HIPCK(hipEventSynchronize(e1));
+        float ms = 0;
+        HIPCK(hipEventElapsedTime(&ms, e0, e1));
+        copy_all(false);
+        return ms;
+      };
+      static const int cands[] = {1024, 2048, 4096, 8192, 16384};
+      float best = 1e30f;
+      int pick = 0;
+      for (int cnd : cands) {
+        zt_gs = cnd;
+        const float ms = timed([&] { gs_sweeps(0); });
+        if (ms < best) { best = ms; pick = cnd; }
+      }
+      zt_gs = pick;
+      best = 1e30f;
+      pick = 0;
+      for (int cnd : cands) {
+        zt_rp = cnd;
+        const float ms = timed([&] { restrict_level(0); prolong_level(1, w); });
+        if (ms < best) { best = ms; pick = cnd; }
+      }
+      zt_rp = pick;
+    } catch (...) {
+      if (e0) (void)hipEventDestroy(e0);
+      if (e1) (void)hipEventDestroy(e1);
+      (void)hipStreamSynchronize(st);
+      dfree(save);
+      throw;
+    }
+    HIPCK(hipStreamSynchronize(st));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    dfree(save);
   }
 
   // One solver! iteration on the device: Vcycle!, smooth! of the finest level, L₂ and the ω update
@@ -797,6 +874,7 @@ struct FlowBase {
   virtual void set_field(int which, const void* src, size_t nbytes) = 0;
   virtual void apply_ic() = 0;
   virtual double cfl() = 0;
+  virtual void autotune(int reps, int* targets) = 0;
   // learned warm start of the pressure solves (wl_flow_set_warmstart)
   nn::WarmNet* net = nullptr;
   void* net_owner = nullptr;  // the wl_net_s handle, which lists the flows attached to it
@@ -1168,6 +1246,13 @@ struct FlowT : FlowBase {
     return nit;
   }
 
+  void autotune(int reps, int* targets) override {
+    sync();
+    if (mu1_dirty) mu1_rows();
+    mg.ensure_lrows();
+    mg.autotune(reps, (const T*)B.tsc + 3);
+    if (targets) { targets[0] = mg.tgt_gs(); targets[1] = mg.tgt_rp(); }
+  }
   // CFL(a) (src/Flow.jl:234-237) of the current u: min(10, 1/(max flux out of a cell + 5ν)),
   // formed in T; σ is its scratch (its stale ghosts included, as in the reference)
   double cfl() override {
@@ -1398,12 +1483,12 @@ struct BatchT : BatchBase {
         if (f.cmask == full) {
           Box b = box_inside(f.g);
           if (f.g.D == 3) { b.lo[2] = 1; b.hi[2] = c.g.N[2] - 2; }
-          const ZT t = make_zt(b, 2 * zt_target_default());
+          const ZT t = make_zt(b, mg.tgt_rp());
           af.push_back(lev_args(F[s], l, t));
           ag.push_back(ALev<T>{});
           add_blocks(bf, s, zt_blocks(t));
         } else {
-          const ZT t = make_zt(box_inside(c.g), 2 * zt_target_default());
+          const ZT t = make_zt(box_inside(c.g), mg.tgt_rp());
           ag.push_back(lev_args(F[s], l, t));
           af.push_back(ALev<T>{});
           add_blocks(bg, s, zt_blocks(t));
@@ -1420,7 +1505,7 @@ struct BatchT : BatchBase {
         const bool prol = l < Lt[s];              // prolongation into level l from l+1
         const Geo& g = F[s]->mg.lv[l < (int)F[s]->mg.lv.size() ? l : 0].g;
         if (smooth) {
-          const ZT th = make_zt(box_half(g));
+          const ZT th = make_zt(box_half(g), F[s]->mg.tgt_gs());
           ags.push_back(lev_args(F[s], l, th));
           add_blocks(bgs, s, zt_blocks(th));
           const ZT ti = make_zt(box_inside(g));
@@ -1430,7 +1515,7 @@ struct BatchT : BatchBase {
           ags.push_back(ALev<T>{}); agi.push_back(ALev<T>{});
         }
         if (prol) {
-          const ZT tp = make_zt(box_inside(g), 2 * zt_target_default());
+          const ZT tp = make_zt(box_inside(g), F[s]->mg.tgt_rp());
           agp.push_back(lev_args(F[s], l, tp));
           add_blocks(bgp, s, zt_blocks(tp));
         } else {
@@ -2114,6 +2199,15 @@ int wl_flow_set_field_fn(wl_flow_t h, int which, const void* src, size_t nbytes)
   API_BEGIN
   if (!h) throw WlErr(WL_EINVAL, "null handle");
   h->impl->set_field(which, src, nbytes);
+  API_END
+}
+int wl_flow_autotune(wl_flow_t h, int reps, int32_t* targets) {
+  API_BEGIN
+  if (!h) throw WlErr(WL_EINVAL, "null handle");
+  if (reps < 1 || reps > 1000) throw WlErr(WL_EINVAL, "reps must be in 1..1000");
+  int t[2] = {0, 0};
+  h->impl->autotune(reps, t);
+  if (targets) { targets[0] = t[0]; targets[1] = t[1]; }
   API_END
 }
 int wl_flow_cfl(wl_flow_t h, double* dt) {
