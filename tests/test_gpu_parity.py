@@ -5,6 +5,7 @@ identical V-cycle counts; element-wise kernels are additionally checked for bit 
 both sides evaluate the same IEEE operations in the same order.
 """
 import math
+import os
 import sys
 
 import numpy as np
@@ -162,6 +163,39 @@ def test_c2_sphere_ten_steps(wl):
         say(f"  step {k + 1}: vs F32 u {eu:.1e} p {ep:.1e} F {ef:.1e}; vs F64 u {rel(u, o64.flow.get('u')):.1e} "
             f"p {rel(p, o64.flow.get('p')):.1e} F {ef64:.1e}; n {sim.pois.n[-2:]}")
         assert eu <= 1e-5 and ep <= 1e-5 and ef <= 1e-3
+    assert sim.pois.n == osim.n
+
+
+@pytest.mark.skipif(os.environ.get("WL_LONG_PARITY") != "1",
+                    reason="about 7 minutes of oracle time: set WL_LONG_PARITY=1 to run it")
+def test_c2_sphere_1000_steps_forces(wl):
+    """The 3D sphere (256x128x128, Re 3700, Float32) for 1000 steps: pressure_force at every step
+    within 1e-3 (relative to the largest force of the run) of the Float32 oracle and every V-cycle
+    count equal; u and p at the end within 1e-5 (relative L2); the Float64 oracle's deviations are
+    reported beside them."""
+    say("case: 3D sphere 256x128x128, 1000 steps, pressure_force every step")
+    n, nu = (256, 128, 128), 32 / 3700
+    sim, osim = oracle_pair(wl, n, (1, 0, 0), 16, (64, 64, 64), nu)
+    o64 = O.OSimulation(n, (1, 0, 0), 32, nu=nu, dtype=O.WL_F64,
+                        bodies=[{"kind": 1, "center": (64, 64, 64), "radius": 16}])
+    Fg, F32, F64 = [], [], []
+    for k in range(1000):
+        wl.sim_step(sim, remeasure=False)
+        osim.step(remeasure=False)
+        o64.step(remeasure=False)
+        Fg.append(wl.pressure_force(sim))
+        F32.append(osim.pressure_force())
+        F64.append(o64.pressure_force())
+        if k % 100 == 99:
+            say(f"  step {k + 1}: F gpu {Fg[-1]} f32 {F32[-1]} f64 {F64[-1]}")
+    Fg, F32, F64 = np.array(Fg), np.array(F32), np.array(F64)
+    scale = np.max(np.abs(F32))
+    e32 = float(np.max(np.abs(Fg - F32)) / scale)
+    e64 = float(np.max(np.abs(Fg - F64)) / scale)
+    eu, ep = rel(sim.flow.u, osim.flow.get("u")), rel(sim.flow.p, osim.flow.get("p"))
+    say(f"  max force error vs F32 oracle {e32:.3e}, vs F64 oracle {e64:.3e}; final u {eu:.3e} p {ep:.3e} "
+        f"vs F32; u {rel(sim.flow.u, o64.flow.get('u')):.3e} p {rel(sim.flow.p, o64.flow.get('p')):.3e} vs F64")
+    assert e32 <= 1e-3 and eu <= 1e-5 and ep <= 1e-5
     assert sim.pois.n == osim.n
 
 
