@@ -942,14 +942,22 @@ __device__ __forceinline__ void k_inc_restrict_full_body(const Geo& gf, const Ge
                                                            const T* __restrict__ ef, const LRows<T>& Lc,
                                                            T* __restrict__ rc, T* __restrict__ xc,
                                                            T* __restrict__ ec, int lid) {
-  __shared__ T sR[2][BY][64];
+  // The 64x4 cells of a tile are dealt so that each wave holds 32 x by 2 y: wave w takes tile
+  // columns (w&1)·32 + 0..31 and rows 2(w>>1) + 0..1.  A coarse cell's children in x and y are then
+  // lanes l, l+1, l+32, l+33 of one wave and are summed with lane shuffles (no LDS, no barrier).
   int cx, cy, z0, z1;
   zt_cell_id(t, lid, cx, cy, z0, z1);
-  const int tx = threadIdx.x, ty = threadIdx.y;
+  const int lane = (int)threadIdx.x + 64 * (int)threadIdx.y, w = lane >> 6, l = lane & 63;
+  const int ox = (w & 1) * 32 + (l & 31), oy = (w >> 1) * 2 + (l >> 5);
+  cx += ox - (int)threadIdx.x;
+  cy += oy - (int)threadIdx.y;
   const bool in = cx <= t.b.hi[0] && cy <= t.b.hi[1];
+  const bool lead = in && l < 32 && !(l & 1);  // even tile offsets in x and y
   auto xv = [&](int64_t J) { return ef[J]; };
   constexpr int NZ = D == 3 ? 2 : 1;
   for (int kc = z0; kc <= z1; ++kc) {
+    T rq[NZ];
+#pragma unroll
     for (int q = 0; q < NZ; ++q) {
       const int cz = D == 3 ? 2 * kc - 1 + q : 0;
       T rn = 0;
@@ -961,21 +969,26 @@ __device__ __forceinline__ void k_inc_restrict_full_body(const Geo& gf, const Ge
         rf[J] = rn;
         xf[J] = xf[J] + T(1) * ef[J];
       }
-      sR[q][ty][tx] = rn;
+      rq[q] = rn;
     }
-    __syncthreads();
-    if (in && !(tx & 1) && !(ty & 1)) {
-      T s = 0;
-      for (int q = 0; q < NZ; ++q)
-        for (int yy = 0; yy < 2; ++yy)
-          for (int xx = 0; xx < 2; ++xx) s += sR[q][ty + yy][tx + xx];
+    // children summed in z, y, x order (restrict, src/MultiLevelPoisson.jl:13-19), as the
+    // reference's x-fastest loop does
+    T s = 0;
+#pragma unroll
+    for (int q = 0; q < NZ; ++q) {
+      const T c10 = __shfl_down(rq[q], 1), c01 = __shfl_down(rq[q], 32), c11 = __shfl_down(rq[q], 33);
+      s += rq[q];
+      s += c10;
+      s += c01;
+      s += c11;
+    }
+    if (lead) {
       const int ccx = (cx + 1) >> 1, ccy = (cy + 1) >> 1, ccz = D == 3 ? kc : 0;
       const int64_t Ic = at(gc, ccx, ccy, ccz);
       rc[Ic] = s;
       xc[Ic] = 0;
       ec[Ic] = s * inv_diag(diag_at<T>(gc, lcell(gc, Lc, ccx, ccy, ccz), Ic));
     }
-    __syncthreads();
   }
 }
 template <class T, int D>
