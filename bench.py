@@ -28,16 +28,19 @@ def sphere_sim(wl, n, R, center, Re, L):
     return wl.Simulation(n, (1, 0, 0), L, ν=L / Re, body=wl.Sphere(center, R), T=np.float32)
 
 
-def run_gpu(wl, n, R, center, Re, L, steps, warmup, profile_steps=None, mode=None, tune=True):
+def run_gpu(wl, n, R, center, Re, L, steps, warmup, profile_steps=None, mode=None, tune=False, tiles=None):
     """Warm up, choose the tile targets with the autotuner (set-up, before the timed region), time
     `steps` sim_step!s, then run `profile_steps` more with per-kernel HIP events recorded on the
     library's stream (the events add launch gaps, so they stay out of the timed steps)."""
     sim = sphere_sim(wl, n, R, center, Re, L)
     if mode is not None:
         sim.pois.set_solver(mode=mode)
+    if tiles is not None:  # the targets of an earlier tuned run, set without timing (for profiling)
+        sim.flow.set_tiles(*tiles)
+    elif tune:
+        tiles = sim.flow.autotune(5)
     for _ in range(warmup):
         wl.sim_step(sim, remeasure=False)
-    tiles = sim.flow.autotune(5) if tune else None
     n0 = len(sim.pois.n)
     t0 = time.perf_counter()
     for _ in range(steps):
@@ -392,6 +395,10 @@ def main():
     ap.add_argument("--no-c5", action="store_true")
     ap.add_argument("--c5-steps", type=int, default=20)
     ap.add_argument("--c3-main", action="store_true", help="time the 512x256x256 case as the main line")
+    ap.add_argument("--tiles", help="GS,RP tile targets to set (main line)")
+    ap.add_argument("--autotune", action="store_true",
+                    help="autotune the main line's tiles (the 256x128x128 case gains nothing from it; "
+                         "the 512x256x256 section and the batch always tune)")
     ap.add_argument("--no-c4", action="store_true")
     ap.add_argument("--c4-steps", type=int, default=30)
     ap.add_argument("--train-warmstart", metavar="OUT", help="train the warm-start network and save its weights")
@@ -423,7 +430,8 @@ def main():
         n, R, center, Re, L = (512, 256, 256), 32, (128, 128, 128), 1e4, 64
         traffic_file, label = "hbm_traffic_c3.json", "3D flow past a sphere 512x256x256, Re=1e4"
         a.no_c3 = True
-    r = run_gpu(wl, n, R, center, Re, L, a.steps, a.warmup)
+    r = run_gpu(wl, n, R, center, Re, L, a.steps, a.warmup, tune=a.autotune or a.c3_main,
+                tiles=tuple(int(v) for v in a.tiles.split(",")) if a.tiles else None)
     roof, rows = roofline(r["stats"], os.path.join(ROOT, "profiles", traffic_file))
     out = dict(metric=METRIC, value=round(r["mcells"], 2), unit="Mcell-steps/s", n_gpus=1, steps=a.steps,
                warmup=a.warmup, ms_per_step=round(r["ms_per_step"], 4), higher_is_better=True, scaling="weak",
@@ -432,7 +440,9 @@ def main():
                            sphere_radius=R, Re=Re,
                            method="plain", remeasure=False),
                sim_steps_per_s=round(r["steps_per_s"], 3), vcycles_per_step=round(r["vcycles_per_step"], 3),
-               tiles=dict(autotuned=True, gs_target=r["tiles"][0], restrict_prolong_target=r["tiles"][1]),
+               tiles=dict(autotuned=r["tiles"] is not None and not a.tiles,
+                          gs_target=r["tiles"][0] if r["tiles"] else "default",
+                          restrict_prolong_target=r["tiles"][1] if r["tiles"] else "default"),
                roofline=roof, profiled_ms_per_step=round(r["profiled_ms_per_step"], 4),
                timing="value from the timed steps with no per-kernel events; per-kernel HIP-event times "
                       "from a second pass of the same number of steps")
@@ -440,7 +450,7 @@ def main():
         out["kernels"] = rows[:14]
     if not a.no_c3:
         n3, R3, c3, Re3, L3 = (512, 256, 256), 32, (128, 128, 128), 1e4, 64
-        r3 = run_gpu(wl, n3, R3, c3, Re3, L3, a.c3_steps, 3)
+        r3 = run_gpu(wl, n3, R3, c3, Re3, L3, a.c3_steps, 3, tune=True)
         roof3, rows3 = roofline(r3["stats"], os.path.join(ROOT, "profiles", "hbm_traffic_c3.json"))
         out["c3"] = dict(workload="3D sphere 512x256x256, r=32, Re=1e4, plain FP32 MG", steps=a.c3_steps,
                          tiles=list(r3["tiles"]),
@@ -448,7 +458,7 @@ def main():
                          vcycles_per_step=round(r3["vcycles_per_step"], 3), roofline=roof3,
                          kernels_L0=[k for k in rows3 if k["kernel"].endswith("@L0")][:10])
         del r3
-        r3m = run_gpu(wl, n3, R3, c3, Re3, L3, a.c3_steps, 3, profile_steps=0, mode=wl.WL_SOLVER_MIXED)
+        r3m = run_gpu(wl, n3, R3, c3, Re3, L3, a.c3_steps, 3, profile_steps=0, mode=wl.WL_SOLVER_MIXED, tune=True)
         out["c3_mixed"] = dict(workload="the same 512x256x256 sphere with FP32 V-cycles inside an FP64 "
                                         "defect-correction loop (coarsest-level smoothing in FP64)", steps=a.c3_steps,
                                mcells=round(r3m["mcells"], 2), ms_per_step=round(r3m["ms_per_step"], 3),

@@ -212,6 +212,8 @@ int wl_flow_warm_stats(wl_flow_t h, double* r2, int cap, int* count); /* (before
  * change; the fields are saved and restored around the timing.  A batch created afterwards uses
  * each member's targets. */
 int wl_flow_autotune(wl_flow_t h, int reps, int32_t* targets);
+/* Set the two targets directly (0: the default), e.g. to repeat a tuned run without timing. */
+int wl_flow_set_tiles(wl_flow_t h, int32_t gs, int32_t rp);
 
 /* Σr² residual history of the last mom_step! (both projections); returns the count written. */
 int wl_flow_last_residuals(wl_flow_t h, double* r2, int cap, int* count);

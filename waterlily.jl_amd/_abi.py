@@ -71,6 +71,7 @@ SIGNATURES = {
     "wl_flow_apply_ic": (_i, [_vp]),
     "wl_flow_cfl": (_i, [_vp, _P(_d)]),
     "wl_flow_autotune": (_i, [_vp, _i, _P(C.c_int32)]),
+    "wl_flow_set_tiles": (_i, [_vp, C.c_int32, C.c_int32]),
     "wl_batch_create": (_i, [_P(_vp), _P(_vp), _i]),
     "wl_batch_destroy": (_i, [_vp]),
     "wl_batch_step": (_i, [_vp, _P(_d), _P(_d), _P(_d), _P(_d), _P(C.c_int16)]),

@@ -245,6 +245,10 @@ class Flow:
         A.check(A.load().wl_flow_autotune(self.h, int(reps), t))
         return int(t[0]), int(t[1])
 
+    def set_tiles(self, gs, rp):
+        """Set the tile targets directly (0: the defaults), as autotune would."""
+        A.check(A.load().wl_flow_set_tiles(self.h, int(gs), int(rp)))
+
     def residuals(self):
         """Σr² after residual! and after every V-cycle of the last mom_step (both projections)."""
         buf = (C.c_double * 512)()

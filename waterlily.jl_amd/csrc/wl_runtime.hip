@@ -875,6 +875,7 @@ struct FlowBase {
   virtual void apply_ic() = 0;
   virtual double cfl() = 0;
   virtual void autotune(int reps, int* targets) = 0;
+  virtual void set_tiles(int gs, int rp) = 0;
   // learned warm start of the pressure solves (wl_flow_set_warmstart)
   nn::WarmNet* net = nullptr;
   void* net_owner = nullptr;  // the wl_net_s handle, which lists the flows attached to it
@@ -1246,6 +1247,12 @@ struct FlowT : FlowBase {
     return nit;
   }
 
+  void set_tiles(int gs, int rp) override {
+    sync();
+    mg.free_graphs();
+    mg.zt_gs = gs;
+    mg.zt_rp = rp;
+  }
   void autotune(int reps, int* targets) override {
     sync();
     if (mu1_dirty) mu1_rows();
@@ -2208,6 +2215,13 @@ int wl_flow_autotune(wl_flow_t h, int reps, int32_t* targets) {
   int t[2] = {0, 0};
   h->impl->autotune(reps, t);
   if (targets) { targets[0] = t[0]; targets[1] = t[1]; }
+  API_END
+}
+int wl_flow_set_tiles(wl_flow_t h, int32_t gs, int32_t rp) {
+  API_BEGIN
+  if (!h) throw WlErr(WL_EINVAL, "null handle");
+  if (gs < 0 || rp < 0 || gs > (1 << 20) || rp > (1 << 20)) throw WlErr(WL_EINVAL, "tile targets must be in 0..2^20");
+  h->impl->set_tiles(gs, rp);
   API_END
 }
 int wl_flow_cfl(wl_flow_t h, double* dt) {
