@@ -24,33 +24,36 @@ struct LRows {
 };
 template <class T>
 __host__ __device__ inline LRows<T> lrows_plain(const T* L) { return LRows<T>{L, nullptr, {T(0), T(0), T(0)}}; }
-// the coefficients seen from one cell (x, y, z)
+// The coefficients seen from one cell (x, y, z): l[i] = L[I, i] and h[i] = L[I + e_i, i].  lcell
+// reads all of them up front and without branches, so that their loads issue together with the
+// cell's other loads: a uniform row reads (and discards) the first cells of L, which stay in
+// cache, and takes the row constants; a 2D level reads direction 1 again in place of the absent
+// direction 2 (l[2], h[2] are then never used); on an upper ghost plane h of that direction reads
+// L[I] instead (callers use h only at cells with an upper neighbour).
 template <class T>
 struct LC {
-  const T* L;
-  int64_t n;
-  int x, N0m1;
-  bool u;
-  T c0, c1, c2;
-  __device__ __forceinline__ T cu(int i) const { return i == 0 ? c0 : (i == 1 ? c1 : c2); }
+  T l[3], h[3];
   // L[I, i]
-  __device__ __forceinline__ T lo(int64_t I, int i) const {
-    return u ? ((i == 0 && x == 1) ? T(0) : cu(i)) : L[I + i * n];
-  }
+  __device__ __forceinline__ T lo(int64_t, int i) const { return l[i]; }
   // L[I + e_i, i]
-  __device__ __forceinline__ T hi(int64_t I, int i, int64_t si) const {
-    return u ? ((i == 0 && x + 1 == N0m1) ? T(0) : cu(i)) : L[I + si + i * n];
-  }
+  __device__ __forceinline__ T hi(int64_t, int i, int64_t) const { return h[i]; }
 };
+// direction i, or direction 1 for i = 2 on a 2D level (an in-bounds stand-in whose values are unused)
+__device__ __forceinline__ int dir_or_1(const Geo& g, int i) { return (i == 2 && g.D != 3) ? 1 : i; }
 template <class T>
 __device__ __forceinline__ LC<T> lcell(const Geo& g, const LRows<T>& a, int x, int y, int z) {
+  const int64_t I = at(g, x, y, z);
+  const bool u = a.rows != nullptr && a.rows[y + (int64_t)g.N[1] * z] != 0;
+  const int64_t J = u ? 0 : I;
   LC<T> c;
-  c.L = a.L;
-  c.n = g.n;
-  c.x = x;
-  c.N0m1 = g.N[0] - 1;
-  c.u = a.rows != nullptr && a.rows[y + (int64_t)g.N[1] * z] != 0;
-  c.c0 = a.c[0]; c.c1 = a.c[1]; c.c2 = a.c[2];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    const int k = dir_or_1(g, i);
+    const int64_t Jh = J + g.s[k] < g.n ? J + g.s[k] : J;  // an upper ghost cell has no L[I + e_k]
+    const T lv = a.L[J + k * g.n], hv = a.L[Jh + k * g.n];
+    c.l[i] = u ? ((i == 0 && x == 1) ? T(0) : a.c[i]) : lv;
+    c.h[i] = u ? ((i == 0 && x + 1 == g.N[0] - 1) ? T(0) : a.c[i]) : hv;
+  }
   return c;
 }
 // plain pointer view (every coefficient loaded)
@@ -449,7 +452,9 @@ __device__ __forceinline__ void k_bdim2_body(const Geo& g, const ZT& t, T* __res
   const bool band = rows[c1 + (int64_t)g.N[1] * c2] != 0;
   const bool vnz = vrows[c1 + (int64_t)g.N[1] * c2] != 0;
   const LC<T> m0 = lcell(g, mu0, c0, c1, c2);
-  for (int i = 0; i < D; ++i) {
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    if (i >= D) continue;
     const int64_t k = I + i * n;
     T s = 0;
     if (band)
@@ -673,8 +678,20 @@ struct LevelP {
 template <class T, class LA>
 __device__ __forceinline__ T diag_at(const Geo& g, const LA& L, int64_t I) {
   T s = 0;
-  for (int i = 0; i < g.D; ++i) s -= (L.lo(I, i) + L.hi(I, i, g.s[i]));
+  s -= (L.lo(I, 0) + L.hi(I, 0, g.s[0]));
+  s -= (L.lo(I, 1) + L.hi(I, 1, g.s[1]));
+  if (g.D == 3) s -= (L.lo(I, 2) + L.hi(I, 2, g.s[2]));
   return s;
+}
+// the same from an LC, the third term taken by a select (no branch between the cell's loads and
+// its arithmetic)
+template <class T>
+__device__ __forceinline__ T diag_at(const Geo& g, const LC<T>& L, int64_t) {
+  T s = 0;
+  s -= (L.l[0] + L.h[0]);
+  s -= (L.l[1] + L.h[1]);
+  const T s3 = s - (L.l[2] + L.h[2]);
+  return g.D == 3 ? s3 : s;
 }
 template <class T>
 __device__ __forceinline__ T diag_at(const Geo& g, const T* __restrict__ L, int64_t I) {
@@ -683,15 +700,47 @@ __device__ __forceinline__ T diag_at(const Geo& g, const T* __restrict__ L, int6
 template <class T>
 __device__ __forceinline__ T inv_diag(T Dg) { return Dg == T(0) ? Dg : T(1) / Dg; }
 
+// The six neighbour values x(I ∓ e_i) of a cell, read before any arithmetic (a 2D level reads the
+// direction-1 pair again as the unused third pair).
+template <class T>
+struct Nb {
+  T m[3], p[3];
+};
+template <class T, class X>
+__device__ __forceinline__ Nb<T> gather_nb(const Geo& g, int64_t I, X xv) {
+  Nb<T> v;
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    const int64_t si = g.s[dir_or_1(g, i)];
+    v.m[i] = xv(I - si);
+    v.p[i] = xv(I + si);
+  }
+  return v;
+}
+// mult(I,L,D,x) = x(I)·D + Σ_i (x(I−e_i)·L[I,i] + x(I+e_i)·L[I+e_i,i]), from gathered values
+template <class T, class LA>
+__device__ __forceinline__ T mult_nb(const Geo& g, const LA& L, int64_t I, T Dg, T x0, const Nb<T>& v) {
+  T s = x0 * Dg;
+  s += (v.m[0] * L.lo(I, 0) + v.p[0] * L.hi(I, 0, g.s[0]));
+  s += (v.m[1] * L.lo(I, 1) + v.p[1] * L.hi(I, 1, g.s[1]));
+  if (g.D == 3) s += (v.m[2] * L.lo(I, 2) + v.p[2] * L.hi(I, 2, g.s[2]));
+  return s;
+}
+// the same from an LC, the third term taken by a select
+template <class T>
+__device__ __forceinline__ T mult_nb(const Geo& g, const LC<T>& L, int64_t, T Dg, T x0, const Nb<T>& v) {
+  T s = x0 * Dg;
+  s += (v.m[0] * L.l[0] + v.p[0] * L.h[0]);
+  s += (v.m[1] * L.l[1] + v.p[1] * L.h[1]);
+  const T s3 = s + (v.m[2] * L.l[2] + v.p[2] * L.h[2]);
+  return g.D == 3 ? s3 : s;
+}
 // mult(I,L,D,x) with x read through a functor (so fused kernels can synthesize x)
 template <class T, class LA, class X>
 __device__ __forceinline__ T mult_at(const Geo& g, const LA& L, int64_t I, T Dg, X xv) {
-  T s = xv(I) * Dg;
-  for (int i = 0; i < g.D; ++i) {
-    const int64_t si = g.s[i];
-    s += (xv(I - si) * L.lo(I, i) + xv(I + si) * L.hi(I, i, si));
-  }
-  return s;
+  const T x0 = xv(I);
+  const Nb<T> v = gather_nb<T>(g, I, xv);
+  return mult_nb<T>(g, L, I, Dg, x0, v);
 }
 template <class T, class X>
 __device__ __forceinline__ T mult_at(const Geo& g, const T* __restrict__ L, int64_t I, T Dg, X xv) {
@@ -703,11 +752,14 @@ __device__ __forceinline__ T mult_at(const Geo& g, const T* __restrict__ L, int6
 template <class T, class LA>
 __device__ __forceinline__ T increment_cell(const Geo& g, const LA& L, T* __restrict__ r,
                                             T* __restrict__ x, const T* __restrict__ e, T w, int64_t I) {
+  auto ev = [&](int64_t J) { return e[J]; };
+  const T e0 = e[I];
+  const Nb<T> v = gather_nb<T>(g, I, ev);
+  const T r0 = r[I], x0 = x[I];
   const T Dg = diag_at<T>(g, L, I);
-  auto xv = [&](int64_t J) { return e[J]; };
-  const T rn = r[I] - w * mult_at<T>(g, L, I, Dg, xv);
+  const T rn = r0 - w * mult_nb<T>(g, L, I, Dg, e0, v);
   r[I] = rn;
-  x[I] = x[I] + w * e[I];
+  x[I] = x0 + w * e0;
   return rn;
 }
 template <class T>
@@ -720,21 +772,22 @@ __device__ __forceinline__ T increment_cell(const Geo& g, const T* __restrict__ 
 template <class T>
 __device__ __forceinline__ void gs_cell(const Geo& g, const LRows<T>& LR, const T* __restrict__ r,
                                         T* __restrict__ e, int it, int c0, int c1, int c2) {
-  int q[3] = {c0, c1, c2};
-  const int Ld = g.D - 1;
-  int front = 0;
-  for (int d = 0; d < Ld; ++d) front += q[d] + 1;  // 1-based front indices
-  const int kv = q[Ld] + 1;                        // 1-based half index
-  q[Ld] = 2 * kv - 1 - (front + it) % 2 - 1;       // back to 0-based
-  const int64_t I = at(g, q[0], q[1], q[2]);
-  const LC<T> L = lcell(g, LR, q[0], q[1], q[2]);
+  const bool d3 = g.D == 3;
+  const int front = (c0 + 1) + (d3 ? c1 + 1 : 0);  // 1-based front indices
+  const int par = (front + it) % 2;
+  // the last index: 1-based half index kv = c+1 becomes 2kv − 1 − par, back to 0-based
+  const int x = c0, y = d3 ? c1 : 2 * c1 - par, z = d3 ? 2 * c2 - par : c2;
+  const int64_t I = at(g, x, y, z);
+  auto ev = [&](int64_t J) { return e[J]; };
+  const Nb<T> v = gather_nb<T>(g, I, ev);
+  const T r0 = r[I];
+  const LC<T> L = lcell(g, LR, x, y, z);
   const T Dg = diag_at<T>(g, L, I);
-  T s = r[I];
-  for (int i = 0; i < g.D; ++i) {
-    const int64_t si = g.s[i];
-    s -= (e[I - si] * L.lo(I, i) + e[I + si] * L.hi(I, i, si));
-  }
-  e[I] = s * inv_diag(Dg);
+  T s = r0;
+  s -= (v.m[0] * L.l[0] + v.p[0] * L.h[0]);
+  s -= (v.m[1] * L.l[1] + v.p[1] * L.h[1]);
+  const T s3 = s - (v.m[2] * L.l[2] + v.p[2] * L.h[2]);
+  e[I] = (d3 ? s3 : s) * inv_diag(Dg);
 }
 // Jacobi!'s increment (ω=1) of the 2^D children of coarse cell c, in x-fastest order, their new
 // residuals summed into the coarse r (restrict!), coarse x = 0 (fill!) and coarse ϵ = r·iD
@@ -772,24 +825,34 @@ template <class T>
 __device__ __forceinline__ void prolong_cell(const Geo& gf, const Geo& gc, int cm, int perdir, const LRows<T>& LR,
                                              T* __restrict__ rf, T* __restrict__ xf, T* __restrict__ ef,
                                              const T* __restrict__ xc, T w, const int* c) {
-  int q[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};  // q[d][o]: coarse index of c_d + o - 1
-  for (int d = 0; d < gf.D; ++d)
-    for (int o = 0; o < 3; ++o) q[d][o] = pro_map(c[d] + o - 1, gf.N[d], (perdir >> d) & 1, (cm >> d) & 1);
+  const bool d3 = gf.D == 3;
+  int q[3][3];  // q[d][o]: coarse index of c_d + o - 1 (0 along the absent direction of a 2D level)
+#pragma unroll
+  for (int d = 0; d < 3; ++d)
+#pragma unroll
+    for (int o = 0; o < 3; ++o)
+      q[d][o] = (d < 2 || d3) ? pro_map(c[d] + o - 1, gf.N[d], (perdir >> d) & 1, (cm >> d) & 1) : 0;
+  // every coarse value is read unconditionally ("no value" reads cell 0 and is replaced by 0)
   auto xcv = [&](int a, int b, int z) -> T {
-    return (a < 0 || b < 0 || z < 0) ? T(0) : xc[(int64_t)a + gc.s[1] * b + gc.s[2] * z];
+    const bool ok = a >= 0 && b >= 0 && z >= 0;
+    const T v = xc[ok ? (int64_t)a + gc.s[1] * b + gc.s[2] * z : 0];
+    return ok ? v : T(0);
   };
   const int64_t I = at(gf, c[0], c[1], c[2]);
+  const T e0 = xcv(q[0][1], q[1][1], q[2][1]);
+  const T xm0 = xcv(q[0][0], q[1][1], q[2][1]), xp0 = xcv(q[0][2], q[1][1], q[2][1]);
+  const T xm1 = xcv(q[0][1], q[1][0], q[2][1]), xp1 = xcv(q[0][1], q[1][2], q[2][1]);
+  const T xm2 = xcv(q[0][1], q[1][1], q[2][0]), xp2 = xcv(q[0][1], q[1][1], q[2][2]);
+  const T r0 = rf[I], x0 = xf[I];
   const LC<T> Lf = lcell(gf, LR, c[0], c[1], c[2]);
   const T Dg = diag_at<T>(gf, Lf, I);
-  const T e0 = xcv(q[0][1], q[1][1], q[2][1]);
   T s = e0 * Dg;
-  s += (xcv(q[0][0], q[1][1], q[2][1]) * Lf.lo(I, 0) + xcv(q[0][2], q[1][1], q[2][1]) * Lf.hi(I, 0, 1));
-  s += (xcv(q[0][1], q[1][0], q[2][1]) * Lf.lo(I, 1) + xcv(q[0][1], q[1][2], q[2][1]) * Lf.hi(I, 1, gf.s[1]));
-  if (gf.D == 3)
-    s += (xcv(q[0][1], q[1][1], q[2][0]) * Lf.lo(I, 2) + xcv(q[0][1], q[1][1], q[2][2]) * Lf.hi(I, 2, gf.s[2]));
-  const T rn = rf[I] - w * s;
+  s += (xm0 * Lf.l[0] + xp0 * Lf.h[0]);
+  s += (xm1 * Lf.l[1] + xp1 * Lf.h[1]);
+  const T s3 = s + (xm2 * Lf.l[2] + xp2 * Lf.h[2]);
+  const T rn = r0 - w * (d3 ? s3 : s);
   rf[I] = rn;
-  xf[I] = xf[I] + w * e0;
+  xf[I] = x0 + w * e0;
   ef[I] = rn * inv_diag(Dg);
 }
 
@@ -1504,7 +1567,9 @@ __device__ __forceinline__ void k_correct_body(const Geo& g, const ZT& t, T* __r
     const T xI = x[I];
     if (in) {
       const LC<T> L = lcell(g, LR, c[0], c[1], c[2]);
-      for (int i = 0; i < g.D; ++i) u[I + i * g.n] -= L.lo(I, i) * (xI - x[I - g.s[i]]);
+#pragma unroll
+      for (int i = 0; i < 3; ++i)
+        if (i < g.D) u[I + i * g.n] -= L.lo(I, i) * (xI - x[I - g.s[i]]);
     }
     p[I] = xI / dt;
   }
@@ -1915,10 +1980,14 @@ __global__ void __launch_bounds__(256) k_residual64(Geo g, ZT t, LRows<T> LR, co
       };
       const LC<T> L = lcell(g, LR, c0, c1, c2);
       double Dg = 0;
-      for (int i = 0; i < g.D; ++i) Dg -= ((double)L.lo(I, i) + (double)L.hi(I, i, g.s[i]));
+#pragma unroll
+      for (int i = 0; i < 3; ++i)
+        if (i < g.D) Dg -= ((double)L.lo(I, i) + (double)L.hi(I, i, g.s[i]));
       const double xI = xv(I);
       double ax = xI * Dg;
-      for (int i = 0; i < g.D; ++i) {
+#pragma unroll
+      for (int i = 0; i < 3; ++i) {
+        if (i >= g.D) continue;
         const int64_t si = g.s[i];
         ax += (xv(I - si) * (double)L.lo(I, i) + xv(I + si) * (double)L.hi(I, i, si));
       }
