@@ -767,6 +767,22 @@ __device__ __forceinline__ T increment_cell(const Geo& g, const T* __restrict__ 
                                             T* __restrict__ x, const T* __restrict__ e, T w, int64_t I) {
   return increment_cell<T>(g, LPtr<T>{L, g.n}, r, x, e, w, I);
 }
+// the same at cell (cx, cy, cz) of a level with row flags: the values, then the row flag and L
+template <class T>
+__device__ __forceinline__ T increment_cell(const Geo& g, const LRows<T>& LR, T* __restrict__ r, T* __restrict__ x,
+                                            const T* __restrict__ e, T w, int cx, int cy, int cz) {
+  const int64_t I = at(g, cx, cy, cz);
+  auto ev = [&](int64_t J) { return e[J]; };
+  const T e0 = e[I];
+  const Nb<T> v = gather_nb<T>(g, I, ev);
+  const T r0 = r[I], x0 = x[I];
+  const LC<T> L = lcell(g, LR, cx, cy, cz);
+  const T Dg = diag_at<T>(g, L, I);
+  const T rn = r0 - w * mult_nb<T>(g, L, I, Dg, e0, v);
+  r[I] = rn;
+  x[I] = x0 + w * e0;
+  return rn;
+}
 // one GS-RB update at half-range cell c (src/Poisson.jl:116-128): the last index is doubled and
 // shifted so colour `it` updates cells whose 1-based index sum is even for odd it
 template <class T>
@@ -869,22 +885,34 @@ __device__ __forceinline__ void k_residual_body(const Geo& g, const ZT& t, const
   if (c0 <= t.b.hi[0] && c1 <= t.b.hi[1]) {
     for (int c2 = z0; c2 <= z1; ++c2) {
       const int64_t I = at(g, c0, c1, c2);
+      // all reads first: the source, x and its neighbours, then L
       T zv;
       if (FLOW) {
+        T up[3], uc[3];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+          const int k = dir_or_1(g, i);
+          up[i] = u[I + g.s[k] + k * g.n];
+          uc[i] = u[I + k * g.n];
+        }
         T s = 0;
-        for (int i = 0; i < g.D; ++i) s += (u[I + g.s[i] + i * g.n] - u[I + i * g.n]);
-        zv = s;
-        z[I] = zv;
+        s += (up[0] - uc[0]);
+        s += (up[1] - uc[1]);
+        const T s3 = s + (up[2] - uc[2]);
+        zv = g.D == 3 ? s3 : s;
       } else {
         zv = z[I];
       }
+      auto xv = [&](int64_t J) { return xin[J] * scale; };
+      const T x0 = xv(I);
+      const Nb<T> v = gather_nb<T>(g, I, xv);
       const LC<T> L = lcell(g, LR, c0, c1, c2);
       const T Dg = diag_at<T>(g, L, I);
       const T iD = inv_diag(Dg);
-      auto xv = [&](int64_t J) { return xin[J] * scale; };
-      const T rv = iD == T(0) ? T(0) : zv - mult_at<T>(g, L, I, Dg, xv);
+      const T rv = iD == T(0) ? T(0) : zv - mult_nb<T>(g, L, I, Dg, x0, v);
+      if (FLOW) z[I] = zv;
       r[I] = rv;
-      if (write_x) xout[I] = xv(I);
+      if (write_x) xout[I] = x0;
       acc += (double)rv;
     }
   }
@@ -958,7 +986,7 @@ __device__ __forceinline__ void k_increment_body(const Geo& g, const ZT& t, cons
   double acc = 0;
   if (c0 <= t.b.hi[0] && c1 <= t.b.hi[1]) {
     for (int c2 = z0; c2 <= z1; ++c2) {
-      const T rn = increment_cell<T>(g, lcell(g, LR, c0, c1, c2), r, x, e, w, at(g, c0, c1, c2));
+      const T rn = increment_cell<T>(g, LR, r, x, e, w, c0, c1, c2);
       acc += (double)rn * (double)rn;
     }
   }
@@ -1566,10 +1594,18 @@ __device__ __forceinline__ void k_correct_body(const Geo& g, const ZT& t, T* __r
     for (int d = 0; d < g.D; ++d) in &= c[d] >= 1 && c[d] <= g.N[d] - 2;
     const T xI = x[I];
     if (in) {
-      const LC<T> L = lcell(g, LR, c[0], c[1], c[2]);
+      // the components' u and x(I − e_i) are read before any store (a 2D level rereads direction 1)
+      T uv[3], xm[3];
 #pragma unroll
-      for (int i = 0; i < 3; ++i)
-        if (i < g.D) u[I + i * g.n] -= L.lo(I, i) * (xI - x[I - g.s[i]]);
+      for (int i = 0; i < 3; ++i) {
+        const int k = dir_or_1(g, i);
+        uv[i] = u[I + k * g.n];
+        xm[i] = x[I - g.s[k]];
+      }
+      const LC<T> L = lcell(g, LR, c[0], c[1], c[2]);
+      u[I] = uv[0] - L.l[0] * (xI - xm[0]);
+      u[I + g.n] = uv[1] - L.l[1] * (xI - xm[1]);
+      if (g.D == 3) u[I + 2 * g.n] = uv[2] - L.l[2] * (xI - xm[2]);
     }
     p[I] = xI / dt;
   }
@@ -1978,19 +2014,18 @@ __global__ void __launch_bounds__(256) k_residual64(Geo g, ZT t, LRows<T> LR, co
         if (FLOW) return (double)(pin[J] * scale);
         return einc ? xa[J] + (double)(xc[J] + wp[0] * einc[J]) : xa[J] + (double)xc[J];
       };
+      // x and its neighbours, then L, before the arithmetic
+      const double xI = xv(I);
+      const Nb<double> v = gather_nb<double>(g, I, xv);
       const LC<T> L = lcell(g, LR, c0, c1, c2);
       double Dg = 0;
-#pragma unroll
-      for (int i = 0; i < 3; ++i)
-        if (i < g.D) Dg -= ((double)L.lo(I, i) + (double)L.hi(I, i, g.s[i]));
-      const double xI = xv(I);
+      Dg -= ((double)L.l[0] + (double)L.h[0]);
+      Dg -= ((double)L.l[1] + (double)L.h[1]);
+      if (g.D == 3) Dg -= ((double)L.l[2] + (double)L.h[2]);
       double ax = xI * Dg;
-#pragma unroll
-      for (int i = 0; i < 3; ++i) {
-        if (i >= g.D) continue;
-        const int64_t si = g.s[i];
-        ax += (xv(I - si) * (double)L.lo(I, i) + xv(I + si) * (double)L.hi(I, i, si));
-      }
+      ax += (v.m[0] * (double)L.l[0] + v.p[0] * (double)L.h[0]);
+      ax += (v.m[1] * (double)L.l[1] + v.p[1] * (double)L.h[1]);
+      if (g.D == 3) ax += (v.m[2] * (double)L.l[2] + v.p[2] * (double)L.h[2]);
       const double rv = Dg == 0.0 ? 0.0 : (double)zv - ax;
       xb[I] = xI;
       r[I] = (T)rv;
