@@ -451,17 +451,38 @@ __device__ __forceinline__ void k_bdim2_body(const Geo& g, const ZT& t, T* __res
   const int64_t n = g.n;
   const bool band = rows[c1 + (int64_t)g.N[1] * c2] != 0;
   const bool vnz = vrows[c1 + (int64_t)g.N[1] * c2] != 0;
+  // μ₀ first (its loads are issued only for rows without the row constants), then every
+  // component's f and u (a 2D level rereads component 1), V and the μ₁ band terms where the row
+  // has them: all reads come before the first store
   const LC<T> m0 = lcell(g, mu0, c0, c1, c2);
+  T fv[3], uv[3], vv[3] = {T(0), T(0), T(0)}, sb[3] = {T(0), T(0), T(0)};
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    const int64_t k = I + dir_or_1(g, i) * n;
+    fv[i] = f[k];
+    uv[i] = PRED ? uold[k] : u[k];
+  }
+  if (vnz) {
+#pragma unroll
+    for (int i = 0; i < 3; ++i) vv[i] = V[I + dir_or_1(g, i) * n];
+  }
+  if (band) {
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      if (i >= D) continue;
+      const int64_t k = I + i * n;
+      T s = 0;
+      for (int j = 0; j < D; ++j) s += mu1[I + (i + (int64_t)D * j) * n] * (f[k + g.s[j]] - f[k - g.s[j]]);
+      sb[i] = s;
+    }
+  }
 #pragma unroll
   for (int i = 0; i < 3; ++i) {
     if (i >= D) continue;
     const int64_t k = I + i * n;
-    T s = 0;
-    if (band)
-      for (int j = 0; j < D; ++j) s += mu1[I + (i + (int64_t)D * j) * n] * (f[k + g.s[j]] - f[k - g.s[j]]);
-    const T inc = s / T(2) + (vnz ? V[k] : T(0)) + m0.lo(I, i) * f[k];
-    if (PRED) u[k] = uold[k] * T(0) + inc;
-    else u[k] = (u[k] + inc) * T(0.5);
+    const T inc = sb[i] / T(2) + (vnz ? vv[i] : T(0)) + m0.l[i] * fv[i];
+    if (PRED) u[k] = uv[i] * T(0) + inc;
+    else u[k] = (uv[i] + inc) * T(0.5);
   }
   }
 }
