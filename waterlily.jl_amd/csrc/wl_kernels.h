@@ -852,11 +852,11 @@ __device__ __forceinline__ void inc_restrict_cell(const Geo& gf, const Geo& gc, 
 // The coarse index of a fine index is (c+1)>>1 along a coarsened direction (down(I)=(I+2)÷2 in
 // 1-based indices); a ghost index maps to its periodic image, or to "no value" (ϵ = 0).
 __device__ __forceinline__ int pro_map(int c, int N, bool per, bool coarse) {
-  if (c == 0 || c == N - 1) {
-    if (!per) return -1;
-    c = c == 0 ? N - 2 : 1;
-  }
-  return coarse ? (c + 1) >> 1 : c;
+  // selects only (no branches): the ghost's periodic image, then the coarse index
+  const bool lo = c == 0, hi = c == N - 1;
+  const int m = lo ? N - 2 : (hi ? 1 : c);
+  const int q = coarse ? (m + 1) >> 1 : m;
+  return ((lo | hi) & !per) ? -1 : q;
 }
 template <class T>
 __device__ __forceinline__ void prolong_cell(const Geo& gf, const Geo& gc, int cm, int perdir, const LRows<T>& LR,
