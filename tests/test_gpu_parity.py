@@ -166,6 +166,29 @@ def test_c2_sphere_ten_steps(wl):
     assert sim.pois.n == osim.n
 
 
+def test_c3_sphere_steps_bitwise(wl):
+    """The largest single-GPU configuration, 3D sphere 512x256x256 (r = 32, Re 1e4, Float32, the
+    tuned tile targets of the bench): three steps bit for bit against the Float32 oracle (u, p,
+    every Δt and V-cycle count), and the pressure force within 1e-9 relative (both sum in FP64,
+    in different orders)."""
+    say("case: 3D sphere 512x256x256, 3 steps, bitwise")
+    n, nu = (512, 256, 256), 64 / 1e4
+    sim, osim = oracle_pair(wl, n, (1, 0, 0), 32, (128, 128, 128), nu)
+    sim.flow.set_tiles(16384, 16384)
+    for k in range(3):
+        wl.sim_step(sim, remeasure=False)
+        osim.step(remeasure=False)
+        say(f"  step {k + 1}: Δt {sim.flow.Δt[-1]} oracle {osim.flow.dt[-1]}; n {sim.pois.n[-2:]} oracle {osim.n[-2:]}")
+    u, p = sim.flow.u, sim.flow.p
+    uo, po = osim.flow.get("u"), osim.flow.get("p")
+    say(f"  u bit-equal {np.mean(u == uo):.6f}  p bit-equal {np.mean(p == po):.6f}")
+    assert np.array_equal(u, uo) and np.array_equal(p, po)
+    assert sim.flow.Δt == osim.flow.dt and sim.pois.n == osim.n
+    F, Fo = wl.pressure_force(sim), osim.pressure_force()
+    say(f"  pressure force {F} oracle {Fo}")
+    assert np.max(np.abs(F - Fo)) <= 1e-9 * np.max(np.abs(Fo))
+
+
 @pytest.mark.skipif(os.environ.get("WL_LONG_PARITY") != "1",
                     reason="about 7 minutes of oracle time: set WL_LONG_PARITY=1 to run it")
 def test_c2_sphere_1000_steps_forces(wl):
