@@ -1677,10 +1677,30 @@ __global__ void __launch_bounds__(256) k_cfl(Geo g, ZT t, const T* __restrict__ 
 // Reductions' second stage (one block): Σ partials (double) or max partials.
 // =============================================================================================
 // mode 0: out[0] = Σ ; mode 1 (residual! mean): out[0] = T(Σ)/N_in as T, out[1] = |s| > 2eps
+// ω of solver!(ml) kept on the device (src/MultiLevelPoisson.jl:110,117-121): st[3] = ω,
+// st[4] = r₂ (both in T).  Init sets ω = 1 and r₂ = Σr² after residual!; control takes the new
+// Σr² (the value stored in dsc[1], passed in: the fused finish must not read it back through
+// another pointer), adapts ω, stores r₂ and keeps the old r₂ in dsc[6].
+template <class T>
+__device__ __forceinline__ void mg_init_state(double r2sum, T* __restrict__ st) {
+  st[3] = T(1);
+  st[4] = (T)r2sum;
+}
+template <class T>
+__device__ __forceinline__ void mg_control_state(double r2sum, double* __restrict__ dsc, T* __restrict__ st) {
+  const T rnew = (T)r2sum;
+  T w = st[3];
+  const T r2 = st[4];
+  if (rnew >= r2) w = (T)fmax(0.2, 0.9 * (double)w);
+  else if (rnew < r2) w = (T)fmin(1.0, 1.02 * (double)w);
+  st[3] = w;
+  st[4] = rnew;
+  dsc[6] = (double)r2;
+}
 template <class T>
 __global__ void __launch_bounds__(256) k_finish_sum(const double* __restrict__ partial, int64_t np,
                                                     double* __restrict__ outd, T* __restrict__ outT, int mode,
-                                                    double ninside) {
+                                                    double ninside, double* __restrict__ dsc = nullptr) {
   double acc = 0;
   for (int64_t q = threadIdx.x; q < np; q += blockDim.x) acc += partial[q];
   const double s = block_sum(acc);
@@ -1691,6 +1711,9 @@ __global__ void __launch_bounds__(256) k_finish_sum(const double* __restrict__ p
       outT[0] = sm;
       outT[1] = fabs(sm) <= T(2) * eps_t<T>() ? T(0) : T(1);
     }
+    // the solver's state update right after its Σr² (into dsc[1]) instead of a launch of its own
+    if (mode == 2) mg_init_state<T>(s, outT);
+    if (mode == 3) mg_control_state<T>(s, dsc, outT);
   }
 }
 template <class T>
@@ -1981,24 +2004,14 @@ __global__ void __launch_bounds__(256) k_uu(int64_t n, int D, const T* __restric
     }
 }
 
-// ω of solver!(ml) kept on the device (src/MultiLevelPoisson.jl:110,117-121): st[3] = ω,
-// st[4] = r₂ (both in T).  k_mg_init sets ω = 1 and r₂ = Σr² after residual!; k_mg_control
-// takes the new Σr² (dsc[1], rounded to T), adapts ω, stores r₂ and keeps the old r₂ in dsc[6].
+// the solver state updates (mg_init_state, mg_control_state) as launches of their own
 template <class T>
 __global__ void k_mg_init(const double* __restrict__ dsc, T* __restrict__ st) {
-  if (threadIdx.x == 0) { st[3] = T(1); st[4] = (T)dsc[1]; }
+  if (threadIdx.x == 0) mg_init_state<T>(dsc[1], st);
 }
 template <class T>
 __global__ void k_mg_control(double* __restrict__ dsc, T* __restrict__ st) {
-  if (threadIdx.x != 0) return;
-  const T rnew = (T)dsc[1];
-  T w = st[3];
-  const T r2 = st[4];
-  if (rnew >= r2) w = (T)fmax(0.2, 0.9 * (double)w);
-  else if (rnew < r2) w = (T)fmin(1.0, 1.02 * (double)w);
-  st[3] = w;
-  st[4] = rnew;
-  dsc[6] = (double)r2;
+  if (threadIdx.x == 0) mg_control_state<T>(dsc[1], dsc, st);
 }
 
 // ---- mixed precision: FP32 V-cycles inside an FP64 defect-correction loop ----

@@ -368,9 +368,11 @@ struct MG {
     for (size_t l = 0; l < lv.size(); ++l) build_lrows((int)l);
   }
 
+  // mode 0: Σ into dsc[slot]; 1: also the mean and its flag (tsc[0..1]); 2 and 3 (slot 1): then the
+  // solver's init or ω control on the device state, as k_mg_init / k_mg_control would
   double finish_sum(int64_t np, int slot, int mode = 0, double ninside = 1) {
     hipLaunchKernelGGL(k_finish_sum<T>, dim3(1), dim3(256), 0, st, B->partial, np, B->dsc + slot, (T*)B->tsc, mode,
-                       ninside);
+                       ninside, B->dsc);
     KCK();
     return 0;
   }
@@ -403,7 +405,8 @@ struct MG {
     finish_sum(nb, 0, 1, (double)ninside(p.g));
     submean_l2();
   }
-  void submean_l2() {
+  // init: also set the solver's start state (ω = 1, r₂ = Σr²), so mg_loop(..., true) skips k_mg_init
+  void submean_l2(bool init = false) {
     Level<T>& p = lv[0];
     ZT b = make_zt(box_inside(p.g));
     const int64_t nb = zt_blocks(b);
@@ -411,7 +414,7 @@ struct MG {
     hipLaunchKernelGGL(k_submean_l2<T>, zt_grid(b), kBlock, 0, st, p.g, b, lr(0), p.r, p.e, (const T*)B->tsc,
                        B->partial);
     KCK();
-    finish_sum(nb, 1);
+    finish_sum(nb, 1, init ? 2 : 0);
   }
   static int64_t ninside(const Geo& g) {
     int64_t m = 1;
@@ -466,7 +469,8 @@ struct MG {
       KCK();
     }
   }
-  void increment(int l, const T* w, bool l2) {
+  // fmode: the finish mode of the Σr² (3: the ω control follows on the device)
+  void increment(int l, const T* w, bool l2, int fmode = 0) {
     Level<T>& p = lv[l];
     perbc(p.g, p.e);
     ZT b = make_zt(box_inside(p.g));
@@ -477,7 +481,7 @@ struct MG {
       hipLaunchKernelGGL((k_increment<T, false>), zt_grid(b), kBlock, 0, st, p.g, b, lr(l), p.r, p.x, p.e, w,
                          B->partial);
     KCK();
-    if (l2) finish_sum(zt_blocks(b), 1);
+    if (l2) finish_sum(zt_blocks(b), 1, fmode);
   }
 
   // LDS bytes the levels l..end need in the coarse tail (L, x, r, ϵ of each)
@@ -663,9 +667,7 @@ struct MG {
     if (!first) eps_init(0);
     vcycle(w);
     gs_sweeps(0);
-    increment(0, w, true);
-    hipLaunchKernelGGL(k_mg_control<T>, dim3(1), dim3(64), 0, st, B->dsc, (T*)B->tsc);
-    KCK();
+    increment(0, w, true, 3);  // L₂, then the ω update in the same single-block finish
   }
   // With WL_GRAPHS=1 in the environment when the solver is built, each kind of iteration is
   // captured once as a HIP graph and replayed; otherwise (the default: replay measured ~2% slower
@@ -701,9 +703,12 @@ struct MG {
 
   // solver!(ml) loop (src/MultiLevelPoisson.jl:108-127) after residual!: returns V-cycle count.
   // The host reads Σr² back once per iteration for the stop test, as the reference does.
-  int mg_loop(double tol, int itmx, std::vector<double>* hist) {
-    hipLaunchKernelGGL(k_mg_init<T>, dim3(1), dim3(64), 0, st, (const double*)B->dsc, (T*)B->tsc);
-    KCK();
+  // init_done: submean_l2(true) has already set the start state.
+  int mg_loop(double tol, int itmx, std::vector<double>* hist, bool init_done = false) {
+    if (!init_done) {
+      hipLaunchKernelGGL(k_mg_init<T>, dim3(1), dim3(64), 0, st, (const double*)B->dsc, (T*)B->tsc);
+      KCK();
+    }
     int n = 0;
     if (itmx <= 0) {
       if (hist) hist->push_back((double)(T)read_dsc(1));
@@ -1232,10 +1237,11 @@ struct FlowT : FlowBase {
     }
     mg.perbc(g, xw);
     mg.finish_sum(nb, 0, 1, (double)MG<T>::ninside(g));
-    mg.submean_l2();
     const bool use_net = net && (!(net_flags & WL_WARM_PREDICTOR) || w == T(1));
+    // without a guess the solver's start state is set in submean's finish (the guess would change Σr²)
+    mg.submean_l2(!use_net);
     if (use_net) warm_guess();
-    const int nit = mg.mg_loop(tol, itmx, &r2hist);
+    const int nit = mg.mg_loop(tol, itmx, &r2hist, !use_net);
     if (use_net && (net_flags & WL_WARM_RECORD)) net->record<T>(g, (const T*)xw, st);
     ZT ba = make_zt(box_all(g), 2 * zt_target_default());
     {
