@@ -300,11 +300,12 @@ __device__ __forceinline__ void k_conv_lds_body(const Geo& g, const ZT& t, const
   T Wn[3] = {0, 0, 0}, Hn[NHT], Hc[NHT], Vn[3] = {0, 0, 0};
   for (int k = 0; k < NHT; ++k) Hc[k] = ld_halo(k, z0);
   T rpart[3] = {0, 0, 0}, Fzp[3] = {0, 0, 0}, Vp[3] = {0, 0, 0};
+  T U0c[3] = {0, 0, 0};  // the corrector's u⁰ of the plane this iteration finishes
 
   struct T3 { T v[3]; };
-  // u⁰ of the corrector is read here rather than prefetched: the registers it would hold keep the
-  // kernel within 128 VGPRs (two workgroups per CU)
-  auto finish = [&](int cz, T3 rr, T3 uvals, T3 Vv) {
+  // the corrector's u⁰ is read at the start of the iteration that finishes its plane, so the
+  // load overlaps the flux evaluations (a plane earlier would need three more live registers)
+  auto finish = [&](int cz, T3 rr, T3 uvals, T3 Vv, T3 u0vals) {
     const int64_t I = at(g, cx, cy, cz);
 #pragma unroll
     for (int i = 0; i < D; ++i) {
@@ -312,13 +313,17 @@ __device__ __forceinline__ void k_conv_lds_body(const Geo& g, const ZT& t, const
       if (acc.on) r += (T)acc.add[i];
       if (acc.field) r += ((const T*)acc.field)[I + i * n];
       const T uv = uvals.v[i];
-      const T u0v = PRED ? uv : u0[I + i * n];
+      const T u0v = PRED ? uv : u0vals.v[i];
       f[I + i * n] = u0v + dt * r - Vv.v[i];
     }
   };
   auto ld_cell = [&](const T* a, int i, int z) -> T { return (outxy && z <= N2 - 1) ? a[at(g, cx, cy, z) + i * n] : T(0); };
 
   for (int cz = z0; cz <= zend; ++cz) {
+    if (!PRED) {
+      const int zf = D == 3 ? cz - 1 : cz;  // the plane finished in this iteration
+      for (int i = 0; i < D; ++i) U0c[i] = (D == 2 || cz > z0) ? ld_cell(u0, i, zf) : T(0);
+    }
     if (!PF && cz > z0) {  // without prefetch: advance the window and load the halo now
       for (int i = 0; i < D; ++i) { W[i][0] = W[i][1]; W[i][1] = W[i][2]; W[i][2] = W[i][3]; W[i][3] = D == 3 ? ldc(i, cz + 1) : T(0); }
       for (int k = 0; k < NHT; ++k) Hc[k] = ld_halo(k, cz);
@@ -341,6 +346,7 @@ __device__ __forceinline__ void k_conv_lds_body(const Geo& g, const ZT& t, const
     {
       const bool vnz = outxy && cz <= N2 - 1 && vrows[cy + (int64_t)N1 * cz];  // V is +0 in unflagged rows
       for (int i = 0; i < D; ++i) Vn[i] = vnz ? ld_cell(V, i, cz) : T(0);
+
     }
     __syncthreads();
     const int c[3] = {cx, cy, cz};
@@ -388,7 +394,9 @@ __device__ __forceinline__ void k_conv_lds_body(const Geo& g, const ZT& t, const
         rxy[i] = r;
       }
     if (D == 2) {
-      if (outxy) finish(cz, T3{{rxy[0], rxy[1], rxy[2]}}, T3{{W[0][2], W[1][2], W[2][2]}}, T3{{Vn[0], Vn[1], Vn[2]}});
+      if (outxy)
+        finish(cz, T3{{rxy[0], rxy[1], rxy[2]}}, T3{{W[0][2], W[1][2], W[2][2]}}, T3{{Vn[0], Vn[1], Vn[2]}},
+               T3{{U0c[0], U0c[1], U0c[2]}});
     } else {
       if (cz > z0 && outxy) {  // plane cz-1 is complete: add its z fluxes
         const bool zin = cx >= 1 && cy >= 1 && cz - 1 >= 1 && cz - 1 <= N2 - 2;
@@ -397,7 +405,8 @@ __device__ __forceinline__ void k_conv_lds_body(const Geo& g, const ZT& t, const
           rr[i] = rpart[i];
           if (zin) { rr[i] += Fzp[i]; rr[i] -= F[i][2]; }
         }
-        finish(cz - 1, T3{{rr[0], rr[1], rr[2]}}, T3{{W[0][1], W[1][1], W[2][1]}}, T3{{Vp[0], Vp[1], Vp[2]}});
+        finish(cz - 1, T3{{rr[0], rr[1], rr[2]}}, T3{{W[0][1], W[1][1], W[2][1]}}, T3{{Vp[0], Vp[1], Vp[2]}},
+               T3{{U0c[0], U0c[1], U0c[2]}});
       }
       for (int i = 0; i < 3; ++i) { rpart[i] = rxy[i]; Fzp[i] = F[i][2]; Vp[i] = Vn[i]; }
     }
@@ -419,7 +428,10 @@ __device__ __forceinline__ void k_conv_lds_body(const Geo& g, const ZT& t, const
     }
   }
   if (D == 3 && zend == z1 && outxy) {  // top ghost plane: no z fluxes
-    finish(z1, T3{{rpart[0], rpart[1], rpart[2]}}, T3{{W[0][2], W[1][2], W[2][2]}}, T3{{Vp[0], Vp[1], Vp[2]}});
+    if (!PRED)
+      for (int i = 0; i < D; ++i) U0c[i] = ld_cell(u0, i, z1);
+    finish(z1, T3{{rpart[0], rpart[1], rpart[2]}}, T3{{W[0][2], W[1][2], W[2][2]}}, T3{{Vp[0], Vp[1], Vp[2]}},
+           T3{{U0c[0], U0c[1], U0c[2]}});
   }
 }
 template <class T, int S, bool PRED, int D, bool PF>
