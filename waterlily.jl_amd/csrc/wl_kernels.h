@@ -1663,11 +1663,20 @@ __device__ __forceinline__ void k_cfl_body(const Geo& g, const ZT& t, const T* _
       for (int d = 0; d < g.D; ++d) in &= c[d] >= 1 && c[d] <= g.N[d] - 2;
       T v;
       if (in) {
-        T s = 0;
-        for (int i = 0; i < g.D; ++i) {
-          const T a = u[I + g.s[i] + i * g.n], bq = -u[I + i * g.n];
-          s += ((a > T(0) ? a : T(0)) + (bq > T(0) ? bq : T(0)));
+        // every component's two faces first (a 2D level rereads component 1), then the sum
+        T up[3], uc[3];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+          const int k = dir_or_1(g, i);
+          up[i] = u[I + g.s[k] + k * g.n];
+          uc[i] = u[I + k * g.n];
         }
+        auto out = [](T a, T b) { return (a > T(0) ? a : T(0)) + (b > T(0) ? b : T(0)); };
+        T s = 0;
+        s += out(up[0], -uc[0]);
+        s += out(up[1], -uc[1]);
+        const T s3 = s + out(up[2], -uc[2]);
+        s = g.D == 3 ? s3 : s;
         sigma[I] = s;
         v = s;
       } else {
