@@ -2041,8 +2041,10 @@ __global__ void k_mg_control(double* __restrict__ dsc, T* __restrict__ st) {
 // (level-0 x, started from 0).  FLOW: first pass of a projection, z = div(u) and
 // x64 = (double)(p·scale) (the warm start x .*= w·Δt, src/Flow.jl:225); otherwise
 // x64 ← x64 + c, read on the fly at the stencil's cells (xa and xb are distinct buffers).
-// Partials: Σr in partial[b], Σr² in partial[nb + b], both in double.
-template <class T, bool FLOW>
+// Partials: Σr in partial[b], Σr² in partial[nb + b], both in double.  EINC: einc and wp are
+// given (the correction's last increment is fused), as a template argument so the neighbour
+// reads carry no branch.
+template <class T, bool FLOW, bool EINC = false>
 __global__ void __launch_bounds__(256) k_residual64(Geo g, ZT t, LRows<T> LR, const T* __restrict__ pin, T scale,
                                                     T* __restrict__ z, const T* __restrict__ u,
                                                     const double* __restrict__ xa, const T* __restrict__ xc,
@@ -2067,7 +2069,7 @@ __global__ void __launch_bounds__(256) k_residual64(Geo g, ZT t, LRows<T> LR, co
       // einc: the correction's last increment! (c + ω·ϵ in T, src/Poisson.jl:103) taken on the fly
       auto xv = [&](int64_t J) -> double {
         if (FLOW) return (double)(pin[J] * scale);
-        return einc ? xa[J] + (double)(xc[J] + wp[0] * einc[J]) : xa[J] + (double)xc[J];
+        return EINC ? xa[J] + (double)(xc[J] + wp[0] * einc[J]) : xa[J] + (double)xc[J];
       };
       // x and its neighbours, then L, before the arithmetic
       const double xI = xv(I);

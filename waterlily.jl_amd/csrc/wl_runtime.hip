@@ -772,10 +772,15 @@ struct MG {
       {
         ProfScope ps(P, st, "residual64", 0,
                      (double)ncells_inside(p.g) * (sizeof(T) * (p.g.D + 2 + (fuse ? 1 : 0)) + 2 * sizeof(double)));
-        hipLaunchKernelGGL((k_residual64<T, false>), zt_grid(bi), kBlock, 0, st, p.g, bi, lr(0), (const T*)nullptr,
-                           T(0), p.z, (const T*)nullptr, (const double*)x64[x64cur], (const T*)p.x,
-                           x64[1 - x64cur], p.r, B->partial, nb, fuse ? (const T*)p.e : (const T*)nullptr,
-                           fuse ? w : (const T*)nullptr);
+        if (fuse)
+          hipLaunchKernelGGL((k_residual64<T, false, true>), zt_grid(bi), kBlock, 0, st, p.g, bi, lr(0),
+                             (const T*)nullptr, T(0), p.z, (const T*)nullptr, (const double*)x64[x64cur],
+                             (const T*)p.x, x64[1 - x64cur], p.r, B->partial, nb, (const T*)p.e, w);
+        else
+          hipLaunchKernelGGL((k_residual64<T, false, false>), zt_grid(bi), kBlock, 0, st, p.g, bi, lr(0),
+                             (const T*)nullptr, T(0), p.z, (const T*)nullptr, (const double*)x64[x64cur],
+                             (const T*)p.x, x64[1 - x64cur], p.r, B->partial, nb, (const T*)nullptr,
+                             (const T*)nullptr);
         KCK();
       }
       x64cur = 1 - x64cur;
