@@ -2059,9 +2059,18 @@ __global__ void __launch_bounds__(256) k_residual64(Geo g, ZT t, LRows<T> LR, co
       const int64_t I = at(g, c0, c1, c2);
       T zv;
       if (FLOW) {
+        T up[3], uc[3];  // every component first (a 2D level rereads component 1)
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+          const int k = dir_or_1(g, i);
+          up[i] = u[I + g.s[k] + k * g.n];
+          uc[i] = u[I + k * g.n];
+        }
         T s = 0;
-        for (int i = 0; i < g.D; ++i) s += (u[I + g.s[i] + i * g.n] - u[I + i * g.n]);
-        zv = s;
+        s += (up[0] - uc[0]);
+        s += (up[1] - uc[1]);
+        const T s3 = s + (up[2] - uc[2]);
+        zv = g.D == 3 ? s3 : s;
         z[I] = zv;
       } else {
         zv = z[I];
