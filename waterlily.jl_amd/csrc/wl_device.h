@@ -61,11 +61,13 @@ struct ZT {
 // Workgroups are dealt round-robin to the 8 XCDs (each with its own L2).  Re-number them so that
 // XCD k receives one contiguous range of tiles: neighbouring tiles, which read each other's halo
 // rows and planes, then share an L2.  The map is a bijection for any grid size.
-__device__ __forceinline__ int xcd_tile_id() {
-  const int nb = gridDim.x, bid = blockIdx.x;
+// the tile of block bid of an nb-block launch (bid and nb may be virtual: a grid-stride loop over
+// the blocks of a launch of nb blocks)
+__device__ __forceinline__ int xcd_tile_of(int bid, int nb) {
   const int q = nb >> 3, rr = nb & 7, x = bid & 7, k = bid >> 3;
   return x * q + (x < rr ? x : rr) + k;
 }
+__device__ __forceinline__ int xcd_tile_id() { return xcd_tile_of((int)blockIdx.x, (int)gridDim.x); }
 // (x, y) cell of this thread and the z range of tile `lid` (a tile index of this launch's box)
 __device__ __forceinline__ void zt_cell_id(const ZT& t, int lid, int& i, int& j, int& z0, int& z1) {
   const int tx = lid % t.nbx, r1 = lid / t.nbx;

@@ -907,14 +907,17 @@ __device__ __forceinline__ void prolong_cell(const Geo& gf, const Geo& gc, int c
 
 // ---- residual! (src/Poisson.jl:92-98) fused with mom_project!'s z = div(u) and x .*= dt
 // (src/Flow.jl:225).  xin·scale is written to xout; with FLOW the source z is div(u).
-template <class T, bool FLOW>
+// EL2: also what k_submean_l2 forms when no mean is subtracted, from the same values over the
+// same tiles: ϵ = r·iD into eout and the block Σr² into partial[pstride + pidx].
+template <class T, bool FLOW, bool EL2 = false>
 __device__ __forceinline__ void k_residual_body(const Geo& g, const ZT& t, const LRows<T>& LR, const T* __restrict__ xin,
                                                   T* __restrict__ xout, T scale, int write_x, T* __restrict__ z,
                                                   const T* __restrict__ u, T* __restrict__ r,
-                                                  double* __restrict__ partial, int lid, int pidx) {
+                                                  double* __restrict__ partial, int lid, int pidx,
+                                                  T* __restrict__ eout = nullptr, int64_t pstride = 0) {
   int c0, c1, z0, z1;
   zt_cell_id(t, lid, c0, c1, z0, z1);
-  double acc = 0;
+  double acc = 0, acc2 = 0;
   if (c0 <= t.b.hi[0] && c1 <= t.b.hi[1]) {
     for (int c2 = z0; c2 <= z1; ++c2) {
       const int64_t I = at(g, c0, c1, c2);
@@ -947,17 +950,27 @@ __device__ __forceinline__ void k_residual_body(const Geo& g, const ZT& t, const
       r[I] = rv;
       if (write_x) xout[I] = x0;
       acc += (double)rv;
+      if (EL2) {
+        eout[I] = rv * iD;
+        acc2 += (double)rv * (double)rv;
+      }
     }
   }
   const double s = block_sum(acc);
   if (threadIdx.x == 0 && threadIdx.y == 0) partial[pidx] = s;
+  if (EL2) {
+    const double s2 = block_sum(acc2);
+    if (threadIdx.x == 0 && threadIdx.y == 0) partial[pstride + pidx] = s2;
+  }
 }
-template <class T, bool FLOW>
+template <class T, bool FLOW, bool EL2 = false>
 __global__ void __launch_bounds__(256) k_residual(Geo g, ZT t, LRows<T> LR, const T* __restrict__ xin,
                                                   T* __restrict__ xout, T scale, int write_x, T* __restrict__ z,
                                                   const T* __restrict__ u, T* __restrict__ r,
-                                                  double* __restrict__ partial) {
-  k_residual_body<T, FLOW>(g, t, LR, xin, xout, scale, write_x, z, u, r, partial, xcd_tile_id(), (int)blockIdx.x);
+                                                  double* __restrict__ partial, T* __restrict__ eout = nullptr,
+                                                  int64_t pstride = 0) {
+  k_residual_body<T, FLOW, EL2>(g, t, LR, xin, xout, scale, write_x, z, u, r, partial, xcd_tile_id(),
+                                (int)blockIdx.x, eout, pstride);
 }
 
 // r -= s (when |s| > 2eps) ; Σr² partials ; ϵ = r·iD (Jacobi!/GS init of the first V-cycle)
@@ -987,6 +1000,17 @@ __global__ void __launch_bounds__(256) k_submean_l2(Geo g, ZT t, LRows<T> LR, T*
                                                     T* __restrict__ e, const T* __restrict__ sdev,
                                                     double* __restrict__ partial) {
   k_submean_l2_body<T>(g, t, LR, r, e, sdev, partial, xcd_tile_id(), (int)blockIdx.x);
+}
+// The same, only when the mean is subtracted, over a small grid: each block walks the blocks
+// b = blockIdx.x + k·gridDim.x of the nb-block launch above (same tiles, same partial slots), so an
+// unflagged call costs one small dispatch.
+template <class T>
+__global__ void __launch_bounds__(256) k_submean_l2_cond(Geo g, ZT t, LRows<T> LR, T* __restrict__ r,
+                                                         T* __restrict__ e, const T* __restrict__ sdev,
+                                                         double* __restrict__ partial, int nb) {
+  if (sdev[1] == T(0)) return;
+  for (int b = (int)blockIdx.x; b < nb; b += (int)gridDim.x)
+    k_submean_l2_body<T>(g, t, LR, r, e, sdev, partial, xcd_tile_of(b, nb), b);
 }
 
 // ϵ = r·iD on inside (Jacobi!/GS-RB init, src/Poisson.jl:112,142)

@@ -370,9 +370,10 @@ struct MG {
 
   // mode 0: Σ into dsc[slot]; 1: also the mean and its flag (tsc[0..1]); 2 and 3 (slot 1): then the
   // solver's init or ω control on the device state, as k_mg_init / k_mg_control would
-  double finish_sum(int64_t np, int slot, int mode = 0, double ninside = 1) {
-    hipLaunchKernelGGL(k_finish_sum<T>, dim3(1), dim3(256), 0, st, B->partial, np, B->dsc + slot, (T*)B->tsc, mode,
-                       ninside, B->dsc);
+  // the partials start at B->partial + poff
+  double finish_sum(int64_t np, int slot, int mode = 0, double ninside = 1, int64_t poff = 0) {
+    hipLaunchKernelGGL(k_finish_sum<T>, dim3(1), dim3(256), 0, st, (const double*)B->partial + poff, np, B->dsc + slot,
+                       (T*)B->tsc, mode, ninside, B->dsc);
     KCK();
     return 0;
   }
@@ -405,16 +406,22 @@ struct MG {
     finish_sum(nb, 0, 1, (double)ninside(p.g));
     submean_l2();
   }
-  // init: also set the solver's start state (ω = 1, r₂ = Σr²), so mg_loop(..., true) skips k_mg_init
-  void submean_l2(bool init = false) {
+  // init: also set the solver's start state (ω = 1, r₂ = Σr²), so mg_loop(..., true) skips k_mg_init.
+  // cond: the residual kernel already wrote ϵ and the Σr² partials (at partial + nb) as they are
+  // without a mean subtraction; the kernel redoes them only when the mean is subtracted.
+  void submean_l2(bool init = false, bool cond = false) {
     Level<T>& p = lv[0];
     ZT b = make_zt(box_inside(p.g));
     const int64_t nb = zt_blocks(b);
     ProfScope ps(P, st, "submean_l2", 0, (double)sizeof(T) * (p.g.D + 3) * ncells_inside(p.g));
-    hipLaunchKernelGGL(k_submean_l2<T>, zt_grid(b), kBlock, 0, st, p.g, b, lr(0), p.r, p.e, (const T*)B->tsc,
-                       B->partial);
+    if (cond)
+      hipLaunchKernelGGL(k_submean_l2_cond<T>, dim3((unsigned)std::min<int64_t>(nb, 256)), kBlock, 0, st, p.g, b,
+                         lr(0), p.r, p.e, (const T*)B->tsc, B->partial + nb, (int)nb);
+    else
+      hipLaunchKernelGGL(k_submean_l2<T>, zt_grid(b), kBlock, 0, st, p.g, b, lr(0), p.r, p.e, (const T*)B->tsc,
+                         B->partial);
     KCK();
-    finish_sum(nb, 1, init ? 2 : 0);
+    finish_sum(nb, 1, init ? 2 : 0, 1, cond ? nb : 0);
   }
   static int64_t ninside(const Geo& g) {
     int64_t m = 1;
@@ -1235,16 +1242,18 @@ struct FlowT : FlowBase {
     ZT bi = make_zt(box_inside(g));
     const int64_t nb = zt_blocks(bi);
     {
-    ProfScope ps(&prof, st, "residual", 0, (double)sizeof(T) * (2 * D + 4) * ncells_inside(g));
-    hipLaunchKernelGGL((k_residual<T, true>), zt_grid(bi), kBlock, 0, st, g, bi, mg.lr(0), (const T*)p, xw, dt, 1,
-                       sigma, (const T*)u, l0.r, B.partial);
+    // with ϵ = r·iD and the Σr² partials (at partial + nb) as submean would form them unflagged
+    ProfScope ps(&prof, st, "residual", 0, (double)sizeof(T) * (2 * D + 5) * ncells_inside(g));
+    hipLaunchKernelGGL((k_residual<T, true, true>), zt_grid(bi), kBlock, 0, st, g, bi, mg.lr(0), (const T*)p, xw, dt,
+                       1, sigma, (const T*)u, l0.r, B.partial, l0.e, nb);
     KCK();
     }
     mg.perbc(g, xw);
     mg.finish_sum(nb, 0, 1, (double)MG<T>::ninside(g));
     const bool use_net = net && (!(net_flags & WL_WARM_PREDICTOR) || w == T(1));
-    // without a guess the solver's start state is set in submean's finish (the guess would change Σr²)
-    mg.submean_l2(!use_net);
+    // without a guess the solver's start state is set in submean's finish (the guess would change
+    // Σr²); submean itself runs only where the mean is subtracted
+    mg.submean_l2(!use_net, true);
     if (use_net) warm_guess();
     const int nit = mg.mg_loop(tol, itmx, &r2hist, !use_net);
     if (use_net && (net_flags & WL_WARM_RECORD)) net->record<T>(g, (const T*)xw, st);
