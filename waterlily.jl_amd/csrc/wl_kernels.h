@@ -733,8 +733,8 @@ __device__ __forceinline__ T diag_at(const Geo& g, const T* __restrict__ L, int6
 template <class T>
 __device__ __forceinline__ T inv_diag(T Dg) { return Dg == T(0) ? Dg : T(1) / Dg; }
 
-// The six neighbour values x(I ∓ e_i) of a cell, read before any arithmetic (a 2D level reads the
-// direction-1 pair again as the unused third pair).
+// The six neighbour values x(I ∓ e_i) of a cell, read before any arithmetic (on a 2D level the
+// unused third pair is 0).
 template <class T>
 struct Nb {
   T m[3], p[3];
@@ -743,10 +743,16 @@ template <class T, class X>
 __device__ __forceinline__ Nb<T> gather_nb(const Geo& g, int64_t I, X xv) {
   Nb<T> v;
 #pragma unroll
-  for (int i = 0; i < 3; ++i) {
-    const int64_t si = g.s[dir_or_1(g, i)];
-    v.m[i] = xv(I - si);
-    v.p[i] = xv(I + si);
+  for (int i = 0; i < 2; ++i) {
+    v.m[i] = xv(I - g.s[i]);
+    v.p[i] = xv(I + g.s[i]);
+  }
+  // the third pair only on a 3D level (a uniform branch; nothing is used before all are issued)
+  v.m[2] = T(0);
+  v.p[2] = T(0);
+  if (g.D == 3) {
+    v.m[2] = xv(I - g.s[2]);
+    v.p[2] = xv(I + g.s[2]);
   }
   return v;
 }
