@@ -207,6 +207,7 @@ def copy_calibration(nbytes=1 << 30, reps=5):
 
 # rocprofv3 kernel names -> the library's profiler keys (level 0 = the largest dispatch of a name)
 PROF_KEYS = {"k_inc_restrict_full": "inc_restrict", "k_mg_tail": "mg_tail", "k_mg_tail_lds": "mg_tail", "k_bdim2": "bdim2", "k_residual": "residual", "k_submean_l2": "submean_l2",
+             "k_submean_l2_cond": "submean_l2",
              "k_inc_restrict": "inc_restrict", "k_gs_half": "gs_half", "k_increment": "increment",
              "k_prolong_inc": "prolong_inc", "k_eps_init": "eps_init", "k_correct": "correct", "k_cfl": "cfl"}
 
