@@ -24,12 +24,13 @@ struct LRows {
 };
 template <class T>
 __host__ __device__ inline LRows<T> lrows_plain(const T* L) { return LRows<T>{L, nullptr, {T(0), T(0), T(0)}}; }
-// The coefficients seen from one cell (x, y, z): l[i] = L[I, i] and h[i] = L[I + e_i, i].  lcell
-// reads all of them up front and without branches, so that their loads issue together with the
-// cell's other loads: a uniform row reads (and discards) the first cells of L, which stay in
-// cache, and takes the row constants; a 2D level reads direction 1 again in place of the absent
-// direction 2 (l[2], h[2] are then never used); on an upper ghost plane h of that direction reads
-// L[I] instead (callers use h only at cells with an upper neighbour).
+// The coefficients seen from one cell (x, y, z): l[i] = L[I, i] and h[i] = L[I + e_i, i], all
+// formed up front with selects (no branch between them and their use).  A uniform row takes the
+// row constants: its L loads (pointed at the first cells of L) are then dead, so the compiler
+// issues them only for the lanes of other rows.  A 2D level reads direction 1 again in place of
+// the absent direction 2 (l[2], h[2] are then never used).  On the last plane of the array h of
+// that direction reads L[I] instead (callers use h only at cells with an upper neighbour).  The
+// stencils read a cell's field values before calling lcell, so the row flag arrives with them.
 template <class T>
 struct LC {
   T l[3], h[3];
