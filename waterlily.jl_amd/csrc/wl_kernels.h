@@ -877,10 +877,13 @@ __device__ __forceinline__ int pro_map(int c, int N, bool per, bool coarse) {
   const int q = coarse ? (m + 1) >> 1 : m;
   return ((lo | hi) & !per) ? -1 : q;
 }
-template <class T>
+// CINC: the coarse level's final increment!(coarse) is folded in: its x + ω·ϵ is formed here, at
+// the coarse cells read (the same T operations as increment!), from xc and ec.
+template <class T, bool CINC = false>
 __device__ __forceinline__ void prolong_cell(const Geo& gf, const Geo& gc, int cm, int perdir, const LRows<T>& LR,
                                              T* __restrict__ rf, T* __restrict__ xf, T* __restrict__ ef,
-                                             const T* __restrict__ xc, T w, const int* c) {
+                                             const T* __restrict__ xc, T w, const int* c,
+                                             const T* __restrict__ ec = nullptr) {
   const bool d3 = gf.D == 3;
   int q[3][3];  // q[d][o]: coarse index of c_d + o - 1 (0 along the absent direction of a 2D level)
 #pragma unroll
@@ -891,7 +894,9 @@ __device__ __forceinline__ void prolong_cell(const Geo& gf, const Geo& gc, int c
   // every coarse value is read unconditionally ("no value" reads cell 0 and is replaced by 0)
   auto xcv = [&](int a, int b, int z) -> T {
     const bool ok = a >= 0 && b >= 0 && z >= 0;
-    const T v = xc[ok ? (int64_t)a + gc.s[1] * b + gc.s[2] * z : 0];
+    const int64_t J = ok ? (int64_t)a + gc.s[1] * b + gc.s[2] * z : 0;
+    T v = xc[J];
+    if (CINC) v = v + w * ec[J];
     return ok ? v : T(0);
   };
   const int64_t I = at(gf, c[0], c[1], c[2]);
@@ -1233,24 +1238,27 @@ __global__ void __launch_bounds__(64 * GBY) k_gs_pair(Geo g, ZT t, LRows<T> LR, 
   }
 }
 
-// prolongate! + increment! on the fine level, then ϵ := r·iD (the fine level's GS-RB init)
-template <class T>
+// prolongate! + increment! on the fine level, then ϵ := r·iD (the fine level's GS-RB init);
+// CINC: with the coarse level's own increment! folded in (prolong_cell)
+template <class T, bool CINC = false>
 __device__ __forceinline__ void k_prolong_inc_body(const Geo& gf, const Geo& gc, const ZT& t, int cm, int perdir,
                                                      const LRows<T>& Lf, T* __restrict__ rf,
                                                      T* __restrict__ xf, T* __restrict__ ef,
-                                                     const T* __restrict__ xc, const T* __restrict__ wp, int lid) {
+                                                     const T* __restrict__ xc, const T* __restrict__ wp, int lid,
+                                                     const T* __restrict__ ec = nullptr) {
   const T w = *wp;
   int c[3], z0, z1;
   zt_cell_id(t, lid, c[0], c[1], z0, z1);
   if (c[0] > t.b.hi[0] || c[1] > t.b.hi[1]) return;
-  for (c[2] = z0; c[2] <= z1; ++c[2]) prolong_cell(gf, gc, cm, perdir, Lf, rf, xf, ef, xc, w, c);
+  for (c[2] = z0; c[2] <= z1; ++c[2]) prolong_cell<T, CINC>(gf, gc, cm, perdir, Lf, rf, xf, ef, xc, w, c, ec);
 }
-template <class T>
+template <class T, bool CINC = false>
 __global__ void __launch_bounds__(256) k_prolong_inc(Geo gf, Geo gc, ZT t, int cm, int perdir,
                                                      LRows<T> Lf, T* __restrict__ rf,
                                                      T* __restrict__ xf, T* __restrict__ ef,
-                                                     const T* __restrict__ xc, const T* __restrict__ wp) {
-  k_prolong_inc_body<T>(gf, gc, t, cm, perdir, Lf, rf, xf, ef, xc, wp, xcd_tile_id());
+                                                     const T* __restrict__ xc, const T* __restrict__ wp,
+                                                     const T* __restrict__ ec = nullptr) {
+  k_prolong_inc_body<T, CINC>(gf, gc, t, cm, perdir, Lf, rf, xf, ef, xc, wp, xcd_tile_id(), ec);
 }
 
 // ---- the coarse end of the V-cycle in one workgroup ----

@@ -440,6 +440,10 @@ struct MG {
   // sweeps run (on the 256x128x128 sphere a pair took 61 µs against 2×20 µs for two half sweeps:
   // its plane-by-plane barriers and one-colour-per-phase lanes cost more than the traffic saved)
   bool use_gs_pair = false;
+  // a flow's own solver, whose coarse levels nobody reads, skips the launch of each coarse level's
+  // last increment! (prolong_level(c, w, true)); a standalone MultiLevelPoisson keeps the
+  // reference's level state exactly
+  bool fold_coarse_inc = false;
   bool gs_pair_ok(int l) const {
     const Geo& g = lv[l].g;
     return use_gs_pair && perdir == 0 && g.D == 3 && g.N[2] % 2 == 0 && g.N[0] >= 3 && g.N[1] >= 3 && g.N[2] >= 4;
@@ -554,6 +558,10 @@ struct MG {
     for (int c = Lt > 0 ? Lt : Lm; c >= 1; --c) {  // up-leg: smooth!(coarse), prolongate! + increment!(fine)
       if (!(Lt > 0 && c == Lt)) {
         gs_sweeps(c);
+        if (fold_coarse_inc) {  // smooth!'s increment!(c) formed inside the prolongation
+          prolong_level(c, w, true);
+          continue;
+        }
         increment(c, w, false);
       }
       prolong_level(c, w);
@@ -585,14 +593,20 @@ struct MG {
     }
     KCK();
   }
-  // prolongate! from level c into c-1, with increment!(ω) and ϵ := r·iD there (src/MultiLevelPoisson.jl:98-100)
-  void prolong_level(int c, const T* w) {
+  // prolongate! from level c into c-1, with increment!(ω) and ϵ := r·iD there (src/MultiLevelPoisson.jl:98-100).
+  // fold: level c's own last increment! (x += ω·ϵ) is formed at the coarse cells read instead of
+  // being launched; its r update is dead (the next V-cycle's restrict! overwrites r and x there).
+  void prolong_level(int c, const T* w, bool fold = false) {
     Level<T>& f = lv[c - 1];
     ZT bf = make_zt(box_inside(f.g), tgt_rp());
     ProfScope ps(P, st, "prolong_inc", c - 1,
-                 (double)sizeof(T) * ((f.g.D + 5) * ncells_inside(f.g) + ncells_inside(lv[c].g)));
-    hipLaunchKernelGGL(k_prolong_inc<T>, zt_grid(bf), kBlock, 0, st, f.g, lv[c].g, bf, f.cmask, perdir, lr(c - 1), f.r,
-                       f.x, f.e, (const T*)lv[c].x, w);
+                 (double)sizeof(T) * ((f.g.D + 5) * ncells_inside(f.g) + (fold ? 2 : 1) * ncells_inside(lv[c].g)));
+    if (fold)
+      hipLaunchKernelGGL((k_prolong_inc<T, true>), zt_grid(bf), kBlock, 0, st, f.g, lv[c].g, bf, f.cmask, perdir,
+                         lr(c - 1), f.r, f.x, f.e, (const T*)lv[c].x, w, (const T*)lv[c].e);
+    else
+      hipLaunchKernelGGL((k_prolong_inc<T, false>), zt_grid(bf), kBlock, 0, st, f.g, lv[c].g, bf, f.cmask, perdir,
+                         lr(c - 1), f.r, f.x, f.e, (const T*)lv[c].x, w, (const T*)nullptr);
     KCK();
   }
 
@@ -944,6 +958,7 @@ struct FlowT : FlowBase {
     mg.P = &prof;
     mg.perdir = d.perdir_mask;
     mg.multilevel = true;
+    mg.fold_coarse_inc = true;
     mg.build(D, Ng, mu0, xw, sigma);
     // initial condition u = uBC(i,x,0) everywhere, then BC! (src/Flow.jl:138-141)
     double U0[3];
