@@ -126,6 +126,7 @@ struct ALev {  // one multigrid level of one simulation (and its coarse neighbou
   const T* wp;
   double* partial;
   const T* sdev;
+  int cinc;  // prolongation: the coarse level's last increment! is formed on the fly (prolong_cell)
 };
 template <class T>
 __global__ void __launch_bounds__(256) kb_submean(const ALev<T>* __restrict__ A, const BBlk* __restrict__ blk) {
@@ -173,14 +174,18 @@ __global__ void __launch_bounds__(256) kb_gs_half(const ALev<T>* __restrict__ A,
   const ALev<T>& a = A[b.sim];
   k_gs_half_body<T>(a.g, a.t, a.L, a.r, a.e, it, b.lid);
 }
-// prolongate! from the coarse level (xc) into this level, with this level's increment!
+// prolongate! from the coarse level (xc) into this level, with this level's increment!; for a
+// member that smoothed the coarse level by launches, that level's last increment! is folded in
 template <class T>
 __global__ void __launch_bounds__(256) kb_prolong(const ALev<T>* __restrict__ A, const BBlk* __restrict__ blk,
                                                   const int* __restrict__ act) {
   const BBlk b = blk[blockIdx.x];
   if (act && !act[b.sim]) return;
   const ALev<T>& a = A[b.sim];
-  k_prolong_inc_body<T>(a.g, a.gc, a.t, a.cm, a.perdir, a.L, a.r, a.x, a.e, a.xc, a.wp, b.lid);
+  if (a.cinc)
+    k_prolong_inc_body<T, true>(a.g, a.gc, a.t, a.cm, a.perdir, a.L, a.r, a.x, a.e, a.xc, a.wp, b.lid, a.ec);
+  else
+    k_prolong_inc_body<T, false>(a.g, a.gc, a.t, a.cm, a.perdir, a.L, a.r, a.x, a.e, a.xc, a.wp, b.lid);
 }
 // the coarse tail of each simulation: one workgroup per simulation (k_mg_tail_lds)
 template <class T>

@@ -1559,6 +1559,7 @@ struct BatchT : BatchBase {
         if (prol) {
           const ZT tp = make_zt(box_inside(g), F[s]->mg.tgt_rp());
           agp.push_back(lev_args(F[s], l, tp));
+          agp.back().cinc = l + 1 < Lt[s] ? 1 : 0;  // level l+1 smoothed by launches: fold its increment!
           add_blocks(bgp, s, zt_blocks(tp));
         } else {
           agp.push_back(ALev<T>{});
@@ -1796,12 +1797,7 @@ struct BatchT : BatchBase {
     hipLaunchKernelGGL(kb_mg_tail<T>, dim3(ns), dim3(1024), tail_lds, st, tails, tail_sims, (const int*)act);
     KCK();
     for (int c = Cmax; c >= 1; --c) {  // members with c < Lt: smooth!(c); members with c <= Lt: prolongate!
-      if (gs[c].nblk) {
-        gs_sweeps(c);
-        hipLaunchKernelGGL((kb_increment<T, false>), dim3(incr[c].nblk), kBlock, 0, st, (const ALev<T>*)incr[c].args,
-                           incr[c].blk, (const int*)act);
-        KCK();
-      }
+      if (gs[c].nblk) gs_sweeps(c);  // their last increment! is folded into the prolongation below
       const Launch& P = pro[c - 1];
       if (P.nblk) {
         hipLaunchKernelGGL(kb_prolong<T>, dim3(P.nblk), kBlock, 0, st, (const ALev<T>*)P.args, P.blk, (const int*)act);
