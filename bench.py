@@ -256,6 +256,11 @@ def summarize(stats_csv, fetch_csv, write_csv, out_md, out_json, cmd, calib_byte
                      (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3))
         lines += ["", "## Level-0 dispatches (the largest grid of each kernel, from the kernel trace)", "",
                   "| profiler key | kernel | dispatches | avg us |", "|---|---|---|---|"]
+        # a key's level 0 is its largest grid over all its kernels (an instance that runs on coarser
+        # levels only is listed as such)
+        kmax = collections.defaultdict(int)
+        for (k, kn), v in grids.items():
+            kmax[k] = max(kmax[k], max(g for g, _ in v))
         for (k, kn), v in sorted(grids.items(), key=lambda kv: -sum(d for g, d in kv[1])):
             gmax = max(g for g, _ in v)
             d0 = sorted(d for g, d in v if g == gmax)
@@ -269,7 +274,8 @@ def summarize(stats_csv, fetch_csv, write_csv, out_md, out_json, cmd, calib_byte
                 if np.mean(hi) > 3 * np.mean(lo):
                     note = f" (split from {len(lo)} same-grid dispatches of a coarser level, avg {np.mean(lo):.2f} us)"
                     d0 = hi
-            lines.append(f"| {k}@L0 | {kn} | {len(d0)} | {np.mean(d0):.2f}{note} |")
+            label = f"{k}@L0" if gmax == kmax[k] else f"{k} (coarser levels only)"
+            lines.append(f"| {label} | {kn} | {len(d0)} | {np.mean(d0):.2f}{note} |")
 
     def counters(fn, cname):
         per = collections.defaultdict(list)
@@ -304,7 +310,8 @@ def summarize(stats_csv, fetch_csv, write_csv, out_md, out_json, cmd, calib_byte
         wv = max(write.get((k, kn), [0.0])) * 1024 * w_scale
         traffic[k].append(fv + wv)
         lines.append(f"| {kn.split('(')[0].replace('void ', '')} | {fv / 1e6:.1f} | {wv / 1e6:.1f} | {(fv + wv) / 1e6:.1f} |")
-    tj = {f"{k}@L0": float(np.mean(v)) for k, v in traffic.items()}
+    # the level-0 launch of a key moves the most bytes of all its kernels' launches
+    tj = {f"{k}@L0": float(np.max(v)) for k, v in traffic.items()}
     with open(out_md, "w") as fh:
         fh.write("\n".join(lines) + "\n")
     with open(out_json, "w") as fh:

@@ -61,6 +61,40 @@ def test_summary_splits_levels_sharing_the_largest_grid(tmp_path):
     assert "gs_half@L0" in md2.read_text() and "HBM traffic" not in md2.read_text()
 
 
+def test_summary_level0_over_all_instances_of_a_key(tmp_path):
+    """Two instances of one key (the prolongation with and without the folded coarse increment):
+    the one on the largest grid is level 0, the other is listed as coarser-only, and the key's
+    traffic is the larger one, not their mean."""
+    hot = "void wl::k_prolong_inc<float, true>(wl::Geo, wl::Geo, wl::ZT, int, int)"
+    cold = "void wl::k_prolong_inc<float, false>(wl::Geo, wl::Geo, wl::ZT, int, int)"
+    stats = tmp_path / "y_kernel_stats.csv"
+    trace = tmp_path / "y_kernel_trace.csv"
+    _write(stats, [dict(Name=hot, Calls=2, AverageNs=30000, TotalDurationNs=60000, Percentage=90),
+                   dict(Name=cold, Calls=2, AverageNs=4000, TotalDurationNs=8000, Percentage=10)],
+           ["Name", "Calls", "AverageNs", "TotalDurationNs", "Percentage"])
+    rows = [dict(Kernel_Name=hot, Grid_Size_X=262144, Grid_Size_Y=4, Grid_Size_Z=1, Start_Timestamp=0,
+                 End_Timestamp=30000) for _ in range(2)]
+    rows += [dict(Kernel_Name=cold, Grid_Size_X=4096, Grid_Size_Y=4, Grid_Size_Z=1, Start_Timestamp=0,
+                  End_Timestamp=4000) for _ in range(2)]
+    _write(trace, rows, ["Kernel_Name", "Grid_Size_X", "Grid_Size_Y", "Grid_Size_Z", "Start_Timestamp",
+                         "End_Timestamp"])
+    fields = ["Kernel_Name", "Counter_Name", "Counter_Value"]
+    fetch, write = tmp_path / "f.csv", tmp_path / "w.csv"
+    _write(fetch, [dict(Kernel_Name="__amd_rocclr_copyBuffer", Counter_Name="FETCH_SIZE", Counter_Value=524288),
+                   dict(Kernel_Name=hot, Counter_Name="FETCH_SIZE", Counter_Value=80000),
+                   dict(Kernel_Name=cold, Counter_Name="FETCH_SIZE", Counter_Value=2000)], fields)
+    _write(write, [dict(Kernel_Name="__amd_rocclr_copyBuffer", Counter_Name="WRITE_SIZE", Counter_Value=1048576),
+                   dict(Kernel_Name=hot, Counter_Name="WRITE_SIZE", Counter_Value=40000),
+                   dict(Kernel_Name=cold, Counter_Name="WRITE_SIZE", Counter_Value=1000)], fields)
+    md, js = tmp_path / "s.md", tmp_path / "t.json"
+    bench.summarize(str(stats), str(fetch), str(write), str(md), str(js), "cmd")
+    text = md.read_text()
+    assert "| prolong_inc@L0 | wl::k_prolong_inc<float, true> | 2 | 30.00 |" in text
+    assert "| prolong_inc (coarser levels only) | wl::k_prolong_inc<float, false> |" in text
+    t = json.loads(js.read_text())
+    assert t["prolong_inc@L0"] == pytest.approx((80000 * 2.0 + 40000 * 1.0) * 1024)
+
+
 def test_initial_network_weights():
     wl = load_package()
     w = wl.init_weights(3)
