@@ -1,6 +1,6 @@
 """GPU parity: the gfx950 path through the C ABI against the oracle on the same inputs.
 
-Bar: results within relative L2 1e-5 of the Float32 oracle after a step,
+Bar: results within relative L2 1e-5 of the Float32 oracle after a step (the required bound),
 identical V-cycle counts; element-wise kernels are additionally checked for bit equality where
 both sides evaluate the same IEEE operations in the same order.
 """

@@ -90,7 +90,7 @@ def test_trained_network_on_held_out_flow(wl):
     a held-out flow (seed 9): on the pairs that flow records, its relative loss is well below 1,
     the loss of the reference's warm start alone (zero increment), i.e. it predicts most of the
     level-1 pressure increment; with its guess on, every projection still iterates to Σr² < 1e-4
-    (or itmx), so the solver stays the judge of the result."""
+    (or itmx), so the solver still decides when the result is converged."""
     import os
     say("case: trained warm-start network, held-out random-ellipsoid flow 256x128x128")
     path = os.path.join(os.path.dirname(os.path.dirname(__file__)), "waterlily.jl_amd", "data", "warmnet_c4.npz")
