@@ -1468,7 +1468,8 @@ __device__ __forceinline__ void k_mg_tail_lds_body(const TailDesc<T>& d, const T
   };
   auto inside = [](const SLev<T>& q, int* lo, int* hi) {
     lo[0] = lo[1] = 1; hi[0] = q.N0 - 2; hi[1] = q.N1 - 2;
-    if (q.D == 3) { lo[2] = 1; hi[2] = q.N2 - 2; } else { lo[2] = hi[2] = 0; }
+    lo[2] = q.D == 3 ? 1 : 0;  // selects, not branches: the bounds stay in registers
+    hi[2] = q.D == 3 ? q.N2 - 2 : 0;
   };
   for (int l = 0; l < d.n; ++l) {  // copy in
     const SLev<T> q = lev(l);
@@ -1502,8 +1503,8 @@ __device__ __forceinline__ void k_mg_tail_lds_body(const TailDesc<T>& d, const T
     tile_loop(lo, hi, bx, by, bz, [&](int i, int j, int k) {
       int c[3] = {i, j, k}, m[3] = {i, j, k};
       bool ghost = false;
-      for (int r = 0; r < q.D; ++r) {
-        if (!((d.perdir >> r) & 1)) continue;
+      for (int r = 0; r < 3; ++r) {
+        if (r >= q.D || !((d.perdir >> r) & 1)) continue;
         if (c[r] == 0) { m[r] = N[r] - 2; ghost = true; }
         else if (c[r] == N[r] - 1) { m[r] = 1; ghost = true; }
       }
@@ -1545,9 +1546,10 @@ __device__ __forceinline__ void k_mg_tail_lds_body(const TailDesc<T>& d, const T
     perbc(p, p.e);
     int lo[3], hi[3];
     inside(p, lo, hi);
-    int hh[3] = {hi[0], hi[1], hi[2]};
     const int Ld = p.D - 1;
-    hh[Ld] = (Ld == 2 ? p.N2 : p.N1) / 2 - 1;  // half_rangek
+    // half_rangek on the last axis; every index below is a constant or a select, so the
+    // per-cell coordinates stay in registers
+    const int hh[3] = {hi[0], Ld == 1 ? p.N1 / 2 - 1 : hi[1], Ld == 2 ? p.N2 / 2 - 1 : hi[2]};
     if (d.exact && l == d.n - 1 && p.n <= TAIL_CG_MAX && !d.perdir) {
       // Mixed precision: smooth! of the coarsest level in double: ϵ = r·iD and the four GS-RB
       // half sweeps (src/Poisson.jl:141-148) with ϵ, D and iD held in double, then ϵ rounded to T
@@ -1570,11 +1572,11 @@ __device__ __forceinline__ void k_mg_tail_lds_body(const TailDesc<T>& d, const T
           for (int k = lo[2]; k <= hh[2]; ++k)
             for (int j = lo[1]; j <= hh[1]; ++j)
               for (int i = lo[0]; i <= hh[0]; ++i) {
-                int q[3] = {i, j, k};
-                int front = 0;
-                for (int dd = 0; dd < Ld; ++dd) front += q[dd] + 1;
-                q[Ld] = 2 * (q[Ld] + 1) - 1 - (front + it) % 2 - 1;
-                const int I = q[0] + p.s1 * q[1] + p.s2 * q[2];
+                const int front = Ld == 2 ? (i + 1) + (j + 1) : i + 1;
+                const int par = (front + it) % 2;
+                const int jj = Ld == 1 ? 2 * (j + 1) - 1 - par - 1 : j;
+                const int kk = Ld == 2 ? 2 * (k + 1) - 1 - par - 1 : k;
+                const int I = i + p.s1 * jj + p.s2 * kk;
                 double sv = (double)p.r[I];
                 sv -= (E[I - 1] * (double)p.L[I] + E[I + 1] * (double)p.L[I + 1]);
                 sv -= (E[I - p.s1] * (double)p.L[p.n + I] + E[I + p.s1] * (double)p.L[p.n + I + p.s1]);
@@ -1593,11 +1595,11 @@ __device__ __forceinline__ void k_mg_tail_lds_body(const TailDesc<T>& d, const T
     } else
     for (int it = 1; it <= 4; ++it) {
       tile_loop(lo, hh, bx, by, bz, [&](int i, int j, int k) {
-        int q[3] = {i, j, k};
-        int front = 0;
-        for (int dd = 0; dd < Ld; ++dd) front += q[dd] + 1;
-        q[Ld] = 2 * (q[Ld] + 1) - 1 - (front + it) % 2 - 1;
-        const int I = q[0] + p.s1 * q[1] + p.s2 * q[2];
+        const int front = Ld == 2 ? (i + 1) + (j + 1) : i + 1;
+        const int par = (front + it) % 2;
+        const int jj = Ld == 1 ? 2 * (j + 1) - 1 - par - 1 : j;
+        const int kk = Ld == 2 ? 2 * (k + 1) - 1 - par - 1 : k;
+        const int I = i + p.s1 * jj + p.s2 * kk;
         T s = p.r[I];
         s -= (p.e[I - 1] * p.L[I] + p.e[I + 1] * p.L[I + 1]);
         s -= (p.e[I - p.s1] * p.L[p.n + I] + p.e[I + p.s1] * p.L[p.n + I + p.s1]);
@@ -1621,8 +1623,9 @@ __device__ __forceinline__ void k_mg_tail_lds_body(const TailDesc<T>& d, const T
       tile_loop(flo, fhi, bx, by, bz, [&](int i, int j, int k) {
         const int c[3] = {i, j, k};
         int q[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
-        for (int dd = 0; dd < f.D; ++dd)
-          for (int o = 0; o < 3; ++o) q[dd][o] = pro_map(c[dd] + o - 1, N[dd], (d.perdir >> dd) & 1, (f.cm >> dd) & 1);
+        for (int dd = 0; dd < 3; ++dd)
+          for (int o = 0; o < 3; ++o)
+            if (dd < f.D) q[dd][o] = pro_map(c[dd] + o - 1, N[dd], (d.perdir >> dd) & 1, (f.cm >> dd) & 1);
         auto xcv = [&](int a, int b, int z) -> T { return (a < 0 || b < 0 || z < 0) ? T(0) : p.x[a + p.s1 * b + p.s2 * z]; };
         const int I = i + f.s1 * j + f.s2 * k;
         const T e0 = xcv(q[0][1], q[1][1], q[2][1]);
