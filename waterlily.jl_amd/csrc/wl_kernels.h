@@ -1516,6 +1516,17 @@ __device__ __forceinline__ void k_mg_tail_lds_body(const TailDesc<T>& d, const T
     const SLev<T> f = lev(l), c = lev(l + 1);
     perbc(f, f.e);
     int lo[3], hi[3];
+    // Jacobi! on every fine inside cell at once: a cell reads ϵ (not written here) and updates only
+    // its own r and x.  The children of the coarse inside cells are exactly the fine inside cells
+    // (a coarsened direction has an even count of fine inside cells, twice the coarse count).
+    inside(f, lo, hi);
+    tile_loop(lo, hi, bx, by, bz, [&](int i, int j, int k) {
+      const int J = i + f.s1 * j + f.s2 * k;
+      f.r[J] = f.r[J] - T(1) * smult(f, f.e, J);
+      f.x[J] = f.x[J] + T(1) * f.e[J];
+    });
+    __syncthreads();
+    // restrict!: each coarse cell sums its children's new r in the fixed z, y, x order
     inside(c, lo, hi);
     tile_loop(lo, hi, bx, by, bz, [&](int i, int j, int k) {
       const int cc[3] = {i, j, k};
@@ -1526,13 +1537,7 @@ __device__ __forceinline__ void k_mg_tail_lds_body(const TailDesc<T>& d, const T
       T s = 0;
       for (int kk = a0[2]; kk <= a1[2]; ++kk)
         for (int jj = a0[1]; jj <= a1[1]; ++jj)
-          for (int ii = a0[0]; ii <= a1[0]; ++ii) {
-            const int J = ii + f.s1 * jj + f.s2 * kk;
-            const T rn = f.r[J] - T(1) * smult(f, f.e, J);
-            f.r[J] = rn;
-            f.x[J] = f.x[J] + T(1) * f.e[J];
-            s += rn;
-          }
+          for (int ii = a0[0]; ii <= a1[0]; ++ii) s += f.r[ii + f.s1 * jj + f.s2 * kk];
       const int Ic = i + c.s1 * j + c.s2 * k;
       c.r[Ic] = s;
       c.x[Ic] = 0;
