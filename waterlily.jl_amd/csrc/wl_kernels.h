@@ -1436,7 +1436,8 @@ struct SLev {
 template <class F>
 __device__ __forceinline__ void tile_loop(const int* lo, const int* hi, int bx, int by, int bz, F&& f) {
   const int t = threadIdx.x;
-  const int tx = t % bx, ty = (t / bx) % by, tz = t / (bx * by);
+  const int sx = __builtin_ctz(bx), sy = __builtin_ctz(by);  // bx, by uniform: scalar ops
+  const int tx = t & (bx - 1), ty = (t >> sx) & (by - 1), tz = t >> (sx + sy);
   for (int k = lo[2] + tz; k <= hi[2]; k += bz)
     for (int j = lo[1] + ty; j <= hi[1]; j += by)
       for (int i = lo[0] + tx; i <= hi[0]; i += bx) f(i, j, k);
