@@ -275,6 +275,8 @@ def summarize(stats_csv, fetch_csv, write_csv, out_md, out_json, cmd, calib_byte
                     note = f" (split from {len(lo)} same-grid dispatches of a coarser level, avg {np.mean(lo):.2f} us)"
                     d0 = hi
             label = f"{k}@L0" if gmax == kmax[k] else f"{k} (coarser levels only)"
+            if k == "mg_tail":  # one workgroup that runs the coarse levels, never level 0
+                label = "mg_tail (coarse levels, one workgroup)"
             lines.append(f"| {label} | {kn} | {len(d0)} | {np.mean(d0):.2f}{note} |")
 
     def counters(fn, cname):
