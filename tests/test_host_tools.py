@@ -95,6 +95,24 @@ def test_summary_level0_over_all_instances_of_a_key(tmp_path):
     assert t["prolong_inc@L0"] == pytest.approx((80000 * 2.0 + 40000 * 1.0) * 1024)
 
 
+def test_summary_labels_the_coarse_tail(tmp_path):
+    """The coarse tail is one workgroup: its only grid is its largest, yet it never runs level 0,
+    so the kernel-trace table names it as the coarse-level kernel."""
+    tail = "void wl::k_mg_tail_lds<float>(wl::TailDesc<float>, float const*, int, int, int)"
+    stats = tmp_path / "z_kernel_stats.csv"
+    trace = tmp_path / "z_kernel_trace.csv"
+    _write(stats, [dict(Name=tail, Calls=3, AverageNs=30000, TotalDurationNs=90000, Percentage=100)],
+           ["Name", "Calls", "AverageNs", "TotalDurationNs", "Percentage"])
+    _write(trace, [dict(Kernel_Name=tail, Grid_Size_X=1024, Grid_Size_Y=1, Grid_Size_Z=1, Start_Timestamp=0,
+                        End_Timestamp=30000) for _ in range(3)],
+           ["Kernel_Name", "Grid_Size_X", "Grid_Size_Y", "Grid_Size_Z", "Start_Timestamp", "End_Timestamp"])
+    md = tmp_path / "s.md"
+    bench.summarize(str(stats), "-", "-", str(md), "-", "cmd")
+    text = md.read_text()
+    assert "| mg_tail (coarse levels, one workgroup) | wl::k_mg_tail_lds<float> | 3 | 30.00 |" in text
+    assert "mg_tail@L0" not in text
+
+
 def test_initial_network_weights():
     wl = load_package()
     w = wl.init_weights(3)
