@@ -64,21 +64,35 @@ def run_gpu(wl, n, R, center, Re, L, steps, warmup, profile_steps=None, mode=Non
                 profiled_ms_per_step=None if psec is None else 1e3 * psec)
 
 
-def c5_members(wl, n=64, seed0=0):
+def c5_specs(n=64, seed0=0):
     """64 uneven 2D flows (seeds 0..63): grid from {64x64, 128x64, 256x128, 512x256}, one circle or
     ellipse (radius U[m/16, m/6], m the smaller side, centred at (N0/4, N1/2)), Re U[100, 1000] on
-    the diameter (nu = 2R/Re), U = (1, 0), Float32."""
+    the diameter (nu = 2R/Re), U = (1, 0), Float32.  Each spec: grid, body (kind 1 circle with
+    radius, or kind 2 ellipse with axes), R and nu."""
     sizes = [(64, 64), (128, 64), (256, 128), (512, 256)]
-    sims = []
+    specs = []
     for k in range(n):
         rng = np.random.default_rng(seed0 + k)
         N = sizes[int(rng.integers(0, 4))]
         m = min(N)
         R = float(rng.uniform(m / 16, m / 6))
         c = (N[0] / 4, N[1] / 2)
-        body = wl.Sphere(c, R) if rng.uniform() < 0.5 else wl.Ellipsoid(c, (R, float(rng.uniform(0.4, 1.0)) * R))
+        if rng.uniform() < 0.5:
+            body = {"kind": 1, "center": c, "radius": R}
+        else:
+            body = {"kind": 2, "center": c, "axes": (R, float(rng.uniform(0.4, 1.0)) * R)}
         Re = float(rng.uniform(100, 1000))
-        sims.append(wl.Simulation(N, (1, 0), 2 * R, ν=2 * R / Re, body=body, T=np.float32))
+        specs.append(dict(N=N, body=body, R=R, nu=2 * R / Re))
+    return specs
+
+
+def c5_members(wl, n=64, seed0=0):
+    """The flows of c5_specs as Simulations."""
+    sims = []
+    for sp in c5_specs(n, seed0):
+        b = sp["body"]
+        body = wl.Sphere(b["center"], b["radius"]) if b["kind"] == 1 else wl.Ellipsoid(b["center"], b["axes"])
+        sims.append(wl.Simulation(sp["N"], (1, 0), 2 * sp["R"], ν=sp["nu"], body=body, T=np.float32))
     return sims
 
 

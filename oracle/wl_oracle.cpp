@@ -11,7 +11,7 @@
 namespace wlo {
 
 // ---------------------------------------------------------------------------------------------
-// Boundary conditions: src/core.jl:192-243.  Pinned by tests/test_oracle_kat.py (test/test_core.jl:100-151).
+// Boundary conditions: src/core.jl:192-243.  Pinned by tests/test_oracle_kat.py (test/test_core.jl:19-69).
 // uBC is uniform in space, so the tangential Neumann write uBC(I)+a[I±δ]-uBC(I±δ) is the copy
 // a[I±δ] (the @fastmath fold of (U+a)-U with the same U on both sides).
 // ---------------------------------------------------------------------------------------------

@@ -189,8 +189,6 @@ def test_c3_sphere_steps_bitwise(wl):
     assert np.max(np.abs(F - Fo)) <= 1e-9 * np.max(np.abs(Fo))
 
 
-@pytest.mark.skipif(os.environ.get("WL_LONG_PARITY") != "1",
-                    reason="about 7 minutes of oracle time: set WL_LONG_PARITY=1 to run it")
 def test_c2_sphere_1000_steps_forces(wl):
     """The 3D sphere (256x128x128, Re 3700, Float32) for 1000 steps: pressure_force at every step
     within 1e-3 (relative to the largest force of the run) of the Float32 oracle and every V-cycle
@@ -354,6 +352,85 @@ def test_mixed_precision_solver(wl):
     say(f"  vs F64 oracle: mixed u {um:.3e} p {pm:.3e} force {worst_m:.3e} n {mix.pois.n[-4:]}; "
         f"plain u {up:.3e} p {pp:.3e} force {worst_p:.3e} n {pl.pois.n[-4:]}")
     assert um < 1e-4 and pm < 1e-4 and worst_m < 1e-3
+
+
+def test_c3_mixed_precision_vs_f64_oracle(wl):
+    """The mixed-precision configuration at its own size: 3D sphere 512x256x256 (r = 32, Re 1e4,
+    FP32 stencils and V-cycles in an FP64 defect-correction loop) for three steps against the
+    Float64 oracle: u and p within 1e-4 (relative L2), the pressure force within 1e-3, and every
+    projection ending with Σr² < 1e-4 on the FP64 residual; the plain Float32 path's deviation is
+    reported beside it."""
+    say("case: mixed precision 512x256x256, 3 steps vs Float64 oracle")
+    n, nu, c, R = (512, 256, 256), 64 / 1e4, (128, 128, 128), 32
+    mix = wl.Simulation(n, (1, 0, 0), 2 * R, ν=nu, body=wl.Sphere(c, R), T=np.float32)
+    mix.pois.set_solver(mode=wl.WL_SOLVER_MIXED)
+    o64 = O.OSimulation(n, (1, 0, 0), 2 * R, nu=nu, dtype=O.WL_F64,
+                        bodies=[{"kind": 1, "center": c, "radius": R}])
+    for k in range(3):
+        wl.sim_step(mix, remeasure=False)
+        o64.step(remeasure=False)
+        # the history holds Σr² after residual! and after each iteration, projection by projection
+        r2, n1 = mix.flow.residuals(), mix.pois.n[-2]
+        ends = (r2[n1], r2[-1])
+        say(f"  step {k + 1}: n {mix.pois.n[-2:]} (F64 oracle {o64.n[-2:]}), final Σr² {ends}")
+        assert len(r2) == sum(mix.pois.n[-2:]) + 2 and max(ends) < 1e-4
+    um, pm = rel(mix.flow.u, o64.flow.get("u")), rel(mix.flow.p, o64.flow.get("p"))
+    F, Fo = wl.pressure_force(mix), o64.pressure_force()
+    ef = float(np.max(np.abs(F - Fo)) / max(np.max(np.abs(Fo)), 1e-12))
+    say(f"  vs F64 oracle: u {um:.3e} p {pm:.3e} force {ef:.3e} (F {F}, oracle {Fo})")
+    assert um <= 1e-4 and pm <= 1e-4 and ef <= 1e-3
+
+
+def test_c5_batch_members_vs_oracle(wl):
+    """The batched configuration as the bench builds it (64 uneven 2D flows, seeds 0..63, grids
+    64x64..512x256, circles and ellipses, Re 100-1000): two batch steps, every member bit for bit
+    against its own Float32 oracle run (u, p, Δt and both V-cycle counts of each step)."""
+    import bench
+    say("case: 64-member batch, 2 steps, each member vs the Float32 oracle")
+    specs = bench.c5_specs()
+    sims = bench.c5_members(wl)
+    b = wl.Batch(sims)
+    for _ in range(2):
+        b.step()
+    bad = []
+    for k, (sp, s) in enumerate(zip(specs, sims)):
+        o = O.OSimulation(sp["N"], (1, 0), 2 * sp["R"], nu=sp["nu"], bodies=[sp["body"]])
+        for _ in range(2):
+            o.step(remeasure=False)
+        ok = (np.array_equal(s.flow.u, o.flow.get("u")) and np.array_equal(s.flow.p, o.flow.get("p"))
+              and s.flow.Δt == o.flow.dt and s.pois.n == o.n)
+        if not ok:
+            bad.append((k, sp["N"], rel(s.flow.u, o.flow.get("u")), s.pois.n, o.n))
+    say(f"  {len(sims) - len(bad)}/{len(sims)} members bit-identical; mismatches: {bad[:4]}")
+    assert not bad
+
+
+@pytest.mark.parametrize("tail", ["global", "none"])
+def test_coarse_tail_variants(wl, monkeypatch, tail):
+    """The coarse end of the V-cycle three ways: in LDS (default), the same passes on the global
+    arrays (WL_TAIL_CELLS = the first tail level's cell count) and no tail at all (WL_TAIL_CELLS=0,
+    one launch per pass): four steps of a 3D sphere, each bit for bit against the Float32 oracle."""
+    n, c, R, nu = (64, 32, 32), (20, 16, 16), 6, 12 / 400
+    monkeypatch.setenv("WL_TAIL_CELLS", "1200" if tail == "global" else "0")
+    say(f"case: coarse tail {tail}, 3D 64x32x32, 4 steps")
+    sim, osim = oracle_pair(wl, n, (1, 0, 0), R, c, nu)
+    monkeypatch.delenv("WL_TAIL_CELLS")
+    lds, _ = oracle_pair(wl, n, (1, 0, 0), R, c, nu)
+    for name in ("mu0", "mu1", "V"):
+        sim.flow.set(name, osim.flow.get(name))
+        lds.flow.set(name, osim.flow.get(name))
+    sim.pois.update()
+    lds.pois.update()
+    for k in range(4):
+        wl.sim_step(sim, remeasure=False)
+        wl.sim_step(lds, remeasure=False)
+        osim.step(remeasure=False)
+        for name in ("u", "p"):
+            a = sim.flow.get(name)
+            assert np.array_equal(a, lds.flow.get(name)), (k, name)
+            assert np.array_equal(a, osim.flow.get(name)), (k, name)
+    assert sim.pois.n == osim.n == lds.pois.n
+    say("  n =", sim.pois.n)
 
 
 def test_impulsive_box(wl):

@@ -79,7 +79,7 @@ def test_kernel_moments():
         assert kern(2, d) == pytest.approx(-np.trapezoid(x * k, x), abs=1e-6)
 
 
-# ---- test/test_core.jl:100-151: boundary conditions ----
+# ---- test/test_core.jl:19-69: boundary conditions ----
 def test_bc_dirichlet_neumann():
     rng = np.random.default_rng(1)
     U = (1.0, 0.5)

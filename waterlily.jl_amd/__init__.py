@@ -7,7 +7,7 @@ The package directory is named `waterlily.jl_amd`; load it with `load_package()`
 from ._abi import LIB_PATH, WL_SOLVER_MIXED, WL_SOLVER_PLAIN, WaterLilyError, load as load_library  # noqa: F401
 from .api import (  # noqa: F401
     AbstractBody, Batch, BodyUnion, Circle, Ellipsoid, Flow, FlowPoisson, L2, MeanFlow, MultiLevelPoisson, NoBody, Poisson,
-    Simulation, Sphere, cds, coarsen_mask, copy, measure, measure_flow, meanflow_update, mom_step, mult,
+    Simulation, Sphere, cds, coarsen_mask, copy, ic_kwarg, measure, measure_flow, meanflow_update, mom_step, mult,
     pressure_force, pressure_moment, quick, reset, sim_step, sim_time, solver, time, total_force, total_moment,
     update, uu, vanLeer, viscous_force, viscous_moment,
 )

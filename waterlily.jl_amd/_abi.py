@@ -125,5 +125,7 @@ def check(rc):
         msg = load().wl_last_error().decode(errors="replace")
         if rc == -1:
             raise AssertionError(msg)
+        if rc == WL_ENAN:
+            raise FloatingPointError(msg)
         raise WaterLilyError(rc, msg)
     return rc

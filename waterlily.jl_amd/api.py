@@ -113,6 +113,28 @@ def _loc_coords(Ng, i, T):
     return [np.asarray(idx[d] - 1.5 - (0.5 if d == i - 1 else 0.0), dtype=T) for d in range(len(Ng))]
 
 
+def _ic_field(u0, Ng, T):
+    """The initial condition on every cell: apply!(u0, u) for a function u0(i, x) (x: the
+    coordinate arrays of loc(i,I)), or u0[i] per component for a tuple (src/Flow.jl:140)."""
+    D = len(Ng)
+    out = np.zeros(tuple(Ng) + (D,), dtype=T, order="F")
+    for i in range(1, D + 1):
+        v = u0(i, _loc_coords(Ng, i, T)) if callable(u0) else u0[i - 1]
+        out[..., i - 1] = np.broadcast_to(np.asarray(v, dtype=T), Ng)
+    return out
+
+
+def ic_kwarg(u0, uλ):
+    """Resolve the initial-condition keyword (src/Flow.jl:97-101): `uλ` is the deprecated name of
+    `u0`; it warns and is used only when `u0` is not given."""
+    if uλ is None:
+        return u0
+    import warnings
+    warnings.warn("The `uλ` keyword argument is deprecated and will be removed in a future release. "
+                  "Use `u0` instead.", DeprecationWarning, stacklevel=3)
+    return uλ if u0 is None else u0
+
+
 def _fn_field(fns, Ng, T, t):
     """Σ of the functions fns(i, x, t) at loc(i,I) on every cell, in the layout of u (apply!,
     src/util.jl); each function takes coordinate arrays and returns an array (or scalar) of T."""
@@ -129,8 +151,10 @@ def _fn_field(fns, Ng, T, t):
 
 
 class Flow:
-    """Flow(N, uBC; Δt=0.25, ν=0, g=None, λ=quick, T=float32, perdir=(), exitBC=False).
+    """Flow(N, uBC; Δt=0.25, ν=0, g=None, u0=None, λ=quick, T=float32, perdir=(), exitBC=False).
 
+    u0: the initial velocity, a function u0(i, x) or a tuple (default: uBC at t = 0); `uλ` is its
+    deprecated alias (src/Flow.jl:133-142).
     uBC: a tuple (constant); a pair (U, dU/dt) for a boundary velocity uniform in space that varies
     linearly in time; or a function uBC(i, x, t) (x: the coordinate arrays of the cells' loc(i,I))
     evaluated on the host every step, with its time derivative `uBC_dt(i, x, t)` when it depends
@@ -140,7 +164,8 @@ class Flow:
     """
 
     def __init__(self, N, uBC, Δt=0.25, ν=0.0, g=None, λ=quick, T=np.float32, perdir=(), exitBC=False,
-                 dt=None, nu=None, uBC_dt=None):
+                 dt=None, nu=None, uBC_dt=None, u0=None, uλ=None):
+        u0 = ic_kwarg(u0, uλ)
         if dt is not None:
             Δt = dt
         if nu is not None:
@@ -187,10 +212,12 @@ class Flow:
         self.h = C.c_void_p()
         A.check(A.load().wl_flow_create(C.byref(self.h), C.byref(d)))
         self.pois = FlowPoisson(self)
-        if self.uBC_fn is not None:  # u = uBC(i,x,0) everywhere, then BC! and exitBC! (src/Flow.jl:138-141)
-            ic = _fn_field([self.uBC_fn], self.Ng, self.T, 0.0)
-            self.set("u", ic)
-            self.set_field_fn(A.WL_FN_UBC, ic)
+        if self.uBC_fn is not None or u0 is not None:
+            # u = u0 (default uBC(i,x,0)) everywhere, then BC! and exitBC! (src/Flow.jl:138-141)
+            ubc0 = _fn_field([self.uBC_fn], self.Ng, self.T, 0.0) if self.uBC_fn is not None else None
+            self.set("u", _ic_field(u0, self.Ng, self.T) if u0 is not None else ubc0)
+            if ubc0 is not None:
+                self.set_field_fn(A.WL_FN_UBC, ubc0)
             A.check(A.load().wl_flow_apply_ic(self.h))
 
     def set_field_fn(self, which, a):
@@ -439,10 +466,11 @@ def solver(pois, tol=1e-4, itmx=None):
 # Simulation (src/WaterLily.jl:86-161)
 # ---------------------------------------------------------------------------------------------
 class Simulation:
-    """Simulation(dims, uBC, L; U, Δt, ν, ϵ, g, perdir, exitBC, λ, body, T)."""
+    """Simulation(dims, uBC, L; U, Δt, ν, ϵ, g, perdir, u0, exitBC, λ, body, T)."""
 
     def __init__(self, dims, uBC, L, U=None, Δt=0.25, ν=0.0, ϵ=1, g=None, perdir=(), exitBC=False, λ=quick,
-                 body=None, T=np.float32, dt=None, nu=None, eps=None, uBC_dt=None):
+                 body=None, T=np.float32, dt=None, nu=None, eps=None, uBC_dt=None, u0=None, uλ=None):
+        u0 = ic_kwarg(u0, uλ)
         if dt is not None:
             Δt = dt
         if nu is not None:
@@ -456,7 +484,8 @@ class Simulation:
             U = math.sqrt(sum(float(v) ** 2 for v in uBC))
         self.U, self.L, self.ϵ = U, L, ϵ
         self.body = body if body is not None else NoBody()
-        self.flow = Flow(dims, uBC, Δt=Δt, ν=ν, g=g, λ=λ, T=T, perdir=perdir, exitBC=exitBC, uBC_dt=uBC_dt)
+        self.flow = Flow(dims, uBC, Δt=Δt, ν=ν, g=g, λ=λ, T=T, perdir=perdir, exitBC=exitBC, uBC_dt=uBC_dt,
+                         u0=u0)
         measure_flow(self.flow, self.body, t=0.0, ϵ=ϵ)
         self.pois = self.flow.pois
         self.pois.update()
@@ -523,12 +552,15 @@ class Batch:
         out = (C.c_double * n)()
         nit = (C.c_int16 * (2 * n))()
         rc = A.load().wl_batch_step(self.h, dt, t0, t1, out, nit)
-        if rc == A.WL_ENAN:
-            raise FloatingPointError(A.load().wl_last_error().decode())
-        A.check(rc)
+        if rc not in (0, A.WL_ENAN):
+            A.check(rc)
+        # every member took its step: the healthy ones keep their results, a member with a NaN or
+        # Inf in u records a NaN Δt (and the call raises)
         for k, s in enumerate(self.sims):
             s.flow.Δt.append(float(s.flow.T(out[k])))
             s.pois.n.extend([int(nit[2 * k]), int(nit[2 * k + 1])])
+        if rc == A.WL_ENAN:
+            raise FloatingPointError(A.load().wl_last_error().decode())
         return self
 
     def __del__(self):

@@ -1698,12 +1698,17 @@ __global__ void __launch_bounds__(256) k_correct(Geo g, ZT t, T* __restrict__ u,
 }
 
 // CFL (src/Flow.jl:234-244): σ = flux_out(u) on inside; block maxima of σ over every cell.
+// Failure detection (no counterpart in the reference, which would return a garbage Δt): a
+// non-finite velocity face read on inside, or a non-finite σ anywhere, makes the block maximum
+// +Inf, so the host sees a non-finite maximum (max(0,NaN) would otherwise drop a NaN face, and a
+// NaN σ loses every comparison of the max).
 template <class T>
 __device__ __forceinline__ void k_cfl_body(const Geo& g, const ZT& t, const T* __restrict__ u, T* __restrict__ sigma,
                                              T* __restrict__ partial, int lid, int pidx) {
   int c[3], z0, z1;
   zt_cell_id(t, lid, c[0], c[1], z0, z1);
   T m = -T(INFINITY);
+  bool bad = false;
   if (c[0] <= t.b.hi[0] && c[1] <= t.b.hi[1]) {
     for (c[2] = z0; c[2] <= z1; ++c[2]) {
       const int64_t I = at(g, c[0], c[1], c[2]);
@@ -1727,12 +1732,17 @@ __device__ __forceinline__ void k_cfl_body(const Geo& g, const ZT& t, const T* _
         s = g.D == 3 ? s3 : s;
         sigma[I] = s;
         v = s;
+        // a NaN or ±Inf among the faces read makes this sum non-finite (on a 2D level the third
+        // pair repeats component 1)
+        bad |= !isfinite(((up[0] - uc[0]) + (up[1] - uc[1])) + (up[2] - uc[2]));
       } else {
         v = sigma[I];
       }
+      bad |= !isfinite(v);
       m = v > m ? v : m;
     }
   }
+  if (bad) m = T(INFINITY);
   m = block_max<T>(m, -T(INFINITY));
   if (threadIdx.x == 0 && threadIdx.y == 0) partial[pidx] = m;
 }

@@ -12,6 +12,7 @@
 #include <memory>
 #include <algorithm>
 #include <cstdlib>
+#include <limits>
 #include "wl_kernels.h"
 
 using namespace wl;
@@ -220,6 +221,7 @@ struct MG {
 
   double* x64[2] = {nullptr, nullptr};  // FP64 solutions of the mixed-precision loop (ping-pong)
   int x64cur = 0;
+  bool nan_seen = false;  // a solve of the current step met a non-finite Σr² (cleared per step)
   ~MG() {
     free_graphs();
     dfree(x64[0]);
@@ -242,6 +244,8 @@ struct MG {
     {
       const char* e = std::getenv("WL_GRAPHS");
       use_graphs = e && std::atoi(e) != 0;
+      const char* t = std::getenv("WL_TAIL_CELLS");
+      tail_cells = t ? (int64_t)std::atoll(t) : (int64_t)-1;
     }
     Level<T> l0;
     l0.g = make_geo(D, Ng);
@@ -504,11 +508,9 @@ struct MG {
   // First level run by the single-workgroup coarse tail: the first level (>= 1) from which every
   // coarser level fits the tail's LDS image; WL_TAIL_CELLS (cells of the first tail level) makes
   // it a global-memory tail from that size instead.  -1: no tail.
+  int64_t tail_cells = -1;  // WL_TAIL_CELLS when the solver was built (-1: unset)
   int tail_start(bool* in_lds) const {
-    static const int64_t cells = [] {
-      const char* e = std::getenv("WL_TAIL_CELLS");
-      return e ? (int64_t)std::atoll(e) : (int64_t)-1;
-    }();
+    const int64_t cells = tail_cells;
     *in_lds = false;
     if (cells == 0) return -1;
     if (cells > 0) {
@@ -745,6 +747,10 @@ struct MG {
       }
       n++;
       if ((double)rnew < tol) break;
+      if (!std::isfinite(rnew)) {  // NaN/Inf reached Σr²: further V-cycles cannot recover it
+        nan_seen = true;
+        break;
+      }
     }
     perbc(lv[0].g, lv[0].x);
     return n;
@@ -815,6 +821,10 @@ struct MG {
       if (hist) hist->push_back((double)rnew);
       n++;
       if ((double)rnew < tol) break;
+      if (!std::isfinite(rnew)) {
+        nan_seen = true;
+        break;
+      }
     }
     return n;
   }
@@ -1311,6 +1321,8 @@ struct FlowT : FlowBase {
     HIPCK(hipMemcpyAsync(B.htsc, (T*)B.tsc + 2, sizeof(T), hipMemcpyDeviceToHost, st));
     HIPCK(hipStreamSynchronize(st));
     const T m = ((T*)B.htsc)[0];
+    // a non-finite maximum means a NaN or Inf in u (the kernel turns one into +Inf): no Δt
+    if (!std::isfinite(m)) return std::numeric_limits<double>::quiet_NaN();
     const T inv = T(1) / (m + T(5) * (T)desc.nu);
     return (double)(std::isnan(inv) ? inv : std::min(T(10), inv));
   }
@@ -1318,6 +1330,7 @@ struct FlowT : FlowBase {
     if (mu1_dirty) mu1_rows();
     mg.ensure_lrows();
     r2hist.clear();
+    mg.nan_seen = false;
     const T dt = (T)dt_d;
     momentum(true, dt, t0, t1);
     const int n1 = project(T(1), dt, t1);
@@ -1327,6 +1340,7 @@ struct FlowT : FlowBase {
     *dt_next = (double)dtn;
     if (nit) { nit[0] = (int16_t)n1; nit[1] = (int16_t)n2; }
     if (!std::isfinite(dtn)) return set_err(WL_ENAN, "non-finite CFL time step (NaN/Inf in u)");
+    if (mg.nan_seen) return set_err(WL_ENAN, "non-finite Poisson residual (NaN/Inf in the pressure solve)");
     return 0;
   }
 
@@ -1742,7 +1756,9 @@ struct BatchT : BatchBase {
       const T m = hcfl[s];
       const T nu = (T)F[s]->desc.nu;
       const T inv = T(1) / (m + T(5) * nu);
-      const T dtn = std::isnan(inv) ? inv : std::min(T(10), inv);
+      // a non-finite maximum means a NaN or Inf in the member's u (kb_cfl makes one +Inf)
+      const T dtn = !std::isfinite(m) ? std::numeric_limits<T>::quiet_NaN()
+                                       : (std::isnan(inv) ? inv : std::min(T(10), inv));
       dt_next[s] = (double)dtn;
       nit[2 * s] = (int16_t)n1[s];
       nit[2 * s + 1] = (int16_t)n2[s];
@@ -2260,6 +2276,7 @@ int wl_flow_cfl(wl_flow_t h, double* dt) {
   if (!h || !dt) throw WlErr(WL_EINVAL, "null argument");
   h->impl->sync();
   *dt = h->impl->cfl();
+  if (!std::isfinite(*dt)) throw WlErr(WL_ENAN, "non-finite CFL time step (NaN/Inf in u)");
   API_END
 }
 int wl_flow_apply_ic(wl_flow_t h) {
