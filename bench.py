@@ -2,9 +2,10 @@
 """Benchmark of the gfx950 WaterLily time step (one GPU).
 
 A step is one sim_step! (mom_step!: predictor, two pressure projections with the multigrid to
-Σr² < 1e-4, corrector, CFL) of the 3D flow past a sphere, 256x128x128 interior cells, Float32,
-U=(1,0,0), radius 16 at (64,64,64), Re=3700 (ν = U·32/Re), static body (remeasure=false, which
-gives the same fields as remeasuring a static body).  Inputs are resident in HBM before timing.
+Σr² < 1e-4, corrector, CFL) of the 3D flow past a sphere, 512x256x256 interior cells (the largest
+single-GPU configuration), Float32, U=(1,0,0), radius 32 at (128,128,128), Re=1e4 (ν = U·64/Re),
+static body (remeasure=false, which gives the same fields as remeasuring a static body), timed
+after warm-up steps past the impulsive start.  Inputs are resident in HBM before timing.
 
 Prints one JSON line: Mcell-steps/s (interior cells x steps / s), the per-kernel roofline of the
 dominant kernel from HIP events recorded on the library's stream during the timed steps, and the
@@ -160,7 +161,7 @@ def roofline(stats, traffic_file):
                 step_alg_GBs=round(total_b / (total_ms * 1e-3) / 1e9, 1) if total_ms > 0 else None), rows
 
 
-def run_cpu(n, R, center, Re, L, budget_s=15.0):
+def run_cpu(n, R, center, Re, L, budget_s=15.0, label="512x256x256"):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import wl_oracle as O
     threads = os.cpu_count() or 1
@@ -175,7 +176,7 @@ def run_cpu(n, R, center, Re, L, budget_s=15.0):
         if (el > budget_s and k >= 2) or k >= 200:
             break
     return dict(value=round(int(np.prod(n)) * k / el / 1e6, 3), unit="Mcell-steps/s", cores=threads, kind="port",
-                sample=f"{k} sim_step! of the same 256x128x128 sphere flow after 1 warm-up step "
+                sample=f"{k} sim_step! of the same {label} sphere flow after 1 warm-up step "
                        f"({el:.1f} s), oracle built -O3 -fopenmp on {threads} threads")
 
 
@@ -412,17 +413,18 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=100)
-    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=30)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-c3", action="store_true")
-    ap.add_argument("--c3-steps", type=int, default=20)
+    ap.add_argument("--no-c3", action="store_true", help="skip the mixed-precision 512x256x256 line")
+    ap.add_argument("--c3-steps", type=int, default=50)
+    ap.add_argument("--no-c2", action="store_true")
+    ap.add_argument("--c2-steps", type=int, default=100)
     ap.add_argument("--no-c5", action="store_true")
     ap.add_argument("--c5-steps", type=int, default=20)
-    ap.add_argument("--c3-main", action="store_true", help="time the 512x256x256 case as the main line")
+    ap.add_argument("--c2-main", action="store_true", help="time the 256x128x128 case as the main line")
     ap.add_argument("--tiles", help="GS,RP tile targets to set (main line)")
-    ap.add_argument("--autotune", action="store_true",
-                    help="autotune the main line's tiles (the 256x128x128 case gains nothing from it; "
-                         "the 512x256x256 section and the batch always tune)")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the default tile targets on the main line (it tunes them during set-up)")
     ap.add_argument("--no-c4", action="store_true")
     ap.add_argument("--c4-steps", type=int, default=30)
     ap.add_argument("--train-warmstart", metavar="OUT", help="train the warm-start network and save its weights")
@@ -448,13 +450,14 @@ def main():
     if a.train_warmstart:
         train_warmstart(wl, a.train_warmstart, a.train_seeds, a.train_steps)
         return
-    n, R, center, Re, L = (256, 128, 128), 16, (64, 64, 64), 3700.0, 32
-    traffic_file, label = "hbm_traffic.json", "3D flow past a sphere 256x128x128, Re=3700"
-    if a.c3_main:  # the 512x256x256 case as the main line (for profiling it alone)
-        n, R, center, Re, L = (512, 256, 256), 32, (128, 128, 128), 1e4, 64
-        traffic_file, label = "hbm_traffic_c3.json", "3D flow past a sphere 512x256x256, Re=1e4"
-        a.no_c3 = True
-    r = run_gpu(wl, n, R, center, Re, L, a.steps, a.warmup, tune=a.autotune or a.c3_main,
+    n, R, center, Re, L = (512, 256, 256), 32, (128, 128, 128), 1e4, 64
+    traffic_file, label = "hbm_traffic_c3.json", "3D flow past a sphere 512x256x256, Re=1e4"
+    n2, R2, c2, Re2, L2 = (256, 128, 128), 16, (64, 64, 64), 3700.0, 32
+    if a.c2_main:  # the 256x128x128 case as the main line (for profiling it alone)
+        n, R, center, Re, L = n2, R2, c2, Re2, L2
+        traffic_file, label = "hbm_traffic.json", "3D flow past a sphere 256x128x128, Re=3700"
+        a.no_c2 = a.no_c3 = True
+    r = run_gpu(wl, n, R, center, Re, L, a.steps, a.warmup, tune=not a.no_autotune and not a.tiles,
                 tiles=tuple(int(v) for v in a.tiles.split(",")) if a.tiles else None)
     roof, rows = roofline(r["stats"], os.path.join(ROOT, "profiles", traffic_file))
     out = dict(metric=METRIC, value=round(r["mcells"], 2), unit="Mcell-steps/s", n_gpus=1, steps=a.steps,
@@ -472,29 +475,32 @@ def main():
                       "from a second pass of the same number of steps")
     if not a.quiet_kernels:
         out["kernels"] = rows[:14]
+    del r
     if not a.no_c3:
-        n3, R3, c3, Re3, L3 = (512, 256, 256), 32, (128, 128, 128), 1e4, 64
-        r3 = run_gpu(wl, n3, R3, c3, Re3, L3, a.c3_steps, 3, tune=True)
-        roof3, rows3 = roofline(r3["stats"], os.path.join(ROOT, "profiles", "hbm_traffic_c3.json"))
-        out["c3"] = dict(workload="3D sphere 512x256x256, r=32, Re=1e4, plain FP32 MG", steps=a.c3_steps,
-                         tiles=list(r3["tiles"]),
-                         mcells=round(r3["mcells"], 2), ms_per_step=round(r3["ms_per_step"], 3),
-                         vcycles_per_step=round(r3["vcycles_per_step"], 3), roofline=roof3,
-                         kernels_L0=[k for k in rows3 if k["kernel"].endswith("@L0")][:10])
-        del r3
-        r3m = run_gpu(wl, n3, R3, c3, Re3, L3, a.c3_steps, 3, profile_steps=0, mode=wl.WL_SOLVER_MIXED, tune=True)
+        r3m = run_gpu(wl, n, R, center, Re, L, a.c3_steps, a.warmup, profile_steps=0, mode=wl.WL_SOLVER_MIXED,
+                      tune=True)
         out["c3_mixed"] = dict(workload="the same 512x256x256 sphere with FP32 V-cycles inside an FP64 "
-                                        "defect-correction loop (coarsest-level smoothing in FP64)", steps=a.c3_steps,
+                                        "defect-correction loop", steps=a.c3_steps, warmup=a.warmup,
                                mcells=round(r3m["mcells"], 2), ms_per_step=round(r3m["ms_per_step"], 3),
-                               vcycles_per_step=round(r3m["vcycles_per_step"], 3))
+                               vcycles_per_step=round(r3m["vcycles_per_step"], 3),
+                               vs_plain=round(r3m["mcells"] / out["value"], 3))
         del r3m
+    if not a.no_c2:
+        r2 = run_gpu(wl, n2, R2, c2, Re2, L2, a.c2_steps, a.warmup, tune=True)
+        roof2, rows2 = roofline(r2["stats"], os.path.join(ROOT, "profiles", "hbm_traffic.json"))
+        out["c2"] = dict(workload="3D sphere 256x128x128, r=16, Re=3700, plain FP32 MG", steps=a.c2_steps,
+                         warmup=a.warmup, tiles=list(r2["tiles"]),
+                         mcells=round(r2["mcells"], 2), ms_per_step=round(r2["ms_per_step"], 3),
+                         vcycles_per_step=round(r2["vcycles_per_step"], 3), roofline=roof2,
+                         kernels_L0=[k for k in rows2 if k["kernel"].endswith("@L0")][:10])
+        del r2
     if not a.no_c4:
         out["c4"] = run_c4(wl, a.c4_steps, 20)
     if not a.no_c5:
         out["c5"] = run_c5(wl, a.c5_steps, 3)
     if not a.no_cpu:
         out["c1"] = run_c1(wl)
-        out["cpu_baseline"] = run_cpu(n, R, center, Re, L)
+        out["cpu_baseline"] = run_cpu(n, R, center, Re, L, label="x".join(str(v) for v in n))
     else:
         out["cpu_baseline"] = None
     print(json.dumps(out), flush=True)

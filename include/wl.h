@@ -148,6 +148,16 @@ int wl_flow_set_field_fn(wl_flow_t h, int which, const void* src, size_t nbytes)
  * exitBC, perdir) at t = 0 with the WL_FN_UBC field if set, exitBC!(u,u,0) and u⁰ = u (:140-141). */
 int wl_flow_apply_ic(wl_flow_t h);
 
+/* udf!(flow, force!, u, t) (src/Flow.jl:247-257): a user function that mom_step! calls in the
+ * predictor (phase 0, t = t₀, the advecting field is u⁰) and in the corrector (phase 1, t = t₁,
+ * the advecting field is u), after conv_diff! and before accelerate! and BDIM! (:193,207).  It runs
+ * on the calling thread with the stream idle and may download and upload any field of `h`
+ * (typically f, which then holds conv_diff!'s result); a nonzero return aborts the step with
+ * WL_EINVAL.  fn = NULL removes it.  Replaces the `udf` keyword of mom_step!/sim_step!
+ * (src/WaterLily.jl:128-140), which a Julia closure cannot cross the C ABI to provide otherwise. */
+typedef int (*wl_udf_fn)(void* user, wl_flow_t h, int phase, double t);
+int wl_flow_set_udf(wl_flow_t h, wl_udf_fn fn, void* user);
+
 /* CFL(a) (src/Flow.jl:234-237) of the flow's current u, as mom_step! ends with it: used when a
  * restart replaces u (load!, ext/WaterLilyReadVTKExt.jl:38). */
 int wl_flow_cfl(wl_flow_t h, double* dt);

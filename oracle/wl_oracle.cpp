@@ -6,6 +6,7 @@
 #include "wl_oracle.hpp"
 #include <omp.h>
 #include <string>
+#include <stdexcept>
 #include <cstdio>
 
 namespace wlo {
@@ -445,6 +446,11 @@ struct Flow {
   // function-valued uBC and forcing, evaluated by the caller on every cell (empty: not used):
   // ubc = uBC(i,loc(i,I),t₁); acc0/acc1 = g + ∂uBC/∂t at t₀ (predictor) and t₁ (corrector)
   std::vector<T> ubc, acc0, acc1;
+  // udf!(flow, force!, u, t) (src/Flow.jl:255-257): the caller's function, called with the C
+  // handle, the phase (0 predictor, 1 corrector) and t, between conv_diff! and accelerate!
+  int (*udf)(void*, void*, int, double) = nullptr;
+  void* udf_user = nullptr;
+  void* handle = nullptr;
   void Ut(T t, T* out) const {  // uBC(i,x,t), uniform in x
     for (int i = 0; i < 3; ++i) out[i] = has_rate ? U[i] + rate[i] * t : U[i];
   }
@@ -622,6 +628,7 @@ T mom_step(Flow<T>& a, T dt, T t0, T t1, int16_t* nit) {
   a.pois.r2hist.clear();
   // predictor
   conv_diff(a, a.f.data(), a.u0.data(), a.sigma.data());
+  if (a.udf && a.udf(a.udf_user, a.handle, 0, (double)t0) != 0) throw std::runtime_error("udf");
   accelerate(a, a.f.data(), t0, a.acc0);
   BDIM(a, dt);
   a.applyBC(t1);
@@ -629,6 +636,7 @@ T mom_step(Flow<T>& a, T dt, T t0, T t1, int16_t* nit) {
   mom_project(a, T(1), dt, t1);
   // corrector
   conv_diff(a, a.f.data(), a.u.data(), a.sigma.data());
+  if (a.udf && a.udf(a.udf_user, a.handle, 1, (double)t1) != 0) throw std::runtime_error("udf");
   accelerate(a, a.f.data(), t1, a.acc1);
   BDIM(a, dt);
   scale_u(a, T(0.5));
@@ -994,8 +1002,22 @@ int wlo_flow_set_solver(wlo_flow* h, int mode, double tol, int itmx) {
   return 0;
 }
 int wlo_mom_step(wlo_flow* h, double dt, double t0, double t1, double* dt_next, int16_t* nit) {
-  if (h->dtype == WL_F64) *dt_next = mom_step<double>(*(Flow<double>*)h->impl, dt, t0, t1, nit);
-  else *dt_next = (double)mom_step<float>(*(Flow<float>*)h->impl, (float)dt, (float)t0, (float)t1, nit);
+  try {
+    if (h->dtype == WL_F64) *dt_next = mom_step<double>(*(Flow<double>*)h->impl, dt, t0, t1, nit);
+    else *dt_next = (double)mom_step<float>(*(Flow<float>*)h->impl, (float)dt, (float)t0, (float)t1, nit);
+  } catch (const std::runtime_error&) {
+    return WL_EINVAL;  // the udf returned an error
+  }
+  return 0;
+}
+int wlo_flow_set_udf(wlo_flow* h, int (*fn)(void*, void*, int, double), void* user) {
+  auto set = [&](auto* a) {
+    a->udf = fn;
+    a->udf_user = user;
+    a->handle = h;
+  };
+  if (h->dtype == WL_F64) set((Flow<double>*)h->impl);
+  else set((Flow<float>*)h->impl);
   return 0;
 }
 int wlo_pressure_force(wlo_flow* h, const wl_body* b, int nb, double t, double* F) {

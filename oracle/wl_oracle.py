@@ -69,6 +69,7 @@ def lib():
             "wlo_flow_last_residuals": (i, [vp, P(d), i, P(i)]),
             "wlo_flow_phase": (i, [vp, i, d, d, d]),
             "wlo_flow_cfl": (d, [vp]),
+            "wlo_flow_set_udf": (i, [vp, UDF_FN, vp]),
             "wlo_pois_create": (i, [P(vp), i, P(C.c_int32), i, vp, i, i]),
             "wlo_pois_destroy": (i, [vp]),
             "wlo_pois_nlevels": (i, [vp, P(i)]),
@@ -99,6 +100,9 @@ def lib():
             f.restype, f.argtypes = res, args
         _lib = L
     return _lib
+
+
+UDF_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_void_p, C.c_int, C.c_double)
 
 
 def _np_dtype(dtype):
@@ -185,14 +189,24 @@ class OFlow:
     def set_solver(self, tol=1e-4, itmx=32):
         _ck(lib().wlo_flow_set_solver(self.h, 0, tol, itmx), "set_solver")
 
-    def mom_step(self):
-        """mom_step!(flow, pois) with the host-side Δt bookkeeping of src/Flow.jl:157,166."""
+    def mom_step(self, udf=None):
+        """mom_step!(flow, pois) with the host-side Δt bookkeeping of src/Flow.jl:157,166.
+        udf(oflow, phase, t): called between conv_diff! and accelerate! (phase 0 predictor, 1
+        corrector); it reads and writes fields with get/set."""
         dt = self.dt[-1]
         t1 = float(np.sum(np.asarray(self.dt, dtype=self.npdt), dtype=self.npdt))
         t0 = t1 - dt
         out = C.c_double()
         nit = (C.c_int16 * 2)()
-        _ck(lib().wlo_mom_step(self.h, dt, t0, t1, C.byref(out), nit), "mom_step")
+        cb = None
+        if udf is not None:
+            cb = UDF_FN(lambda user, h, phase, t: udf(self, phase, t) or 0)
+            lib().wlo_flow_set_udf(self.h, cb, None)
+        try:
+            _ck(lib().wlo_mom_step(self.h, dt, t0, t1, C.byref(out), nit), "mom_step")
+        finally:
+            if cb is not None:
+                lib().wlo_flow_set_udf(self.h, UDF_FN(), None)
         self.dt.append(float(self.npdt(out.value)))
         return (int(nit[0]), int(nit[1]))
 
@@ -414,15 +428,15 @@ class OSimulation:
             self.flow.measure(self.bodies, t=t, eps=self.eps)
             self.flow.pois_update()
 
-    def step(self, remeasure=True):
+    def step(self, remeasure=True, udf=None):
         if remeasure:
             self.measure()
-        self.n.extend(self.flow.mom_step())
+        self.n.extend(self.flow.mom_step(udf))
 
-    def sim_step(self, t_end, remeasure=True, max_steps=1 << 62):
+    def sim_step(self, t_end, remeasure=True, max_steps=1 << 62, udf=None):
         steps0 = len(self.flow.dt)
         while self.sim_time() < t_end and len(self.flow.dt) - steps0 < max_steps:
-            self.step(remeasure)
+            self.step(remeasure, udf)
 
     def pressure_force(self):
         return self.flow.pressure_force(self.bodies)

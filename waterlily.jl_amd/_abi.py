@@ -46,6 +46,8 @@ class WaterLilyError(RuntimeError):
 
 _P = C.POINTER
 _vp, _i, _d, _sz = C.c_void_p, C.c_int, C.c_double, C.c_size_t
+# wl_udf_fn: int (*)(void* user, wl_flow_t h, int phase, double t)
+UDF_FN = C.CFUNCTYPE(_i, _vp, _vp, _i, _d)
 SIGNATURES = {
     "wl_last_error": (C.c_char_p, []),
     "wl_device_count": (_i, [_P(_i)]),
@@ -70,6 +72,7 @@ SIGNATURES = {
     "wl_flow_set_field_fn": (_i, [_vp, _i, _vp, _sz]),
     "wl_flow_apply_ic": (_i, [_vp]),
     "wl_flow_cfl": (_i, [_vp, _P(_d)]),
+    "wl_flow_set_udf": (_i, [_vp, UDF_FN, _vp]),
     "wl_flow_autotune": (_i, [_vp, _i, _P(C.c_int32)]),
     "wl_flow_set_tiles": (_i, [_vp, C.c_int32, C.c_int32]),
     "wl_batch_create": (_i, [_P(_vp), _P(_vp), _i]),

@@ -328,9 +328,47 @@ def time(flow):
     return float(np.sum(np.asarray(flow.Δt[:-1], dtype=flow.T), dtype=flow.T)) if len(flow.Δt) > 1 else 0.0
 
 
-def mom_step(flow, pois=None):
-    """mom_step!(a::Flow, b::MultiLevelPoisson) (src/Flow.jl:156-167): one predictor-corrector
-    step on the device, then push!(Δt, CFL)."""
+class UdfFlow:
+    """The flow as a udf! sees it during mom_step!: the fields (u, u0, f, p, sigma, V, mu0, mu1) are
+    numpy arrays in Julia layout, downloaded when first read; the ones the function changes are
+    uploaded when it returns.  Other attributes (ν, Δt, D, N, T, ...) are the flow's."""
+    _FIELDS = ("u", "u0", "f", "p", "sigma", "V", "mu0", "mu1")
+
+    def __init__(self, flow):
+        self._flow = flow
+        self._cache = {}
+
+    def __getattr__(self, name):
+        if name in UdfFlow._FIELDS:
+            if name not in self._cache:
+                a = self._flow.get(name)
+                self._cache[name] = (a, a.tobytes())
+            return self._cache[name][0]
+        return getattr(self._flow, name)
+
+    def _flush(self):
+        for name, (a, orig) in self._cache.items():
+            if a.tobytes() != orig:
+                self._flow.set(name, a)
+
+
+def _udf_call(udf, view, u, t, kw):
+    """udf!(flow, force!, u, t; kw...) (src/Flow.jl:255-257): force!(flow, u, t; kw...) when that
+    form applies, else force!(flow, t; kw...)."""
+    import inspect
+    try:
+        inspect.signature(udf).bind(view, u, t, **kw)
+        three = True
+    except TypeError:
+        three = False
+    return udf(view, u, t, **kw) if three else udf(view, t, **kw)
+
+
+def mom_step(flow, pois=None, udf=None, **kw):
+    """mom_step!(a::Flow, b::MultiLevelPoisson; udf, kw...) (src/Flow.jl:156-167): one
+    predictor-corrector step on the device, then push!(Δt, CFL).  udf(flow, [u,] t; kw...) runs
+    between conv_diff! and accelerate! of both phases (the predictor with u = u⁰ at t₀, the
+    corrector with u at t₁) on a UdfFlow view of the fields."""
     pois = flow.pois if pois is None else pois
     dt = flow.Δt[-1]
     t1 = float(np.sum(np.asarray(flow.Δt, dtype=flow.T), dtype=flow.T))
@@ -339,7 +377,27 @@ def mom_step(flow, pois=None):
         flow._functions_for_step(t0, t1)
     out = C.c_double()
     nit = (C.c_int16 * 2)()
-    rc = A.load().wl_mom_step(flow.h, dt, t0, t1, C.byref(out), nit)
+    err = []
+    if udf is not None:
+        def hook(user, h, phase, t):
+            try:
+                view = UdfFlow(flow)
+                _udf_call(udf, view, view.u0 if phase == 0 else view.u, flow.T(t), kw)
+                view._flush()
+                return 0
+            except BaseException as e:  # carried out of the C call and raised below
+                err.append(e)
+                return -1
+        cb = A.UDF_FN(hook)
+        A.check(A.load().wl_flow_set_udf(flow.h, cb, None))
+        try:
+            rc = A.load().wl_mom_step(flow.h, dt, t0, t1, C.byref(out), nit)
+        finally:
+            A.load().wl_flow_set_udf(flow.h, A.UDF_FN(), None)
+        if err:
+            raise err[0]
+    else:
+        rc = A.load().wl_mom_step(flow.h, dt, t0, t1, C.byref(out), nit)
     if rc == A.WL_ENAN:
         raise FloatingPointError(A.load().wl_last_error().decode())
     A.check(rc)
@@ -514,16 +572,16 @@ def sim_time(sim):
     return time(sim.flow) * sim.U / sim.L
 
 
-def sim_step(sim, t_end=None, remeasure=True, max_steps=None):
-    """sim_step!(sim[, t_end]; remeasure, max_steps) (src/WaterLily.jl:128-139)."""
+def sim_step(sim, t_end=None, remeasure=True, max_steps=None, udf=None, **kw):
+    """sim_step!(sim[, t_end]; remeasure, max_steps, udf, kw...) (src/WaterLily.jl:128-139)."""
     if t_end is None:
         if remeasure:
             measure(sim)
-        mom_step(sim.flow, sim.pois)
+        mom_step(sim.flow, sim.pois, udf=udf, **kw)
         return sim
     steps0 = len(sim.flow.Δt)
     while sim_time(sim) < t_end and (max_steps is None or len(sim.flow.Δt) - steps0 < max_steps):
-        sim_step(sim, remeasure=remeasure)
+        sim_step(sim, remeasure=remeasure, udf=udf, **kw)
     return sim
 
 

@@ -79,6 +79,7 @@ struct Accel {
   int on;
   double add[3];       // uniform g(t) + ∂uBC/∂t, formed in T by the host
   const void* field;   // or, when set, g(i,x,t) + ∂uBC/∂t(i,x,t) on every cell (T, layout of u)
+  int raw;             // write conv_diff!'s result alone (a udf! runs next); k_bdim1 finishes f
 };
 
 // Face flux F_j^i at the lower j-face of cell I (src/Flow.jl:8-11,47,56-57,61): variant 0 inner
@@ -151,6 +152,10 @@ __global__ void __launch_bounds__(64 * CBY) k_conv_bdim1(Geo g, ZT t, const T* _
     const int64_t I = at(g, cx, cy, cz);
     for (int i = 0; i < D; ++i) {
       T r = rr[i];
+      if (acc.raw) {
+        f[I + i * n] = r;
+        continue;
+      }
       if (acc.on) r += (T)acc.add[i];
       if (acc.field) r += ((const T*)acc.field)[I + i * n];
       const T uv = u[I + i * n];
@@ -311,6 +316,10 @@ __device__ __forceinline__ void k_conv_lds_body(const Geo& g, const ZT& t, const
 #pragma unroll
     for (int i = 0; i < D; ++i) {
       T r = rr.v[i];
+      if (acc.raw) {
+        f[I + i * n] = r;
+        continue;
+      }
       if (acc.on) r += (T)acc.add[i];
       if (acc.field) r += ((const T*)acc.field)[I + i * n];
       const T uv = uvals.v[i];
@@ -450,7 +459,7 @@ __global__ void __launch_bounds__(64 * CBY, sizeof(T) == 4 ? 4 : 2) k_conv_lds(G
 // ±0·(finite difference), so the sum stays +0 and neither μ₁ nor the f-stencil is read there
 // (identical results for finite f). A wave covers 64 cells of one row, so the branch is uniform.
 // μ₀ is read through the finest level's LRows (μ₀ is its L) and V only in rows where it is not +0.
-template <class T, bool PRED>
+template <class T, bool PRED, bool ADD = false>
 __device__ __forceinline__ void k_bdim2_body(const Geo& g, const ZT& t, T* __restrict__ u, const T* __restrict__ uold,
                                                const T* __restrict__ f, const T* __restrict__ V,
                                                const LRows<T>& mu0, const T* __restrict__ mu1,
@@ -473,7 +482,7 @@ __device__ __forceinline__ void k_bdim2_body(const Geo& g, const ZT& t, T* __res
   for (int i = 0; i < 3; ++i) {
     const int64_t k = I + dir_or_1(g, i) * n;
     fv[i] = f[k];
-    uv[i] = PRED ? uold[k] : u[k];
+    uv[i] = (PRED && !ADD) ? uold[k] : u[k];
   }
   if (vnz) {
 #pragma unroll
@@ -494,17 +503,44 @@ __device__ __forceinline__ void k_bdim2_body(const Geo& g, const ZT& t, T* __res
     if (i >= D) continue;
     const int64_t k = I + i * n;
     const T inc = sb[i] / T(2) + (vnz ? vv[i] : T(0)) + m0.l[i] * fv[i];
-    if (PRED) u[k] = uv[i] * T(0) + inc;
+    if (ADD) u[k] = uv[i] + inc;
+    else if (PRED) u[k] = uv[i] * T(0) + inc;
     else u[k] = (uv[i] + inc) * T(0.5);
   }
   }
 }
-template <class T, bool PRED>
+template <class T, bool PRED, bool ADD = false>
 __global__ void __launch_bounds__(256) k_bdim2(Geo g, ZT t, T* __restrict__ u, const T* __restrict__ uold,
                                                const T* __restrict__ f, const T* __restrict__ V,
                                                LRows<T> mu0, const T* __restrict__ mu1,
                                                const uint8_t* __restrict__ rows, const uint8_t* __restrict__ vrows) {
-  k_bdim2_body<T, PRED>(g, t, u, uold, f, V, mu0, mu1, rows, vrows, xcd_tile_id());
+  k_bdim2_body<T, PRED, ADD>(g, t, u, uold, f, V, mu0, mu1, rows, vrows, xcd_tile_id());
+}
+
+// The step's remaining f-passes when a udf! runs between conv_diff! and accelerate! (src/Flow.jl:
+// 192-194,206-208): accelerate! (the uniform or field forcing) and BDIM!'s first loop,
+// f = u⁰ + Δt·f − V on every cell (:178), in the fused kernels' order of operations.
+template <class T>
+__global__ void __launch_bounds__(256) k_bdim1(Geo g, Box b, const T* __restrict__ u0, T* __restrict__ f,
+                                               const T* __restrict__ V, T dt, Accel acc) {
+  int i, j, k;
+  if (!box_cell(b, i, j, k)) return;
+  const int64_t I = at(g, i, j, k);
+  for (int c = 0; c < g.D; ++c) {
+    const int64_t q = I + c * g.n;
+    T r = f[q];
+    if (acc.on) r += (T)acc.add[c];
+    if (acc.field) r += ((const T*)acc.field)[q];
+    f[q] = u0[q] + dt * r - V[q];
+  }
+}
+// scale_u!(a, s) (src/Flow.jl:211-214): u *= s on inside, every component
+template <class T>
+__global__ void __launch_bounds__(256) k_scale_u(Geo g, Box b, T* __restrict__ u, T s) {
+  int i, j, k;
+  if (!box_cell(b, i, j, k)) return;
+  const int64_t I = at(g, i, j, k);
+  for (int c = 0; c < g.D; ++c) u[I + c * g.n] = u[I + c * g.n] * s;
 }
 
 // Uniform-row flags of a level's face coefficients (LRows). Pass 1, one wavefront per x-row
@@ -1173,6 +1209,277 @@ template <class T>
 __global__ void __launch_bounds__(256) k_gs_half(Geo g, ZT t, LRows<T> LR, const T* __restrict__ r,
                                                  T* __restrict__ e, int it) {
   k_gs_half_body<T>(g, t, LR, r, e, it, xcd_tile_id());
+}
+
+// ---- x-pair kernels: each lane owns the two neighbouring cells (xa, xa+1), xa = 1 + 2p, of an
+// x-row and moves them with one 8-byte (16-byte for double) load or store.  The pairs cover inside
+// exactly when N0 − 2 is even.  A wave is one x-row of 64 pairs (128 cells); a thread walks z with
+// its column's ϵ pairs of planes z−1, z, z+1 in registers; x-neighbours across a pair boundary come
+// from the next lane (wave shuffle), or, at the wave's ends, from memory.  Rows start 4-byte
+// aligned: gfx950 performs dword-aligned multi-dword global accesses.
+typedef float wl_f2u __attribute__((ext_vector_type(2), aligned(4)));
+typedef double wl_d2u __attribute__((ext_vector_type(2), aligned(8)));
+template <class T> struct PairV;
+template <> struct PairV<float> { typedef wl_f2u type; };
+template <> struct PairV<double> { typedef wl_d2u type; };
+template <class T>
+struct P2 {
+  T a, b;
+};
+template <class T>
+__device__ __forceinline__ P2<T> ld2(const T* __restrict__ p) {
+  const typename PairV<T>::type v = *(const typename PairV<T>::type*)p;
+  return P2<T>{v.x, v.y};
+}
+template <class T>
+__device__ __forceinline__ void st2(T* __restrict__ p, T a, T b) {
+  typename PairV<T>::type v;
+  v.x = a;
+  v.y = b;
+  *(typename PairV<T>::type*)p = v;
+}
+// the pair (xa, xa+1) of this thread: pair index p (tile x is in pair units), row y, planes z0..z1
+struct PairCell {
+  int p, y, z0, z1, xa;
+  bool vx;      // the pair exists (lanes past the row end load the last pair and store nothing)
+  bool edge_r;  // the pair to the right is not in this wave
+};
+__device__ __forceinline__ PairCell pair_cell(const ZT& t) {
+  PairCell c;
+  zt_cell(t, c.p, c.y, c.z0, c.z1);
+  c.vx = c.p <= t.b.hi[0];
+  c.xa = 1 + 2 * (c.vx ? c.p : t.b.hi[0]);
+  c.edge_r = threadIdx.x == 63 || c.p >= t.b.hi[0];
+  return c;
+}
+
+// Four GS-RB half sweeps are four launches of this kernel (src/Poisson.jl:124-128,144-146): it
+// updates the cells of colour `it` over inside, the cells GaussSeidelRB!'s half_rangek reaches when
+// the last ghosted extent is even (the caller checks it).  A cell is updated in sweep `it` when
+// the sum of its 0-based indices + it + D + 1 is even (k = 2Iv−1−(Σfront+it)%2 of gauss_rb, 1-based).
+// Every value a sweep reads is of the other colour, which no thread writes during the sweep, so
+// the z window and the neighbour pairs stay valid; the updated cell's own old value is never read.
+template <class T, int D>
+__global__ void __launch_bounds__(256) k_gs_half2(Geo g, ZT t, LRows<T> LR, const T* __restrict__ r,
+                                                  T* __restrict__ e, int it) {
+  const PairCell c = pair_cell(t);
+  if (c.y > t.b.hi[1]) return;  // a whole wave (one row)
+  const int lane = threadIdx.x;
+  const int64_t s1 = g.s[1], s2 = g.s[2];
+  const int64_t rowI = c.xa + s1 * c.y;
+  P2<T> em{T(0), T(0)}, ep{T(0), T(0)};
+  P2<T> e0 = ld2(e + rowI + s2 * c.z0);
+  if (D == 3) {
+    em = ld2(e + rowI + s2 * (c.z0 - 1));
+    ep = ld2(e + rowI + s2 * (c.z0 + 1));
+  }
+  for (int z = c.z0; z <= c.z1; ++z) {
+    const int64_t I = rowI + s2 * z;
+    const P2<T> rr = ld2(r + I);
+    const P2<T> ym = ld2(e + I - s1), yp = ld2(e + I + s1);
+    P2<T> en{T(0), T(0)};
+    if (D == 3 && z < c.z1) en = ld2(e + I + 2 * s2);
+    const bool first = ((c.xa + c.y + z + it + D + 1) & 1) == 0;  // the active cell is xa (wave-uniform)
+    T xm, xp;
+    if (first) {
+      xm = __shfl_up(e0.b, 1, 64);
+      if (lane == 0) xm = e[I - 1];
+      xp = e0.b;
+    } else {
+      xm = e0.a;
+      xp = __shfl_down(e0.a, 1, 64);
+      if (c.edge_r) xp = e[I + 2];
+    }
+    const int ca = first ? c.xa : c.xa + 1;
+    const LC<T> L = lcell(g, LR, ca, c.y, z);
+    const T Dg = diag_at<T>(g, L, 0);
+    T s = first ? rr.a : rr.b;
+    s -= (xm * L.l[0] + xp * L.h[0]);
+    s -= ((first ? ym.a : ym.b) * L.l[1] + (first ? yp.a : yp.b) * L.h[1]);
+    if (D == 3) s -= ((first ? em.a : em.b) * L.l[2] + (first ? ep.a : ep.b) * L.h[2]);
+    const T nv = s * inv_diag(Dg);
+    if (c.vx) e[I + (first ? 0 : 1)] = nv;
+    if (D == 3) {
+      em = e0;
+      if (first) em.a = nv; else em.b = nv;
+      e0 = ep;
+      ep = en;
+    }
+  }
+}
+
+// increment! (src/Poisson.jl:100-104) on pairs: r -= ω·Aϵ; x += ω·ϵ, optionally the Σr² partials
+// (L₂ of the level after a smooth!, src/MultiLevelPoisson.jl:115) and, EJ, ϵ_J = r·iD of the new r
+// into ej: the ϵ the next V-cycle's Jacobi! starts from (src/Poisson.jl:112), written here instead
+// of by a pass of its own (ej is not ϵ, which this kernel reads around each cell).
+template <class T, int D, bool L2, bool EJ>
+__global__ void __launch_bounds__(256) k_increment2(Geo g, ZT t, LRows<T> LR, T* __restrict__ r, T* __restrict__ x,
+                                                    const T* __restrict__ e, const T* __restrict__ wp,
+                                                    double* __restrict__ partial, T* __restrict__ ej) {
+  const T w = *wp;
+  const PairCell c = pair_cell(t);
+  double acc = 0;
+  if (c.y <= t.b.hi[1]) {
+    const int lane = threadIdx.x;
+    const int64_t s1 = g.s[1], s2 = g.s[2];
+    const int64_t rowI = c.xa + s1 * c.y;
+    P2<T> em{T(0), T(0)}, ep{T(0), T(0)};
+    P2<T> e0 = ld2(e + rowI + s2 * c.z0);
+    if (D == 3) {
+      em = ld2(e + rowI + s2 * (c.z0 - 1));
+      ep = ld2(e + rowI + s2 * (c.z0 + 1));
+    }
+    for (int z = c.z0; z <= c.z1; ++z) {
+      const int64_t I = rowI + s2 * z;
+      const P2<T> rr = ld2(r + I), xx = ld2(x + I);
+      const P2<T> ym = ld2(e + I - s1), yp = ld2(e + I + s1);
+      P2<T> en{T(0), T(0)};
+      if (D == 3 && z < c.z1) en = ld2(e + I + 2 * s2);
+      T xl = __shfl_up(e0.b, 1, 64), xr = __shfl_down(e0.a, 1, 64);
+      if (lane == 0) xl = e[I - 1];
+      if (c.edge_r) xr = e[I + 2];
+      const LC<T> La = lcell(g, LR, c.xa, c.y, z), Lb = lcell(g, LR, c.xa + 1, c.y, z);
+      const Nb<T> va{{xl, ym.a, em.a}, {e0.b, yp.a, ep.a}}, vb{{e0.a, ym.b, em.b}, {xr, yp.b, ep.b}};
+      const T Da = diag_at<T>(g, La, 0), Db = diag_at<T>(g, Lb, 0);
+      const T ra = rr.a - w * mult_nb<T>(g, La, 0, Da, e0.a, va);
+      const T rb = rr.b - w * mult_nb<T>(g, Lb, 0, Db, e0.b, vb);
+      if (c.vx) {
+        st2(r + I, ra, rb);
+        st2(x + I, xx.a + w * e0.a, xx.b + w * e0.b);
+        if (EJ) st2(ej + I, ra * inv_diag(Da), rb * inv_diag(Db));
+        if (L2) {
+          acc += (double)ra * (double)ra;
+          acc += (double)rb * (double)rb;
+        }
+      }
+      if (D == 3) {
+        em = e0;
+        e0 = ep;
+        ep = en;
+      }
+    }
+  }
+  if (L2) {
+    const double tt = block_sum(acc);
+    if (threadIdx.x == 0 && threadIdx.y == 0) partial[blockIdx.x] = tt;
+  }
+}
+
+// prolongate! + increment!(ω) + ϵ := r·iD (src/MultiLevelPoisson.jl:98-100,50) on fine pairs, for a
+// level coarsened in every direction without periodic directions.  Both cells of the pair
+// (xa, xa+1) = (2q−1, 2q) read coarse x cell q = p + 1 (down(I) = (I+2)÷2 in 1-based indices); the
+// x-neighbours outside the pair read coarse q − 1 and q + 1, "no value" (0) on a ghost.  The
+// arithmetic per cell is prolong_cell's; CINC folds the coarse level's last increment! in (x + ω·ϵ
+// at the coarse cells read).
+template <class T, int D, bool CINC>
+__global__ void __launch_bounds__(256) k_prolong_inc2(Geo gf, Geo gc, ZT t, LRows<T> LR, T* __restrict__ rf,
+                                                      T* __restrict__ xf, T* __restrict__ ef,
+                                                      const T* __restrict__ xc, const T* __restrict__ wp,
+                                                      const T* __restrict__ ec) {
+  const T w = *wp;
+  const PairCell c = pair_cell(t);
+  if (c.y > t.b.hi[1]) return;
+  const int N0 = gf.N[0], N1 = gf.N[1], N2 = gf.N[2];
+  const int64_t cs1 = gc.s[1], cs2 = gc.s[2];
+  auto cv = [&](int a, int b, int z, bool ok) -> T {  // coarse x (+ ω·ϵ), 0 for "no value"
+    const int64_t J = ok ? (int64_t)a + cs1 * b + cs2 * z : 0;
+    T v = xc[J];
+    if (CINC) v = v + w * ec[J];
+    return ok ? v : T(0);
+  };
+  const int qx = (c.xa + 1) >> 1, qy = (c.y + 1) >> 1;
+  const bool hl = c.xa - 1 >= 1, hr = c.xa + 2 <= N0 - 2, hym = c.y - 1 >= 1, hyp = c.y + 1 <= N1 - 2;
+  const int qym = c.y >> 1, qyp = (c.y + 2) >> 1;
+  const int64_t s1 = gf.s[1], s2 = gf.s[2];
+  for (int z = c.z0; z <= c.z1; ++z) {
+    const int64_t I = c.xa + s1 * c.y + s2 * z;
+    const int qz = D == 3 ? (z + 1) >> 1 : 0;
+    const bool hzm = D == 3 && z - 1 >= 1, hzp = D == 3 && z + 1 <= N2 - 2;
+    const T own = cv(qx, qy, qz, true);
+    const T xl = cv(qx - 1, qy, qz, hl), xr = cv(qx + 1, qy, qz, hr);
+    const T ym = cv(qx, qym, qz, hym), yp = cv(qx, qyp, qz, hyp);
+    const T zm = cv(qx, qy, D == 3 ? z >> 1 : 0, hzm), zp = cv(qx, qy, D == 3 ? (z + 2) >> 1 : 0, hzp);
+    const P2<T> rr = ld2(rf + I), xx = ld2(xf + I);
+    const LC<T> La = lcell(gf, LR, c.xa, c.y, z), Lb = lcell(gf, LR, c.xa + 1, c.y, z);
+    const Nb<T> va{{xl, ym, zm}, {own, yp, zp}}, vb{{own, ym, zm}, {xr, yp, zp}};
+    const T Da = diag_at<T>(gf, La, 0), Db = diag_at<T>(gf, Lb, 0);
+    const T ra = rr.a - w * mult_nb<T>(gf, La, 0, Da, own, va);
+    const T rb = rr.b - w * mult_nb<T>(gf, Lb, 0, Db, own, vb);
+    if (c.vx) {
+      st2(rf + I, ra, rb);
+      st2(xf + I, xx.a + w * own, xx.b + w * own);
+      st2(ef + I, ra * inv_diag(Da), rb * inv_diag(Db));
+    }
+  }
+}
+
+// Jacobi!'s increment! (ω = 1) + restrict! + fill!(coarse x, 0) + the coarse ϵ = r·iD
+// (src/MultiLevelPoisson.jl:92-94,13-19) for a level coarsened in every direction: a lane owns the
+// two x-children (2q−1, 2q) of coarse x cell q; lanes 0..31 of a wave take fine row 2Q−1 and lanes
+// 32..63 row 2Q of coarse row Q, so a coarse cell's four children of a fine plane are in lanes l
+// and l+32 and are summed with one shuffle, in the reference's z, y, x order.  Tiles are in coarse
+// units: 32 coarse x cells by 4 coarse rows (one per wave), z in coarse planes.
+template <class T, int D>
+__global__ void __launch_bounds__(256) k_inc_restrict2(Geo gf, Geo gc, ZT t, LRows<T> Lf, T* __restrict__ rf,
+                                                       T* __restrict__ xf, const T* __restrict__ ef,
+                                                       LRows<T> Lc, T* __restrict__ rc, T* __restrict__ xc,
+                                                       T* __restrict__ ec) {
+  int cx, cy, z0, z1;
+  zt_cell(t, cx, cy, z0, z1);  // threadIdx.x (0..63) is remapped below: the tile is 32 x wide
+  const int lane = threadIdx.x, half = lane >> 5;
+  cx = cx - lane + (lane & 31);
+  if (cy > t.b.hi[1]) return;  // a whole wave (one coarse row)
+  const bool vx = cx <= t.b.hi[0];
+  const int q = vx ? cx : t.b.hi[0];
+  const int xa = 2 * q - 1, fy = 2 * cy - 1 + half;
+  const bool lead = vx && half == 0;
+  const bool edge_l = (lane & 31) == 0, edge_r = (lane & 31) == 31 || cx >= t.b.hi[0];
+  const int64_t s1 = gf.s[1], s2 = gf.s[2];
+  constexpr int NZ = D == 3 ? 2 : 1;
+  for (int kc = z0; kc <= z1; ++kc) {
+    T ra[NZ], rb[NZ];
+#pragma unroll
+    for (int qz = 0; qz < NZ; ++qz) {
+      const int fz = D == 3 ? 2 * kc - 1 + qz : 0;
+      const int64_t I = xa + s1 * fy + s2 * fz;
+      const P2<T> e0 = ld2(ef + I), ym = ld2(ef + I - s1), yp = ld2(ef + I + s1);
+      P2<T> zm{T(0), T(0)}, zp{T(0), T(0)};
+      if (D == 3) {
+        zm = ld2(ef + I - s2);
+        zp = ld2(ef + I + s2);
+      }
+      const P2<T> rr = ld2(rf + I), xx = ld2(xf + I);
+      T xl = __shfl_up(e0.b, 1, 64), xr = __shfl_down(e0.a, 1, 64);
+      if (edge_l) xl = ef[I - 1];
+      if (edge_r) xr = ef[I + 2];
+      const LC<T> La = lcell(gf, Lf, xa, fy, fz), Lb = lcell(gf, Lf, xa + 1, fy, fz);
+      const Nb<T> va{{xl, ym.a, zm.a}, {e0.b, yp.a, zp.a}}, vb{{e0.a, ym.b, zm.b}, {xr, yp.b, zp.b}};
+      const T Da = diag_at<T>(gf, La, 0), Db = diag_at<T>(gf, Lb, 0);
+      ra[qz] = rr.a - T(1) * mult_nb<T>(gf, La, 0, Da, e0.a, va);
+      rb[qz] = rr.b - T(1) * mult_nb<T>(gf, Lb, 0, Db, e0.b, vb);
+      if (vx) {
+        st2(rf + I, ra[qz], rb[qz]);
+        st2(xf + I, xx.a + T(1) * e0.a, xx.b + T(1) * e0.b);
+      }
+    }
+    // children summed in z, y, x order (restrict, src/MultiLevelPoisson.jl:13-19)
+    T s = 0;
+#pragma unroll
+    for (int qz = 0; qz < NZ; ++qz) {
+      const T a1 = __shfl_down(ra[qz], 32, 64), b1 = __shfl_down(rb[qz], 32, 64);
+      s += ra[qz];
+      s += rb[qz];
+      s += a1;
+      s += b1;
+    }
+    if (lead) {
+      const int ccz = D == 3 ? kc : 0;
+      const int64_t Ic = at(gc, cx, cy, ccz);
+      rc[Ic] = s;
+      xc[Ic] = 0;
+      ec[Ic] = s * inv_diag(diag_at<T>(gc, lcell(gc, Lc, cx, cy, ccz), Ic));
+    }
+  }
 }
 
 // Two GS-RB half sweeps in one pass (src/Poisson.jl:116-148, it = 2q−1 then 2q, 3D): colour A

@@ -193,6 +193,9 @@ struct Level {
   Geo g;
   T *L = nullptr, *x = nullptr, *r = nullptr, *e = nullptr, *z = nullptr;
   T* e2 = nullptr;  // second ϵ buffer of the fused GS-RB pairs (ghosts stay 0, as ϵ's do)
+  // ϵ as Jacobi! starts the next V-cycle, r·iD, when the solver keeps it apart from the smoother's
+  // ϵ (the pair-kernel increment writes it; nullptr: the same buffer as e)
+  T* ej = nullptr;
   bool ownL = true, ownx = true, ownz = true;
   int cmask = 0;  // coarsening mask towards the next level
   // uniform-row flags of L: lrow[0, N1·N2) row bits, lrow[N1·N2, 2·N1·N2) the flags
@@ -233,6 +236,7 @@ struct MG {
       dfree(l.r);
       dfree(l.e);
       dfree(l.e2);
+      dfree(l.ej);
       dfree(l.lrow);
     }
   }
@@ -261,6 +265,8 @@ struct MG {
     {
       const char* e = std::getenv("WL_LROWS");
       use_lrows = !e || std::atoi(e) != 0;
+      const char* e3 = std::getenv("WL_PAIRS");
+      use_pairs = !e3 || std::atoi(e3) != 0;
       const char* e2 = std::getenv("WL_GS_PAIR");
       use_gs_pair = e2 && std::atoi(e2) != 0;
     }
@@ -298,6 +304,7 @@ struct MG {
     if (lv.size() <= 2) throw WlErr(WL_EINVAL, "MultiLevelPoisson requires size=a2ⁿ, where n>2");
     for (size_t l = 0; l < lv.size(); ++l)  // before any graph capture: no allocation while capturing
       if (gs_pair_ok((int)l)) lv[l].e2 = dalloc<T>(lv[l].g.n);
+    if (fold_coarse_inc && pairs_ok(0)) lv[0].ej = dalloc<T>(lv[0].g.n);
     static bool attr_set = false;  // the LDS tail may use more than 64 KB of dynamic LDS
     if (!attr_set) {
       HIPCK(hipFuncSetAttribute((const void*)k_mg_tail_lds<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -351,6 +358,25 @@ struct MG {
     for (size_t l = 0; l < lv.size(); ++l)
       if (!lv[l].lrow_ok) build_lrows((int)l);
   }
+  // the buffer Jacobi! reads ϵ from on level l
+  T* jac(int l) const { return lv[l].ej ? lv[l].ej : lv[l].e; }
+  // x-pair kernels (k_gs_half2, k_increment2): an even inside x extent, and an even last ghosted
+  // extent (half_rangek then reaches every inside cell, which the pair sweep updates); WL_PAIRS=0
+  // keeps the one-cell-per-thread kernels
+  bool use_pairs = true;
+  bool pairs_ok(int l) const {
+    const Geo& g = lv[l].g;
+    return use_pairs && (g.N[0] - 2) % 2 == 0 && g.N[0] >= 4 && g.N[g.D - 1] % 2 == 0;
+  }
+  static Box box_pairs(const Geo& g) {  // inside, x in pair units
+    Box b = box_inside(g);
+    b.lo[0] = 0;
+    b.hi[0] = (g.N[0] - 2) / 2 - 1;
+    return b;
+  }
+  // algorithmic bytes of one GS half sweep: r of the updated half and ϵ of the other half (its
+  // neighbours) read, the updated half written; L is not counted (uniform rows form it)
+  double gs_bytes(int l) const { return (double)sizeof(T) * 1.5 * ncells_inside(lv[l].g); }
   LRows<T> lr(int l) const {
     const Level<T>& p = lv[l];
     return LRows<T>{p.L, p.lrow_ok ? p.lrow + (int64_t)p.g.N[1] * p.g.N[2] : nullptr, {p.lc[0], p.lc[1], p.lc[2]}};
@@ -417,12 +443,12 @@ struct MG {
     Level<T>& p = lv[0];
     ZT b = make_zt(box_inside(p.g));
     const int64_t nb = zt_blocks(b);
-    ProfScope ps(P, st, "submean_l2", 0, (double)sizeof(T) * (p.g.D + 3) * ncells_inside(p.g));
+    ProfScope ps(P, st, "submean_l2", 0, (double)sizeof(T) * 3 * ncells_inside(p.g));
     if (cond)
       hipLaunchKernelGGL(k_submean_l2_cond<T>, dim3((unsigned)std::min<int64_t>(nb, 256)), kBlock, 0, st, p.g, b,
-                         lr(0), p.r, p.e, (const T*)B->tsc, B->partial + nb, (int)nb);
+                         lr(0), p.r, jac(0), (const T*)B->tsc, B->partial + nb, (int)nb);
     else
-      hipLaunchKernelGGL(k_submean_l2<T>, zt_grid(b), kBlock, 0, st, p.g, b, lr(0), p.r, p.e, (const T*)B->tsc,
+      hipLaunchKernelGGL(k_submean_l2<T>, zt_grid(b), kBlock, 0, st, p.g, b, lr(0), p.r, jac(0), (const T*)B->tsc,
                          B->partial);
     KCK();
     finish_sum(nb, 1, init ? 2 : 0, 1, cond ? nb : 0);
@@ -436,8 +462,8 @@ struct MG {
   void eps_init(int l) {
     Level<T>& p = lv[l];
     ZT b = make_zt(box_inside(p.g));
-    ProfScope ps(P, st, "eps_init", l, (double)sizeof(T) * (p.g.D + 2) * ncells_inside(p.g));
-    hipLaunchKernelGGL(k_eps_init<T>, zt_grid(b), kBlock, 0, st, p.g, b, lr(l), p.r, p.e);
+    ProfScope ps(P, st, "eps_init", l, (double)sizeof(T) * 2 * ncells_inside(p.g));
+    hipLaunchKernelGGL(k_eps_init<T>, zt_grid(b), kBlock, 0, st, p.g, b, lr(l), p.r, jac(l));
     KCK();
   }
   // fused GS-RB pairs (k_gs_pair) where they apply, with WL_GS_PAIR=1; by default the four half
@@ -474,22 +500,55 @@ struct MG {
       }
       return;
     }
+    if (pairs_ok(l)) {
+      ZT b = make_zt(box_pairs(p.g), tgt_gs());
+      ProfScope ps(P, st, "gs_half", l, gs_bytes(l), 4);
+      for (int it = 1; it <= 4; ++it) {
+        if (p.g.D == 3)
+          hipLaunchKernelGGL((k_gs_half2<T, 3>), zt_grid(b), kBlock, 0, st, p.g, b, lr(l), (const T*)p.r, p.e, it);
+        else
+          hipLaunchKernelGGL((k_gs_half2<T, 2>), zt_grid(b), kBlock, 0, st, p.g, b, lr(l), (const T*)p.r, p.e, it);
+        KCK();
+      }
+      return;
+    }
     Box bh = box_half(p.g);
     if (box_empty(bh)) return;
     ZT b = make_zt(bh, tgt_gs());
     // the four half sweeps are timed as one back-to-back group (no event between them)
-    ProfScope ps(P, st, "gs_half", l, (double)sizeof(T) * (p.g.D + 1.5) * ncells_inside(p.g), 4);
+    ProfScope ps(P, st, "gs_half", l, gs_bytes(l), 4);
     for (int it = 1; it <= 4; ++it) {
       hipLaunchKernelGGL(k_gs_half<T>, zt_grid(b), kBlock, 0, st, p.g, b, lr(l), p.r, p.e, it);
       KCK();
     }
   }
-  // fmode: the finish mode of the Σr² (3: the ω control follows on the device)
+  // fmode: the finish mode of the Σr² (3: the ω control follows on the device).  With pairs and a
+  // separate Jacobi ϵ, an L₂ increment (the last pass of a solver iteration) also writes that ϵ.
+  bool ej_written = false;  // the last increment wrote lv[0].ej (the next iteration skips eps_init)
   void increment(int l, const T* w, bool l2, int fmode = 0) {
     Level<T>& p = lv[l];
     perbc(p.g, p.e);
+    if (pairs_ok(l)) {
+      ZT b = make_zt(box_pairs(p.g));
+      const bool ejw = l2 && p.ej != nullptr;
+      ProfScope ps(P, st, "increment", l, (double)sizeof(T) * (5 + (ejw ? 1 : 0)) * ncells_inside(p.g));
+#define WL_INC2(DD, L2V, EJV)                                                                                    \
+  hipLaunchKernelGGL((k_increment2<T, DD, L2V, EJV>), zt_grid(b), kBlock, 0, st, p.g, b, lr(l), p.r, p.x,         \
+                     (const T*)p.e, w, B->partial, p.ej)
+      if (p.g.D == 3) {
+        if (ejw) WL_INC2(3, true, true); else if (l2) WL_INC2(3, true, false); else WL_INC2(3, false, false);
+      } else {
+        if (ejw) WL_INC2(2, true, true); else if (l2) WL_INC2(2, true, false); else WL_INC2(2, false, false);
+      }
+#undef WL_INC2
+      KCK();
+      if (l == 0) ej_written = ejw;
+      if (l2) finish_sum(zt_blocks(b), 1, fmode);
+      return;
+    }
+    if (l == 0) ej_written = false;
     ZT b = make_zt(box_inside(p.g));
-    ProfScope ps(P, st, "increment", l, (double)sizeof(T) * (p.g.D + 5) * ncells_inside(p.g));
+    ProfScope ps(P, st, "increment", l, (double)sizeof(T) * 5 * ncells_inside(p.g));
     if (l2)
       hipLaunchKernelGGL((k_increment<T, true>), zt_grid(b), kBlock, 0, st, p.g, b, lr(l), p.r, p.x, p.e, w, B->partial);
     else
@@ -573,25 +632,40 @@ struct MG {
   void restrict_level(int l) {
     Level<T>& f = lv[l];
     Level<T>& c = lv[l + 1];
-    perbc(f.g, f.e);
+    T* fe = jac(l);
+    perbc(f.g, fe);
     ProfScope ps(P, st, "inc_restrict", l,
-                 (double)sizeof(T) * ((f.g.D + 5) * ncells_inside(f.g) + 3 * ncells_inside(c.g)));
+                 (double)sizeof(T) * (5 * ncells_inside(f.g) + 3 * ncells_inside(c.g)));
     const int full = (1 << f.g.D) - 1;
-    if (f.cmask == full) {
+    if (f.cmask == full && use_pairs) {
+      // tiles of 32 coarse x cells (a lane owns both x-children) by 4 coarse rows, coarse z planes
+      ZT bc = make_zt(box_inside(c.g), tgt_rp());
+      bc.sx = 32;
+      bc.nbx = (bc.b.hi[0] - bc.b.lo[0] + 32) / 32;
+      const int64_t per = (int64_t)bc.nbx * bc.nby;
+      const int nz = bc.b.hi[2] - bc.b.lo[2] + 1;
+      bc.zc = (int)std::min<int64_t>(32, std::max<int64_t>(1, (int64_t)nz * per / tgt_rp()));
+      if (f.g.D == 3)
+        hipLaunchKernelGGL((k_inc_restrict2<T, 3>), zt_grid(bc), kBlock, 0, st, f.g, c.g, bc, lr(l), f.r, f.x,
+                           (const T*)fe, lr(l + 1), c.r, c.x, c.e);
+      else
+        hipLaunchKernelGGL((k_inc_restrict2<T, 2>), zt_grid(bc), kBlock, 0, st, f.g, c.g, bc, lr(l), f.r, f.x,
+                           (const T*)fe, lr(l + 1), c.r, c.x, c.e);
+    } else if (f.cmask == full) {
       // fine x,y tile ranges, coarse z planes (the fine inside extents are even here)
       Box b = box_inside(f.g);
       if (f.g.D == 3) { b.lo[2] = 1; b.hi[2] = c.g.N[2] - 2; }
       ZT bc = make_zt(b, tgt_rp());
       if (f.g.D == 3)
         hipLaunchKernelGGL((k_inc_restrict_full<T, 3>), zt_grid(bc), kBlock, 0, st, f.g, c.g, bc, lr(l), f.r, f.x,
-                           f.e, lr(l + 1), c.r, c.x, c.e);
+                           (const T*)fe, lr(l + 1), c.r, c.x, c.e);
       else
         hipLaunchKernelGGL((k_inc_restrict_full<T, 2>), zt_grid(bc), kBlock, 0, st, f.g, c.g, bc, lr(l), f.r, f.x,
-                           f.e, lr(l + 1), c.r, c.x, c.e);
+                           (const T*)fe, lr(l + 1), c.r, c.x, c.e);
     } else {
       ZT bc = make_zt(box_inside(c.g), tgt_rp());
-      hipLaunchKernelGGL(k_inc_restrict<T>, zt_grid(bc), kBlock, 0, st, f.g, c.g, bc, f.cmask, lr(l), f.r, f.x, f.e,
-                         lr(l + 1), c.r, c.x, c.e);
+      hipLaunchKernelGGL(k_inc_restrict<T>, zt_grid(bc), kBlock, 0, st, f.g, c.g, bc, f.cmask, lr(l), f.r, f.x,
+                         (const T*)fe, lr(l + 1), c.r, c.x, c.e);
     }
     KCK();
   }
@@ -600,9 +674,24 @@ struct MG {
   // being launched; its r update is dead (the next V-cycle's restrict! overwrites r and x there).
   void prolong_level(int c, const T* w, bool fold = false) {
     Level<T>& f = lv[c - 1];
-    ZT bf = make_zt(box_inside(f.g), tgt_rp());
+    // fine r and x read and written, ϵ written, the coarse x (and ϵ when folded) read
     ProfScope ps(P, st, "prolong_inc", c - 1,
-                 (double)sizeof(T) * ((f.g.D + 5) * ncells_inside(f.g) + (fold ? 2 : 1) * ncells_inside(lv[c].g)));
+                 (double)sizeof(T) * (5 * ncells_inside(f.g) + (fold ? 2 : 1) * ncells_inside(lv[c].g)));
+    if (pairs_ok(c - 1) && f.cmask == (1 << f.g.D) - 1 && perdir == 0) {
+      ZT bp = make_zt(box_pairs(f.g), tgt_rp());
+#define WL_PRO2(DD, CI)                                                                                      \
+  hipLaunchKernelGGL((k_prolong_inc2<T, DD, CI>), zt_grid(bp), kBlock, 0, st, f.g, lv[c].g, bp, lr(c - 1), f.r, f.x, \
+                     f.e, (const T*)lv[c].x, w, (const T*)lv[c].e)
+      if (f.g.D == 3) {
+        if (fold) WL_PRO2(3, true); else WL_PRO2(3, false);
+      } else {
+        if (fold) WL_PRO2(2, true); else WL_PRO2(2, false);
+      }
+#undef WL_PRO2
+      KCK();
+      return;
+    }
+    ZT bf = make_zt(box_inside(f.g), tgt_rp());
     if (fold)
       hipLaunchKernelGGL((k_prolong_inc<T, true>), zt_grid(bf), kBlock, 0, st, f.g, lv[c].g, bf, f.cmask, perdir,
                          lr(c - 1), f.r, f.x, f.e, (const T*)lv[c].x, w, (const T*)lv[c].e);
@@ -687,7 +776,7 @@ struct MG {
   // (src/MultiLevelPoisson.jl:113-121).  `first`: ϵ(0) = r·iD is already set by residual!.
   void iteration(bool first) {
     const T* w = (const T*)B->tsc + 3;
-    if (!first) eps_init(0);
+    if (!first && !ej_written) eps_init(0);
     vcycle(w);
     gs_sweeps(0);
     increment(0, w, true, 3);  // L₂, then the ω update in the same single-block finish
@@ -781,7 +870,7 @@ struct MG {
     while (n < itmx) {
       {
         ProfScope ps(P, st, "mixed_prep", 0, (double)sizeof(T) * (p.g.D + 4) * ncells_inside(p.g));
-        hipLaunchKernelGGL(k_mixed_prep<T>, zt_grid(bi), kBlock, 0, st, p.g, bi, lr(0), p.r, p.e, p.x,
+        hipLaunchKernelGGL(k_mixed_prep<T>, zt_grid(bi), kBlock, 0, st, p.g, bi, lr(0), p.r, jac(0), p.x,
                            (const T*)B->tsc);
         KCK();
       }
@@ -923,6 +1012,10 @@ struct FlowBase {
   int net_flags = 0;
   std::vector<double> warm_hist;  // Σr² before and after the guess, per projection (WL_WARM_STATS)
   virtual void check_net(const nn::WarmNet& n) = 0;
+  // udf! hook (wl_flow_set_udf): called from mom_step! between conv_diff! and accelerate!
+  wl_udf_fn udf = nullptr;
+  void* udf_user = nullptr;
+  wl_flow_t handle = nullptr;  // the C handle the hook receives
 };
 
 static DBodies to_dbodies(const wl_body* b, int nb) {
@@ -1118,6 +1211,7 @@ struct FlowT : FlowBase {
   // or the caller's field for this phase (0 predictor, 1 corrector)
   Accel accel(double t, int phase = 0) const {
     Accel acc;
+    acc.raw = 0;
     acc.field = use_acc[phase] ? (const void*)fn_acc[phase] : nullptr;
     acc.on = acc.field ? 0 : (desc.has_g || desc.has_uBC_rate);
     for (int i = 0; i < 3; ++i) {
@@ -1132,11 +1226,12 @@ struct FlowT : FlowBase {
   template <int S, int DD>
   void conv_bdim(bool pred, T dt, double t) {
     const ZT ba = conv_zt();
-    const Accel acc = accel(t, pred ? 0 : 1);
+    Accel acc = accel(t, pred ? 0 : 1);
+    acc.raw = udf != nullptr;
     const T nu = (T)desc.nu;
     const dim3 cblk(64, CBY, 1);
     {
-      ProfScope ps(&prof, st, pred ? "conv_pred" : "conv_corr", 0, (double)sizeof(T) * (pred ? 3 : 4) * D * g.n);
+      ProfScope ps(&prof, st, pred ? "conv_pred" : "conv_corr", 0, (double)sizeof(T) * (pred ? 2 : 3) * D * g.n);
       if (desc.perdir_mask) {
         if (pred)
           hipLaunchKernelGGL((k_conv_bdim1<T, S, true, DD>), zt_grid(ba), cblk, 0, st, g, ba, (const T*)u, (const T*)u0, f,
@@ -1164,10 +1259,43 @@ struct FlowT : FlowBase {
     // u⁰ .= u (src/Flow.jl:157) as a buffer swap: the old u becomes u⁰; the new u is written on
     // inside by the predictor's BDIM!, and on every ghost by the BC! (and exitBC!) that follow.
     if (pred) std::swap(u, u0);
+    if (udf) {
+      run_udf(pred, dt, t);
+      return;
+    }
     ZT bi = make_zt(box_inside(g), 2 * zt_target_default());
-    ProfScope ps(&prof, st, "bdim2", 0, (double)sizeof(T) * (5 * D + D * D) * ncells_inside(g));
+    ProfScope ps(&prof, st, "bdim2", 0, (double)sizeof(T) * 3 * D * ncells_inside(g));
     if (pred)
       hipLaunchKernelGGL((k_bdim2<T, true>), zt_grid(bi), kBlock, 0, st, g, bi, u, (const T*)u0, (const T*)f,
+                         (const T*)V, mg.lr(0), (const T*)mu1, (const uint8_t*)rows, (const uint8_t*)vrows());
+    else
+      hipLaunchKernelGGL((k_bdim2<T, false>), zt_grid(bi), kBlock, 0, st, g, bi, u, (const T*)u0, (const T*)f,
+                         (const T*)V, mg.lr(0), (const T*)mu1, (const uint8_t*)rows, (const uint8_t*)vrows());
+    KCK();
+  }
+  // A udf! (src/Flow.jl:193,207,255-257) between conv_diff! and accelerate!: the flow holds the
+  // reference's state at that point (predictor: u⁰ = the step's start, u = u⁰ scaled by 0 on inside
+  // by scale_u!(a,0) (:157); f = conv_diff!'s result), the caller's function runs on the host and may
+  // download and upload any field, then accelerate!, BDIM! (and scale_u!(a,½) in the corrector).
+  void run_udf(bool pred, T dt, double t) {
+    if (pred) {
+      HIPCK(hipMemcpyAsync(u, u0, g.n * D * sizeof(T), hipMemcpyDeviceToDevice, st));
+      Box bi = box_inside(g);
+      hipLaunchKernelGGL(k_scale_u<T>, grid_of(bi), kBlock, 0, st, g, bi, u, T(0));
+      KCK();
+    }
+    sync();
+    const int rc = udf(udf_user, handle, pred ? 0 : 1, t);
+    if (rc != 0) throw WlErr(WL_EINVAL, "the user-defined function (udf) returned an error");
+    Accel acc = accel(t, pred ? 0 : 1);
+    Box ba = box_all(g);
+    hipLaunchKernelGGL(k_bdim1<T>, grid_of(ba), kBlock, 0, st, g, ba, (const T*)u0, f, (const T*)V, dt, acc);
+    KCK();
+    if (mu1_dirty) mu1_rows();  // the udf may have uploaded μ₁ or V
+    mg.ensure_lrows();
+    ZT bi = make_zt(box_inside(g), 2 * zt_target_default());
+    if (pred)
+      hipLaunchKernelGGL((k_bdim2<T, true, true>), zt_grid(bi), kBlock, 0, st, g, bi, u, (const T*)u0, (const T*)f,
                          (const T*)V, mg.lr(0), (const T*)mu1, (const uint8_t*)rows, (const uint8_t*)vrows());
     else
       hipLaunchKernelGGL((k_bdim2<T, false>), zt_grid(bi), kBlock, 0, st, g, bi, u, (const T*)u0, (const T*)f,
@@ -1221,7 +1349,7 @@ struct FlowT : FlowBase {
     mg.perbc(g, xw);
     ZT ba = make_zt(box_all(g), 2 * zt_target_default());
     {
-    ProfScope ps(&prof, st, "correct", 0, (double)sizeof(T) * (2 * g.n + 3 * D * ncells_inside(g)));
+    ProfScope ps(&prof, st, "correct", 0, (double)sizeof(T) * (2 * g.n + 2 * D * ncells_inside(g)));
     hipLaunchKernelGGL(k_correct<T>, zt_grid(ba), kBlock, 0, st, g, ba, u, mg.lr(0), (const T*)xw, p, dt);
     KCK();
     }
@@ -1268,9 +1396,9 @@ struct FlowT : FlowBase {
     const int64_t nb = zt_blocks(bi);
     {
     // with ϵ = r·iD and the Σr² partials (at partial + nb) as submean would form them unflagged
-    ProfScope ps(&prof, st, "residual", 0, (double)sizeof(T) * (2 * D + 5) * ncells_inside(g));
+    ProfScope ps(&prof, st, "residual", 0, (double)sizeof(T) * (D + 5) * ncells_inside(g));
     hipLaunchKernelGGL((k_residual<T, true, true>), zt_grid(bi), kBlock, 0, st, g, bi, mg.lr(0), (const T*)p, xw, dt,
-                       1, sigma, (const T*)u, l0.r, B.partial, l0.e, nb);
+                       1, sigma, (const T*)u, l0.r, B.partial, mg.jac(0), nb);
     KCK();
     }
     mg.perbc(g, xw);
@@ -1284,7 +1412,7 @@ struct FlowT : FlowBase {
     if (use_net && (net_flags & WL_WARM_RECORD)) net->record<T>(g, (const T*)xw, st);
     ZT ba = make_zt(box_all(g), 2 * zt_target_default());
     {
-    ProfScope ps(&prof, st, "correct", 0, (double)sizeof(T) * (2 * g.n + 3 * D * ncells_inside(g)));
+    ProfScope ps(&prof, st, "correct", 0, (double)sizeof(T) * (2 * g.n + 2 * D * ncells_inside(g)));
     hipLaunchKernelGGL(k_correct<T>, zt_grid(ba), kBlock, 0, st, g, ba, u, mg.lr(0), (const T*)xw, p, dt);
     KCK();
     }
@@ -2029,6 +2157,7 @@ int wl_flow_create(wl_flow_t* out, const wl_flow_desc* d) {
     if (d->dtype == WL_F64) h->impl.reset(new FlowT<double>(*d));
     else if (d->dtype == WL_F32) h->impl.reset(new FlowT<float>(*d));
     else throw WlErr(WL_EINVAL, "dtype must be WL_F32 or WL_F64");
+    h->impl->handle = h;
   } catch (...) {
     delete h;
     throw;
@@ -2269,6 +2398,14 @@ int wl_flow_set_tiles(wl_flow_t h, int32_t gs, int32_t rp) {
   if (!h) throw WlErr(WL_EINVAL, "null handle");
   if (gs < 0 || rp < 0 || gs > (1 << 20) || rp > (1 << 20)) throw WlErr(WL_EINVAL, "tile targets must be in 0..2^20");
   h->impl->set_tiles(gs, rp);
+  API_END
+}
+int wl_flow_set_udf(wl_flow_t h, wl_udf_fn fn, void* user) {
+  API_BEGIN
+  if (!h) throw WlErr(WL_EINVAL, "null handle");
+  h->impl->sync();
+  h->impl->udf = fn;
+  h->impl->udf_user = fn ? user : nullptr;
   API_END
 }
 int wl_flow_cfl(wl_flow_t h, double* dt) {
