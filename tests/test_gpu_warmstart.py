@@ -116,3 +116,51 @@ def test_trained_network_on_held_out_flow(wl):
         assert len(r2) == n1 + n2 + 2
         assert (r2[n1] < 1e-4 or n1 == 32) and (r2[-1] < 1e-4 or n2 == 32)
     say(f"  V-cycles with the guess: {sim.pois.n[n0:]}")
+
+
+def test_zero_network_with_mixed_solver_is_bit_identical(wl):
+    """The warm start with the mixed-precision solver: a zero guess enters as the loop's zeroth
+    correction, x64 + 0, so the steps equal the mixed-precision steps without a network."""
+    say("case: zero network + mixed solver vs mixed solver, 64x32x32 sphere, 3 steps")
+    a, b = sphere(wl), sphere(wl)
+    for s in (a, b):
+        s.pois.set_solver(mode=wl.WL_SOLVER_MIXED)
+    net = wl.WarmNet.for_flow(b.flow, replay_pairs=8)
+    net.weights = np.zeros(net.nweights, np.float32)
+    wl.set_warmstart(b, net, guess=True, record=True, stats=True)
+    for _ in range(3):
+        wl.sim_step(a, remeasure=False)
+        wl.sim_step(b, remeasure=False)
+    assert np.array_equal(a.flow.u, b.flow.u) and np.array_equal(a.flow.p, b.flow.p)
+    assert a.pois.n == b.pois.n and a.flow.Δt == b.flow.Δt
+    st = wl.warm_stats(b)
+    assert st.shape == (6, 2) and np.array_equal(st[:, 0], st[:, 1])
+    assert net.pairs == 6
+
+
+def test_learned_mixed_force_vs_f64_oracle(wl):
+    """The learned warm start (the committed network) with the mixed-precision solver on a
+    held-out random-ellipsoid flow (seed 9, 256x128x128): 10 steps from the flow's start, every
+    projection ends with Σr² < 1e-4 (or itmx), and the pressure force stays within 1e-3 (relative to
+    its largest component) of the Float64 oracle's at every step."""
+    import os
+    import wl_oracle as O
+    say("case: learned + mixed, held-out 256x128x128 flow (seed 9), 10 steps vs the Float64 oracle")
+    path = os.path.join(os.path.dirname(os.path.dirname(__file__)), "waterlily.jl_amd", "data", "warmnet_c4.npz")
+    sim = wl.learned.random_ellipsoid_simulation(9)
+    sim.pois.set_solver(mode=wl.WL_SOLVER_MIXED)
+    net = wl.WarmNet.load(path, grid=wl.learned.net_grid(sim.flow), replay_pairs=1)
+    wl.set_warmstart(sim, net, guess=True)
+    bodies = [{"kind": 2, "center": b.center, "axes": b.axes} for b in sim.body.parts()]
+    f = sim.flow
+    o64 = O.OSimulation(f.N, (1, 0, 0), sim.L, nu=float(f.ν), dtype=O.WL_F64, bodies=bodies)
+    worst = 0.0
+    for k in range(10):
+        wl.sim_step(sim, remeasure=False)
+        o64.step(remeasure=False)
+        r2, n1 = sim.flow.residuals(), sim.pois.n[-2]
+        assert (r2[n1] < 1e-4 or n1 == 32) and (r2[-1] < 1e-4 or sim.pois.n[-1] == 32)
+        F, Fo = wl.pressure_force(sim), o64.pressure_force()
+        worst = max(worst, float(np.max(np.abs(F - Fo)) / max(np.max(np.abs(Fo)), 1e-12)))
+    say(f"  V-cycles {sim.pois.n}; oracle {o64.n}; worst relative force deviation {worst:.3e}")
+    assert worst <= 1e-3

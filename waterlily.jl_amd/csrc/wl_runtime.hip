@@ -1343,10 +1343,13 @@ struct FlowT : FlowBase {
     }
     mg.perbc(g, mg.x64[0]);
     mg.finish_sum2(nb);
+    const bool use_net = net && (!(net_flags & WL_WARM_PREDICTOR) || w == T(1));
+    if (use_net) warm_guess_mixed(bi, nb);
     const int nit = mg.mixed_loop(tol, itmx, &r2hist);
     hipLaunchKernelGGL(k_round_x<T>, zt_grid(bi), kBlock, 0, st, g, bi, (const double*)mg.x64[mg.x64cur], xw);
     KCK();
     mg.perbc(g, xw);
+    if (use_net && (net_flags & WL_WARM_RECORD)) net->record<T>(g, (const T*)xw, st);
     ZT ba = make_zt(box_all(g), 2 * zt_target_default());
     {
     ProfScope ps(&prof, st, "correct", 0, (double)sizeof(T) * (2 * g.n + 2 * D * ncells_inside(g)));
@@ -1383,6 +1386,38 @@ struct FlowT : FlowBase {
         net->apply<T>(g, xw, st);
         mg.residual_generic();
       }
+    }
+    if (net_flags & WL_WARM_STATS) warm_hist.push_back((double)(T)mg.read_dsc(1));
+  }
+
+  // The learned warm start with the mixed-precision solver: the network sees the FP32 defect and
+  // the warm start rounded to T; its guess enters as the loop's zeroth correction, x64 += guess,
+  // with the FP64 defect formed again, so the loop still iterates on the FP64 Σr².
+  void warm_guess_mixed(const ZT& bi, int64_t nb) {
+    Level<T>& l0 = mg.lv[0];
+    if (net_flags & WL_WARM_STATS) warm_hist.push_back((double)(T)mg.read_dsc(1));
+    hipLaunchKernelGGL(k_round_x<T>, zt_grid(bi), kBlock, 0, st, g, bi, (const double*)mg.x64[mg.x64cur], xw);
+    KCK();
+    {
+      ProfScope ps(&prof, st, "nn_prep", 0, 0.0);
+      net->prep<T>(g, (const T*)l0.r, (const T*)xw, (const T*)mu0, st);
+    }
+    if (net_flags & WL_WARM_GUESS) {
+      {
+        ProfScope ps(&prof, st, "nn_forward", 0, 0.0);
+        net->forward(st);
+      }
+      ProfScope ps(&prof, st, "nn_apply", 0, 0.0);
+      fill(l0.x, l0.g.n, T(0));
+      net->apply<T>(g, l0.x, st);
+      hipLaunchKernelGGL((k_residual64<T, false, false>), zt_grid(bi), kBlock, 0, st, g, bi, mg.lr(0),
+                         (const T*)nullptr, T(0), sigma, (const T*)nullptr, (const double*)mg.x64[mg.x64cur],
+                         (const T*)l0.x, mg.x64[1 - mg.x64cur], l0.r, B.partial, nb, (const T*)nullptr,
+                         (const T*)nullptr);
+      KCK();
+      mg.x64cur = 1 - mg.x64cur;
+      mg.perbc(g, mg.x64[mg.x64cur]);
+      mg.finish_sum2(nb);
     }
     if (net_flags & WL_WARM_STATS) warm_hist.push_back((double)(T)mg.read_dsc(1));
   }
@@ -2208,7 +2243,6 @@ int wl_flow_pois_update(wl_flow_t h) {
 int wl_flow_set_solver(wl_flow_t h, int mode, double tol, int itmx) {
   API_BEGIN
   if (mode != WL_SOLVER_PLAIN && mode != WL_SOLVER_MIXED) throw WlErr(WL_EINVAL, "unknown solver mode");
-  if (mode != WL_SOLVER_PLAIN && h->impl->net) throw WlErr(WL_EINVAL, "the warm start runs with the plain solver");
   if (itmx < 1) throw WlErr(WL_EINVAL, "itmx must be >= 1");
   h->impl->mode = mode;
   h->impl->tol = tol;
@@ -2748,7 +2782,6 @@ int wl_flow_set_warmstart(wl_flow_t h, wl_net_t net, int flags) {
   if (net) {
     if (flags & ~(WL_WARM_GUESS | WL_WARM_RECORD | WL_WARM_STATS | WL_WARM_PREDICTOR))
       throw WlErr(WL_EINVAL, "unknown warm-start flags");
-    if (f->mode != WL_SOLVER_PLAIN) throw WlErr(WL_EINVAL, "the warm start runs with the plain solver");
     f->check_net(*net->impl);
   }
   net_detach(f);
