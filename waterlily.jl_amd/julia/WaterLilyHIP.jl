@@ -20,7 +20,11 @@ the GPU and keeps `Δt` and `pois.n` on the host exactly as `Flow`/`MultiLevelPo
   measured, and their forces and moments integrated, on the device.
 - Boundary values and forcing: tuple `uBC` and `g = nothing` run entirely on the device;
   function-valued `uBC(i,x,t)` and `g(i,x,t)` are evaluated on the host each step by WaterLily's
-  own `apply!`/`accelerate!` and uploaded (src/Flow.jl:69-73,138-141).  `udf` is not supported.
+  own `apply!`/`accelerate!` and uploaded (src/Flow.jl:69-73,138-141).
+- `udf`: `sim_step!(sim; udf, kw...)` runs the user function through `wl_flow_set_udf` between
+  `conv_diff!` and `accelerate!` of both phases, on the host `Flow` mirror whose fields are
+  downloaded before and uploaded after the call (src/Flow.jl:193,207,255-257).
+- `u0` (and its deprecated alias `uλ`): the initial velocity, evaluated with WaterLily's `apply!`.
 - Fields read through `flow.u`, `flow.p`, ... are host copies (downloads); `flow.p = A` uploads.
   In-place writes into such a copy (`copyto!(flow.p, A)`) do not reach the device.
 - `HipMeanFlow`, `HipBatch`, `solver_mode!` and `CFL` mirror `MeanFlow` (src/Metrics.jl:199-261),
@@ -75,14 +79,15 @@ mutable struct HipFlow{D,T} <: AbstractFlow{D,T}
 end
 
 """
-    HipFlow(N, uBC; Δt, ν, g, λ, T, perdir, exitBC, u0, mem)
+    HipFlow(N, uBC; Δt, ν, g, λ, T, perdir, exitBC, u0, uλ, mem)
 
-`flow_ctor` for `Simulation`: same keywords as `Flow` (src/Flow.jl:133-134).  With a function
-`uBC` the initial velocity is `uBC(i,x,0)` followed by `BC!` and `exitBC!` (src/Flow.jl:138-141).
+`flow_ctor` for `Simulation`: same keywords as `Flow` (src/Flow.jl:133-134).  The initial velocity
+is `u0` (a function `u0(i,x)` or a tuple; `uλ` is its deprecated alias), by default `uBC` at t = 0,
+followed by `BC!` and `exitBC!` (src/Flow.jl:135-141).
 """
 function HipFlow(N::NTuple{D}, uBC; Δt=0.25, ν=0., g=nothing, λ=WaterLily.quick, T=Float32,
-                 perdir=(), exitBC=false, u0=nothing, mem=Array, kw...) where D
-    isnothing(u0) || throw(ArgumentError("HipFlow: the device path starts from u = uBC"))
+                 perdir=(), exitBC=false, u0=nothing, uλ=nothing, mem=Array, kw...) where D
+    u0 = WaterLily.ic_kwarg(u0, uλ)
     isnothing(g) || g isa Function || throw(ArgumentError("HipFlow: g must be a function g(i,x,t) or nothing"))
     Ut = uBC isa Function ? ntuple(_ -> 0.0, D) : uBC
     desc = WlFlowDesc(Int32(D), ntuple(i -> i <= D ? Int32(N[i]) : Int32(0), 3), T == Float64 ? 1 : 0,
@@ -93,10 +98,11 @@ function HipFlow(N::NTuple{D}, uBC; Δt=0.25, ν=0., g=nothing, λ=WaterLily.qui
     host = Flow(N, uBC; Δt, ν, g, λ, T, perdir, exitBC)
     f = HipFlow{D,T}(h[], Tuple(N), T[Δt], T(ν), uBC, g, perdir, exitBC, host)
     finalizer(f -> ccall((:wl_flow_destroy, libwl), Cint, (Ptr{Cvoid},), f.h), f)
-    if uBC isa Function  # u = uBC(i,x,0), then BC! and exitBC! on the device
-        ic = field_of((i, x) -> uBC(i, x, zero(T)), f)
+    if uBC isa Function || !isnothing(u0)  # u = u0 (default uBC(i,x,0)), then BC! and exitBC!
+        ubc0 = uBC isa Function ? field_of((i, x) -> uBC(i, x, zero(T)), f) : nothing
+        ic = isnothing(u0) ? ubc0 : u0 isa Function ? field_of(u0, f) : field_of((i, x) -> u0[i], f)
         upload!(f, :u, ic)
-        set_field!(f, FN_UBC, ic)
+        isnothing(ubc0) || set_field!(f, FN_UBC, ubc0)
         check(ccall((:wl_flow_apply_ic, libwl), Cint, (Ptr{Cvoid},), f.h))
     end
     f
@@ -222,19 +228,54 @@ function functions_for_step!(a::HipFlow{D,T}, t₀, t₁) where {D,T}
     end
 end
 
+# udf! through the C ABI: the library calls back (wl_udf_fn) between conv_diff! and accelerate!;
+# the trampoline runs WaterLily's own udf! (src/Flow.jl:255-257) on the host Flow mirror, with the
+# device fields copied in before and out after, and carries a Julia exception back out.
+const UDF_CTX = Ref{Any}(nothing)
+const UDF_FIELDS = (:u, :u⁰, :f, :p, :σ)
+function udf_trampoline(user::Ptr{Cvoid}, h::Ptr{Cvoid}, phase::Cint, t::Float64)::Cint
+    a, udf, kw = UDF_CTX[]
+    try
+        host = a.host
+        for s in UDF_FIELDS
+            copyto!(getfield(host, s), download(a, s))
+        end
+        WaterLily.udf!(host, udf, phase == 0 ? host.u⁰ : host.u, eltype(a.Δt)(t); kw...)
+        for s in UDF_FIELDS
+            upload!(a, s, getfield(host, s))
+        end
+        return Cint(0)
+    catch e
+        UDF_CTX[] = (a, udf, kw, e)
+        return Cint(-1)
+    end
+end
+const UDF_C = Ref{Ptr{Cvoid}}(C_NULL)
+__init__() = (UDF_C[] = @cfunction(udf_trampoline, Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Cint, Float64)))
+
 """
-    mom_step!(a::HipFlow, b::HipPoisson; udf=nothing)
+    mom_step!(a::HipFlow, b::HipPoisson; udf=nothing, kw...)
 
 src/Flow.jl:156-167 on the device: t₁ = sum(Δt), t₀ = t₁-Δt[end]; the CFL Δt is pushed onto
-a.Δt and the two V-cycle counts onto b.n.
+a.Δt and the two V-cycle counts onto b.n; `udf(flow, [u,] t; kw...)` runs in both phases.
 """
 function mom_step!(a::HipFlow{D,T}, b::HipPoisson; udf=nothing, kw...) where {D,T}
-    isnothing(udf) || throw(ArgumentError("HipFlow: user-defined functions cannot run on the device"))
     t₁ = sum(a.Δt); t₀ = t₁ - a.Δt[end]
     functions_for_step!(a, t₀, t₁)
     dt = Ref{Float64}(0); n = zeros(Int16, 2)
-    check(ccall((:wl_mom_step, libwl), Cint, (Ptr{Cvoid}, Float64, Float64, Float64, Ptr{Float64}, Ptr{Int16}),
-                a.h, a.Δt[end], t₀, t₁, dt, n))
+    isnothing(udf) || (UDF_CTX[] = (a, udf, kw);
+                       check(ccall((:wl_flow_set_udf, libwl), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}),
+                                   a.h, UDF_C[], C_NULL)))
+    rc = try
+        ccall((:wl_mom_step, libwl), Cint, (Ptr{Cvoid}, Float64, Float64, Float64, Ptr{Float64}, Ptr{Int16}),
+              a.h, a.Δt[end], t₀, t₁, dt, n)
+    finally
+        isnothing(udf) || ccall((:wl_flow_set_udf, libwl), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}),
+                                a.h, C_NULL, C_NULL)
+    end
+    ctx = UDF_CTX[]; UDF_CTX[] = nothing
+    ctx isa Tuple && length(ctx) == 4 && throw(ctx[4])
+    check(rc)
     push!(a.Δt, T(dt[])); append!(b.n, n)
 end
 
