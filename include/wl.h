@@ -118,6 +118,15 @@ int wl_pressure_force(wl_flow_t h, const wl_body* bodies, int nbodies, double t,
  * any output may be NULL.  Replaces the Julia functions of the same names (Metrics.jl:146-197). */
 int wl_flow_metrics(wl_flow_t h, const wl_body* bodies, int nbodies, double t, const double* x0, double* Fp,
                     double* Fv, double* Mp, double* Mv);
+/* wl_flow_metrics with no device allocation.  No counterpart in the reference: it is for callers
+ * that cannot allocate device memory at call time (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc), where wl_flow_metrics allocates its block-partial buffer
+ * on its first call.  Same arguments and results, bit for bit.  The partials go to a static device
+ * array of 12 MiB per device (flows of at most 131072 blocks of 256 inside cells, e.g. 512×256×256;
+ * larger flows return WL_EINVAL).  All calls on one device share that array: calls that overlap on
+ * one device race. */
+int wl_flow_metrics_noalloc(wl_flow_t h, const wl_body* bodies, int nbodies, double t, const double* x0,
+                            double* Fp, double* Fv, double* Mp, double* Mv);
 
 /* MeanFlow (src/Metrics.jl:199-261): running averages P, U (and UU = u⊗u with uu_stats) on the
  * flow's device.  The time vector t stays with the caller, which forms each update's weight

@@ -62,6 +62,7 @@ SIGNATURES = {
     "wl_mom_step": (_i, [_vp, _d, _d, _d, _P(_d), _P(C.c_int16)]),
     "wl_pressure_force": (_i, [_vp, _P(WlBody), _i, _d, _P(_d)]),
     "wl_flow_metrics": (_i, [_vp, _P(WlBody), _i, _d, _P(_d), _P(_d), _P(_d), _P(_d), _P(_d)]),
+    "wl_flow_metrics_noalloc": (_i, [_vp, _P(WlBody), _i, _d, _P(_d), _P(_d), _P(_d), _P(_d), _P(_d)]),
     "wl_meanflow_create": (_i, [_P(_vp), _vp, _i]),
     "wl_meanflow_destroy": (_i, [_vp]),
     "wl_meanflow_update": (_i, [_vp, _vp, _d]),

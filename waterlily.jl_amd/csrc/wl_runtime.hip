@@ -13,6 +13,7 @@
 #include <algorithm>
 #include <cstdlib>
 #include <limits>
+#include <mutex>
 #include "wl_kernels.h"
 
 using namespace wl;
@@ -993,6 +994,11 @@ struct FlowBase {
   virtual int step(double dt, double t0, double t1, double* dt_next, int16_t* nit) = 0;
   virtual void pressure_force(const wl_body* b, int nb, double t, double* F) = 0;
   virtual void metrics(const wl_body* b, int nb, const double* x0, double* out) = 0;
+  // the same integrals with the block partials in a caller-given device buffer `part` of at
+  // least 12·metrics_blocks() + 12 doubles (wl_flow_metrics_noalloc)
+  virtual int64_t metrics_blocks() const = 0;
+  virtual void metrics_into(const wl_body* b, int nb, const double* x0, double* out, double* part) = 0;
+  int dev = 0;  // the device current when the handle was created (its buffers and stream are there)
   virtual hipStream_t stream() const = 0;
   virtual void sync() = 0;
   virtual Prof& profiler() = 0;
@@ -1049,6 +1055,7 @@ struct FlowT : FlowBase {
     int Ng[3] = {1, 1, 1};
     for (int k = 0; k < D; ++k) Ng[k] = d.n[k] + 2;
     g = make_geo(D, Ng);
+    HIPCK(hipGetDevice(&dev));
     HIPCK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
     const int64_t n = g.n;
     u = dalloc<T>(n * D); u0 = dalloc<T>(n * D); f = dalloc<T>(n * D); V = dalloc<T>(n * D);
@@ -1529,21 +1536,32 @@ struct FlowT : FlowBase {
   int64_t mpart_n = 0;
   void metrics(const wl_body* b, int nb, const double* x0, double* out) override {
     DBodies bs = to_dbodies(b, nb);
-    Box bi = box_inside(g);
-    const int64_t np = nblocks(bi);
+    const int64_t np = metrics_blocks();
     if (mpart_n < 12 * np + 12) {
       dfree(mpart);
       mpart = dalloc<double>(12 * np + 12);
       mpart_n = 12 * np + 12;
     }
+    metrics_run(bs, x0, out, mpart);
+  }
+  int64_t metrics_blocks() const override { return nblocks(box_inside(g)); }
+  void metrics_into(const wl_body* b, int nb, const double* x0, double* out, double* part) override {
+    DBodies bs = to_dbodies(b, nb);
+    metrics_run(bs, x0, out, part);
+  }
+  // k_body_metrics writes 12 rows of np block partials into part, k_finish_rows sums each row
+  // into part[12·np ..]; allocates nothing
+  void metrics_run(const DBodies& bs, const double* x0, double* out, double* part) {
+    Box bi = box_inside(g);
+    const int64_t np = nblocks(bi);
     X0 x;
     for (int k = 0; k < 3; ++k) x.v[k] = k < D ? x0[k] : 0.0;
     hipLaunchKernelGGL(k_body_metrics<T>, grid_of(bi), kBlock, 0, st, g, bi, bs, (const T*)p, (const T*)u,
-                       (T)desc.nu, x, mpart, np);
+                       (T)desc.nu, x, part, np);
     KCK();
-    hipLaunchKernelGGL(k_finish_rows, dim3(12), dim3(256), 0, st, (const double*)mpart, np, np, mpart + 12 * np);
+    hipLaunchKernelGGL(k_finish_rows, dim3(12), dim3(256), 0, st, (const double*)part, np, np, part + 12 * np);
     KCK();
-    HIPCK(hipMemcpyAsync(out, mpart + 12 * np, 12 * sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCK(hipMemcpyAsync(out, part + 12 * np, 12 * sizeof(double), hipMemcpyDeviceToHost, st));
     HIPCK(hipStreamSynchronize(st));
   }
   hipStream_t stream() const override { return st; }
@@ -2167,6 +2185,38 @@ struct wl_pois_s {
   std::unique_ptr<PoisBase> impl;
 };
 
+// ---------------------------------------------------------------------------------------------
+// Scratch of wl_flow_metrics_noalloc.  The block partials of the four body integrals live in this
+// file-scope device array instead of a buffer the library allocates: 12 rows of at most
+// kNoAllocBlocks block sums (blocks of 256 inside cells) plus the 12 results, 12 MiB + 96 B per
+// device, held from the library's load until the process ends.  Every call on one device shares
+// it, so calls that overlap on one device race.  Host code never passes `scratch` itself (its
+// name is a host-side stand-in there): it uses the address hipGetSymbolAddress gives for the
+// device, looked up once per device with that device current.
+// ---------------------------------------------------------------------------------------------
+constexpr int64_t kNoAllocBlocks = 131072;  // 512×256×256 inside cells
+constexpr int64_t kNoAllocDoubles = 12 * kNoAllocBlocks + 12;
+__device__ __align__(256) double scratch[kNoAllocDoubles];
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static double* g_scratch_ptr[kMaxDevices] = {};
+static double* scratch_on(int dev) {
+  if (dev < 0 || dev >= kMaxDevices) throw WlErr(WL_EINVAL, "device index out of range for the scratch table");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    int cur = 0;
+    HIPCK(hipGetDevice(&cur));
+    if (cur != dev) HIPCK(hipSetDevice(dev));
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (cur != dev) HIPCK(hipSetDevice(cur));
+    if (e != hipSuccess || !ptr)
+      throw WlErr(WL_EHIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch_ptr[dev] = (double*)ptr;
+  }
+  return g_scratch_ptr[dev];
+}
+
 // =============================================================================================
 // C ABI
 // =============================================================================================
@@ -2267,6 +2317,28 @@ int wl_flow_metrics(wl_flow_t h, const wl_body* bodies, int nbodies, double t, c
   double out[12];
   h->impl->metrics(bodies, nbodies, x0, out);
   const int D = h->impl->desc.D, nM = D == 2 ? 1 : 3;
+  if (Fp) for (int i = 0; i < D; ++i) Fp[i] = out[i];
+  if (Fv) for (int i = 0; i < D; ++i) Fv[i] = out[3 + i];
+  if (Mp) for (int i = 0; i < nM; ++i) Mp[i] = out[6 + i];
+  if (Mv) for (int i = 0; i < nM; ++i) Mv[i] = out[9 + i];
+  API_END
+}
+
+int wl_flow_metrics_noalloc(wl_flow_t h, const wl_body* bodies, int nbodies, double t, const double* x0,
+                            double* Fp, double* Fv, double* Mp, double* Mv) {
+  API_BEGIN
+  (void)t;
+  if (!h || !x0) throw WlErr(WL_EINVAL, "a flow handle and x0 are required");
+  FlowBase* f = h->impl.get();
+  const int64_t np = f->metrics_blocks();
+  if (np > kNoAllocBlocks)
+    throw WlErr(WL_EINVAL, "wl_flow_metrics_noalloc: the flow needs " + std::to_string(np) +
+                               " partial blocks of 256 inside cells; the static scratch holds " +
+                               std::to_string(kNoAllocBlocks) + " (use wl_flow_metrics)");
+  double* part = scratch_on(f->dev);
+  double out[12];
+  f->metrics_into(bodies, nbodies, x0, out, part);
+  const int D = f->desc.D, nM = D == 2 ? 1 : 3;
   if (Fp) for (int i = 0; i < D; ++i) Fp[i] = out[i];
   if (Fv) for (int i = 0; i < D; ++i) Fv[i] = out[3 + i];
   if (Mp) for (int i = 0; i < nM; ++i) Mp[i] = out[6 + i];
