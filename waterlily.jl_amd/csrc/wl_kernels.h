@@ -1308,6 +1308,212 @@ __global__ void __launch_bounds__(256) k_gs_half2(Geo g, ZT t, LRows<T> LR, cons
   }
 }
 
+// The coefficients, diagonal and inverse diagonal of inside cell I = (x, y, z) as lcell, diag_at
+// and inv_diag form them.  In a uniform row (`uni`, uniform within a wave here, so the branch does
+// not diverge) they take one of three precomputed sets: the row constants, or those with the wall
+// face of x = 1 or x = N0 − 2 zeroed, whose D and 1/D were formed once by the same operations.
+// 32-bit offsets (the level has fewer than 2³¹ cells): the loads take a scalar base and a 32-bit
+// lane offset.
+template <class T>
+struct UniCoef {
+  T D[3], iD[3];  // mid, x = 1, x = N0 − 2
+};
+// formed on the host with the operations of diag_at (3D) and inv_diag, in the same order
+template <class T>
+__host__ __device__ inline UniCoef<T> uni_coef(const T* c) {
+  UniCoef<T> u;
+  for (int k = 0; k < 3; ++k) {
+    const T l0 = k == 1 ? T(0) : c[0], h0 = k == 2 ? T(0) : c[0];
+    T s = 0;
+    s -= (l0 + h0);
+    s -= (c[1] + c[1]);
+    s = s - (c[2] + c[2]);
+    u.D[k] = s;
+    u.iD[k] = s == T(0) ? s : T(1) / s;
+  }
+  return u;
+}
+template <class T>
+struct CellCoef {
+  LC<T> L;
+  T D, iD;
+};
+// a float as an SSA value of the wave (readfirstlane): a select between such values picks a
+// register, never an address in a local aggregate
+__device__ __forceinline__ float wave_val(float v) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, v)));
+}
+__device__ __forceinline__ double wave_val(double v) {
+  const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
+  const unsigned lo = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)u);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)(u >> 32));
+  return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+template <class T>
+__device__ __forceinline__ CellCoef<T> coef_u(const Geo& g, const LRows<T>& a, T c0, T c1, T c2, T D0, T D1, T D2,
+                                              T i0, T i1, T i2, int x, uint32_t I, bool uni) {
+  CellCoef<T> c;
+  if (uni) {
+    const bool lw = x == 1, hw = x + 1 == g.N[0] - 1;
+    c.L.l[0] = lw ? T(0) : c0;
+    c.L.h[0] = hw ? T(0) : c0;
+    c.L.l[1] = c.L.h[1] = c1;
+    c.L.l[2] = c.L.h[2] = c2;
+    c.D = lw ? D1 : (hw ? D2 : D0);
+    c.iD = lw ? i1 : (hw ? i2 : i0);
+  } else {
+    const uint32_t n = (uint32_t)g.n;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      c.L.l[i] = a.L[I + i * n];
+      c.L.h[i] = a.L[I + (uint32_t)g.s[i] + i * n];
+    }
+    c.D = diag_at<T>(g, c.L, 0);
+    c.iD = inv_diag(c.D);
+  }
+  return c;
+}
+// Two GS-RB half sweeps, it and it + 1, in one pass (src/Poisson.jl:124-128,144-146; 3D, x pairs,
+// no periodic directions, even last ghosted extent): colour A (the cells sweep `it` updates) from
+// the old ϵ, then colour B from the new A.  A wave owns one row y and marches z; it forms the new
+// A values of rows y − 1, y, y + 1 itself (its B cells' y-neighbours), so no wave waits for
+// another: no LDS, no barrier.  The new A of plane z + 1 is formed before the B of plane z.  Lanes
+// 0 and 63 are a one-pair halo in x (their new A values feed their neighbours' B); each tile
+// outputs 62 pairs (the first tile 63).  ϵ is read from ein and written to eout (other waves read
+// the old values around their own rows); ghost cells are read from eg, whose ghosts the sweeps keep.
+// Each value is formed by the same operations as in the half sweeps, so results are bit-identical.
+template <class T>
+__global__ void __launch_bounds__(256) k_gs_dbl(Geo g, ZT t, LRows<T> LR, const T* __restrict__ r,
+                                                const T* __restrict__ ein, const T* __restrict__ eg,
+                                                T* __restrict__ eout, int it, UniCoef<T> ucoef) {
+  int pl, y, z0, z1;
+  zt_cell(t, pl, y, z0, z1);
+  y = __builtin_amdgcn_readfirstlane(y);  // one row per wave
+  if (y > t.b.hi[1]) return;              // a whole wave
+  const int lane = threadIdx.x;
+  const int N0 = g.N[0], N1 = g.N[1], N2 = g.N[2];
+  const int P = t.b.hi[0] + 1;  // pairs per row
+  const bool vp = pl <= P - 1;
+  const int pc = vp ? pl : P - 1;  // lanes past the row end repeat the last pair and store nothing
+  const int xa = 1 + 2 * pc;
+  const bool first_tile = pl == lane;
+  const bool out = vp && lane <= 62 && (lane >= 1 || first_tile);
+  const bool edge_r = lane == 63 || pc >= P - 1;
+  // 32-bit offsets (a level has fewer than 2³¹ cells): the loads take a scalar base and a lane offset
+  const uint32_t s1 = (uint32_t)g.s[1], s2 = (uint32_t)g.s[2];
+  const T uc0 = wave_val(LR.c[0]), uc1 = wave_val(LR.c[1]), uc2 = wave_val(LR.c[2]);
+  const T uD0 = wave_val(ucoef.D[0]), uD1 = wave_val(ucoef.D[1]), uD2 = wave_val(ucoef.D[2]);
+  const T ui0 = wave_val(ucoef.iD[0]), ui1 = wave_val(ucoef.iD[1]), ui2 = wave_val(ucoef.iD[2]);
+  // the old ϵ pair at row yy, plane z (ghost rows and planes from eg; rows or planes beyond the
+  // array, never used, give 0)
+  auto ld_eo = [&](int yy, int z) -> P2<T> {
+    if (yy < 0 || yy > N1 - 1 || z < 0 || z > N2 - 1) return P2<T>{T(0), T(0)};
+    const T* src = (yy == 0 || yy == N1 - 1 || z == 0 || z == N2 - 1) ? eg : ein;
+    return ld2(src + ((uint32_t)xa + s1 * (uint32_t)yy + s2 * (uint32_t)z));
+  };
+  // the uniform-row flag of inside row yy, plane z, as loaded (read by the wave one plane later)
+  auto ld_flag = [&](int yy, int z) -> int {
+    if (LR.rows == nullptr || yy < 1 || yy > N1 - 2 || z < 1 || z > N2 - 2) return 0;
+    return (int)LR.rows[(uint32_t)yy + (uint32_t)N1 * (uint32_t)z];
+  };
+  // colour A of row yy, plane z is xa when this is true (k = 2Iv−1−(Σfront+it)%2, src/Poisson.jl:125)
+  auto a_first = [&](int yy, int z) { return ((xa + yy + z + it + 4) & 1) == 0; };
+  // one cell of a pair by value (a select of two values, not of two addresses)
+  auto pick = [](P2<T> v, bool first) -> T {
+    const T a = v.a, b = v.b;
+    return first ? a : b;
+  };
+  // one GS update from its six neighbours (gs_cell's operations, src/Poisson.jl:116-122)
+  auto gauss = [&](int xc, uint32_t I, bool uni, T rv, T xm, T xp, T ym, T yp, T zm, T zp) -> T {
+    const CellCoef<T> c = coef_u(g, LR, uc0, uc1, uc2, uD0, uD1, uD2, ui0, ui1, ui2, xc, I, uni);
+    T sv = rv;
+    sv -= (xm * c.L.l[0] + xp * c.L.h[0]);
+    sv -= (ym * c.L.l[1] + yp * c.L.h[1]);
+    sv -= (zm * c.L.l[2] + zp * c.L.h[2]);
+    return sv * c.iD;
+  };
+  P2<T> EO[5][3];  // old ϵ of rows y − 2 .. y + 2 at planes zn − 1, zn, zn + 1
+#pragma unroll
+  for (int j = 0; j < 5; ++j) {
+    EO[j][0] = P2<T>{T(0), T(0)};
+    EO[j][1] = ld_eo(y - 2 + j, z0 - 2);
+    EO[j][2] = ld_eo(y - 2 + j, z0 - 1);
+  }
+  P2<T> AN[3], AP[3], AY2{T(0), T(0)}, RP1{T(0), T(0)};
+#pragma unroll
+  for (int j = 0; j < 3; ++j) { AN[j] = P2<T>{T(0), T(0)}; AP[j] = AN[j]; }
+  P2<T> R[3] = {{T(0), T(0)}, {T(0), T(0)}, {T(0), T(0)}};
+  int fln[3] = {ld_flag(y - 1, z0 - 1), ld_flag(y, z0 - 1), ld_flag(y + 1, z0 - 1)};
+  bool fB = false;  // the flag of row y at the plane of the B update (the previous plane's)
+  for (int zn = z0 - 1; zn <= z1 + 1; ++zn) {
+    bool fA[3];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      fA[j] = __builtin_amdgcn_readfirstlane(fln[j]) != 0;
+      fln[j] = ld_flag(y - 1 + j, zn + 1);
+    }
+#pragma unroll
+    for (int j = 0; j < 5; ++j) {
+      EO[j][0] = EO[j][1];
+      EO[j][1] = EO[j][2];
+      EO[j][2] = ld_eo(y - 2 + j, zn + 1);
+    }
+    AY2 = AP[1];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) AP[j] = AN[j];
+    RP1 = R[1];
+    // ---- colour A on rows y − 1 .. y + 1 of plane zn (half sweep it) ----
+    const bool zin = zn >= 1 && zn <= N2 - 2;
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      const int yy = y - 1 + j;
+      AN[j] = EO[j + 1][1];
+      if (zin && yy >= 1 && yy <= N1 - 2) {
+        const uint32_t I = (uint32_t)xa + s1 * (uint32_t)yy + s2 * (uint32_t)zn;
+        R[j] = ld2(r + I);
+        const P2<T> own = EO[j + 1][1];
+        const bool fa = a_first(yy, zn);
+        T xm, xp;
+        if (fa) {
+          xm = __shfl_up(own.b, 1, 64);
+          if (lane == 0) xm = (pc == 0 ? eg : ein)[I - 1];
+          xp = own.b;
+        } else {
+          xm = own.a;
+          xp = __shfl_down(own.a, 1, 64);
+          if (edge_r) xp = (pc >= P - 1 ? eg : ein)[I + 2];
+        }
+        const T nv = gauss(fa ? xa : xa + 1, fa ? I : I + 1, fA[j], pick(R[j], fa), xm, xp, pick(EO[j][1], fa),
+                           pick(EO[j + 2][1], fa), pick(EO[j + 1][0], fa), pick(EO[j + 1][2], fa));
+        if (fa) AN[j].a = nv; else AN[j].b = nv;
+      }
+    }
+    // ---- colour B on row y of plane zn − 1 (half sweep it + 1), from the new A ----
+    const int zb = zn - 1;
+    if (zb >= z0 && zb <= z1) {
+      const uint32_t I = (uint32_t)xa + s1 * (uint32_t)y + s2 * (uint32_t)zb;
+      const bool fa = a_first(y, zb);  // the B cell is the other one
+      const P2<T> own = AP[1];
+      T xm, xp;
+      if (fa) {  // B = xa + 1
+        xm = own.a;
+        xp = __shfl_down(own.a, 1, 64);
+        if (edge_r) xp = (pc >= P - 1 ? eg : ein)[I + 2];
+      } else {  // B = xa
+        xm = __shfl_up(own.b, 1, 64);
+        if (lane == 0) xm = (pc == 0 ? eg : ein)[I - 1];
+        xp = own.b;
+      }
+      const T nv = gauss(fa ? xa + 1 : xa, fa ? I + 1 : I, fB, pick(RP1, !fa), xm, xp, pick(AP[0], !fa),
+                         pick(AP[2], !fa), pick(AY2, !fa), pick(AN[1], !fa));
+      if (out) {
+        if (fa) st2(eout + I, own.a, nv); else st2(eout + I, nv, own.b);
+      }
+    }
+    fB = fA[1];
+  }
+}
+
 // increment! (src/Poisson.jl:100-104) on pairs: r -= ω·Aϵ; x += ω·ϵ, optionally the Σr² partials
 // (L₂ of the level after a smooth!, src/MultiLevelPoisson.jl:115) and, EJ, ϵ_J = r·iD of the new r
 // into ej: the ϵ the next V-cycle's Jacobi! starts from (src/Poisson.jl:112), written here instead
@@ -1370,8 +1576,9 @@ __global__ void __launch_bounds__(256) k_increment2(Geo g, ZT t, LRows<T> LR, T*
 // (xa, xa+1) = (2q−1, 2q) read coarse x cell q = p + 1 (down(I) = (I+2)÷2 in 1-based indices); the
 // x-neighbours outside the pair read coarse q − 1 and q + 1, "no value" (0) on a ghost.  The
 // arithmetic per cell is prolong_cell's; CINC folds the coarse level's last increment! in (x + ω·ϵ
-// at the coarse cells read).
-template <class T, int D, bool CINC>
+// at the coarse cells read).  EW = false skips the ϵ store (the fused smooth! that follows on
+// level 0 forms ϵ = r·iD itself).
+template <class T, int D, bool CINC, bool EW = true>
 __global__ void __launch_bounds__(256) k_prolong_inc2(Geo gf, Geo gc, ZT t, LRows<T> LR, T* __restrict__ rf,
                                                       T* __restrict__ xf, T* __restrict__ ef,
                                                       const T* __restrict__ xc, const T* __restrict__ wp,
@@ -1408,7 +1615,7 @@ __global__ void __launch_bounds__(256) k_prolong_inc2(Geo gf, Geo gc, ZT t, LRow
     if (c.vx) {
       st2(rf + I, ra, rb);
       st2(xf + I, xx.a + w * own, xx.b + w * own);
-      st2(ef + I, ra * inv_diag(Da), rb * inv_diag(Db));
+      if (EW) st2(ef + I, ra * inv_diag(Da), rb * inv_diag(Db));
     }
   }
 }
@@ -1480,6 +1687,232 @@ __global__ void __launch_bounds__(256) k_inc_restrict2(Geo gf, Geo gc, ZT t, LRo
       ec[Ic] = s * inv_diag(diag_at<T>(gc, lcell(gc, Lc, cx, cy, ccz), Ic));
     }
   }
+}
+
+// smooth!(p) of the finest level in one pass (src/Poisson.jl:141-148 with GaussSeidelRB!, 3D, no
+// periodic directions, even inside x extent and even last ghosted extent): ϵ = r·iD, the four
+// GS-RB half sweeps it = 1..4 (:116-128), then increment! r -= ω·Aϵ, x += ω·ϵ with Σr² (the L₂
+// the solver loop reads, src/MultiLevelPoisson.jl:115) and ϵ_J = r·iD of the new r (the next
+// V-cycle's Jacobi! start).  Every value is formed by the same operations in the same order as the
+// separate passes, so the results are bit-identical to them.
+//
+// Temporal blocking by z-marching.  A block owns an output box of SM_OX x-cells × SM_OY rows × zc
+// planes and computes, with one thread per x-pair and SM_ROWS rows, a region of 128 x-cells × 32 rows
+// around it (SM_HX/SM_HY cells of halo on each side that is not a domain face).  At time step t
+// the block forms stage 0 (ϵ = r·iD, ghosts loaded) on plane t and stage s = 1..5 on plane t − s,
+// in that order; stage s of a cell reads stage s − 1 values of its six neighbours, which are on
+// plane t − s (written by other threads one time step earlier) or on planes t − s ± 1 of the
+// thread's own column (written by the thread itself, earlier).  So the only barrier is one per time
+// step.  Stage s of a plane updates the cells of colour s only, which no other stage of the same
+// time step reads or writes; a cell outside the region shrunk by s at stage s holds a value no
+// output depends on.  Planes t − 6 .. t sit in a ring of seven LDS planes of (32 + 2) × 130
+// floats; the ring's outer rows and columns hold the domain's ghost values where the region ends
+// at a domain face.  All global loads of a time step are issued one step ahead (r or the ghost ϵ
+// of plane t + 1, x of the next stage-5 plane, the row flags), r of planes t − 5 .. t stays in
+// registers, and the coefficient branch is uniform per wave (a wave is one row), so the stages
+// wait on LDS only.  The new r is written to rout (other blocks read r around their own outputs,
+// so it cannot be updated in place).
+constexpr int SM_OX = 116, SM_OY = 22, SM_HX = 6, SM_HY = 5, SM_RY = 34, SM_NP = 7;
+// one colour array's row: 65 entries (the 64 pairs and one ring column), an odd stride
+constexpr int SM_W = 65, SM_HP = SM_RY * SM_W, SM_PL = 2 * SM_HP;
+constexpr int SM_ROWS = 4, SM_THREADS = 64 * 32 / SM_ROWS;  // rows per thread, threads per block
+template <class T>
+__global__ void __launch_bounds__(SM_THREADS) k_smooth4(Geo g, LRows<T> LR, const T* __restrict__ r, T* __restrict__ rout,
+                                                  T* __restrict__ x, const T* __restrict__ e, T* __restrict__ ej,
+                                                  const T* __restrict__ wp, double* __restrict__ partial, int nbx,
+                                                  int nby, int zc, UniCoef<T> ucoef) {
+  extern __shared__ unsigned char sm_raw[];
+  T* S = (T*)sm_raw;  // SM_NP planes; a plane holds its x-pairs' first cells (A), then their second cells (B)
+  const T w = *wp;
+  const T uc0 = wave_val(LR.c[0]), uc1 = wave_val(LR.c[1]), uc2 = wave_val(LR.c[2]);
+  const T uD0 = wave_val(ucoef.D[0]), uD1 = wave_val(ucoef.D[1]), uD2 = wave_val(ucoef.D[2]);
+  const T ui0 = wave_val(ucoef.iD[0]), ui1 = wave_val(ucoef.iD[1]), ui2 = wave_val(ucoef.iD[2]);
+#define WL_UC uc0, uc1, uc2, uD0, uD1, uD2, ui0, ui1, ui2
+  const int lid = xcd_tile_id();
+  const int tx = lid % nbx, r1 = lid / nbx, ty = r1 % nby, tz = r1 / nby;
+  const int N0 = g.N[0], N1 = g.N[1], N2 = g.N[2];
+  const uint32_t s1 = (uint32_t)g.s[1], s2 = (uint32_t)g.s[2];
+  // output box and region origin
+  const int xo0 = 1 + SM_OX * tx, xo1 = min(SM_OX * (tx + 1), N0 - 2);
+  const int yo0 = 1 + SM_OY * ty, yo1 = min(SM_OY * (ty + 1), N1 - 2);
+  const int zo0 = 1 + zc * tz, zo1 = min(zc * (tz + 1), N2 - 2);
+  const int X0 = tx == 0 ? 1 : xo0 - SM_HX, Y0 = ty == 0 ? 1 : yo0 - SM_HY;
+  const int lane = threadIdx.x, tid = lane + 64 * (int)threadIdx.y;
+  const int wv = __builtin_amdgcn_readfirstlane((int)threadIdx.y);  // one wave per SM_ROWS rows: uniform
+  const int xa = X0 + 2 * lane;           // this thread's pair (xa, xa + 1)
+  // LDS: A(p, ry, l) holds x = X0 + 2l (l = 0..64), B(p, ry, l) holds x = X0 + 2l + 1 (l = −1..63), so
+  // a wave's accesses are unit-stride (no bank conflicts) and the colour a stage updates is one of
+  // the two arrays (wave-uniform)
+  auto pbase = [&](int p) { return (p % SM_NP) * SM_PL; };
+  auto A = [&](int pb, int ry, int l) -> T& { return S[pb + ry * SM_W + l]; };
+  auto B = [&](int pb, int ry, int l) -> T& { return S[pb + SM_HP + ry * SM_W + l + 1]; };
+  const bool pin = xa + 1 <= N0 - 2;       // both cells of the pair inside (N0 − 2 is even)
+  const bool pout = xa >= xo0 && xa + 1 <= xo1;
+  int yq[SM_ROWS];
+  bool yin[SM_ROWS], yout[SM_ROWS];
+  uint32_t rowI[SM_ROWS];  // the pair's offset in each row (plane 0); a plane adds s2
+#pragma unroll
+  for (int q = 0; q < SM_ROWS; ++q) {
+    yq[q] = Y0 + SM_ROWS * wv + q;
+    yin[q] = yq[q] >= 1 && yq[q] <= N1 - 2;
+    yout[q] = yq[q] >= yo0 && yq[q] <= yo1;
+    rowI[q] = (uint32_t)xa + s1 * (uint32_t)yq[q];
+  }
+  // the ring cell of this thread (tid < 2·SM_RY + 256): column X0 − 1 or X0 + 128, row Y0 − 1 or Y0 + 32
+  int rox = 0, rry = 0;  // x offset from X0 (−1 .. 128) and LDS row
+  if (tid < 2 * SM_RY) { rox = tid < SM_RY ? -1 : 128; rry = tid < SM_RY ? tid : tid - SM_RY; }
+  else { const int j = tid - 2 * SM_RY; rox = j & 127; rry = j < 128 ? 0 : SM_RY - 1; }
+  const int rgx = X0 + rox, rgy = Y0 - 1 + rry;
+  // its LDS offset within a plane: A for an even x offset, B for an odd one
+  const int rlds = (rox & 1) ? SM_HP + rry * SM_W + ((rox - 1) >> 1) + 1 : rry * SM_W + (rox >> 1);
+  const bool rgrid = tid < 2 * SM_RY + 256 && rgx >= 0 && rgx <= N0 - 1 && rgy >= 0 && rgy <= N1 - 1;
+  const bool rghost_xy = rgx == 0 || rgx == N0 - 1 || rgy == 0 || rgy == N1 - 1;
+  const uint32_t ringI = rgrid ? (uint32_t)rgx + s1 * (uint32_t)rgy : 0u;
+  // stage-0 planes and the pipeline's end
+  const int zs0 = max(0, zo0 - 5), ze0 = min(N2 - 1, zo1 + 5);
+  // loads of plane z issued ahead: the stage-0 source pair of each row (r inside, the stored ϵ on
+  // ghost cells), the ring value, the row flags
+  auto zinside = [&](int z) { return z >= 1 && z <= N2 - 2; };
+  auto load_src = [&](int z, int q) -> P2<T> {
+    P2<T> v{T(0), T(0)};
+    if (z > ze0 || yq[q] > N1 - 1 || xa > N0 - 1) return v;
+    const uint32_t I = rowI[q] + s2 * (uint32_t)z;
+    if (zinside(z) && yin[q] && pin) return ld2(r + I);
+    v.a = e[I];
+    if (xa + 1 <= N0 - 1) v.b = e[I + 1];
+    return v;
+  };
+  auto load_ring = [&](int z) -> T {
+    return (z <= ze0 && rgrid && (rghost_xy || !zinside(z))) ? e[ringI + s2 * (uint32_t)z] : T(0);
+  };
+  // the uniform-row flags of the block's rows and planes, staged in LDS once (a flag is read in
+  // the loop as one wave-uniform LDS byte, so no global load waits inside a time step)
+  unsigned char* F = sm_raw + (size_t)SM_NP * SM_PL * sizeof(T);
+  for (int i = tid; i < (ze0 - zs0 + 1) * 32; i += SM_THREADS) {
+    const int z = zs0 + (i >> 5), y = Y0 + (i & 31);
+    F[i] = (LR.rows != nullptr && zinside(z) && y >= 1 && y <= N1 - 2) ? LR.rows[y + (uint32_t)N1 * (uint32_t)z] : 0;
+  }
+  __syncthreads();
+  P2<T> rw[6][SM_ROWS];  // r of planes t − k (k = 0..5) of this thread's rows (the stage-0 source for ghosts)
+  int flm = 0;           // their row flags: bit SM_ROWS·k + q (wave-uniform)
+  P2<T> srcn[SM_ROWS], xn[SM_ROWS];
+#pragma unroll
+  for (int q = 0; q < SM_ROWS; ++q) {
+#pragma unroll
+    for (int k = 0; k < 6; ++k) rw[k][q] = P2<T>{T(0), T(0)};
+    srcn[q] = load_src(zs0, q);
+    xn[q] = P2<T>{T(0), T(0)};
+  }
+  T ringn = load_ring(zs0);
+  double acc = 0;
+  for (int t = zs0; t <= ze0 + 5; ++t) {
+#pragma unroll
+    for (int k = 5; k > 0; --k)
+#pragma unroll
+      for (int q = 0; q < SM_ROWS; ++q) rw[k][q] = rw[k - 1][q];
+    {
+      const uint32_t f4 = t <= ze0 ? (uint32_t)__builtin_amdgcn_readfirstlane(
+                                         (int)*(const uint32_t*)(F + (t - zs0) * 32 + SM_ROWS * wv))
+                                   : 0u;
+      int nf = 0;
+#pragma unroll
+      for (int q = 0; q < SM_ROWS; ++q) nf |= ((f4 >> (8 * q)) & 0xff) ? (1 << q) : 0;
+      flm = ((flm << SM_ROWS) | nf) & ((1 << (6 * SM_ROWS)) - 1);
+    }
+    auto fl = [&](int k, int q) -> bool { return (flm >> (SM_ROWS * k + q)) & 1; };
+    const T ring = ringn;
+    P2<T> xc[SM_ROWS];
+#pragma unroll
+    for (int q = 0; q < SM_ROWS; ++q) { xc[q] = xn[q]; rw[0][q] = srcn[q]; }
+    // the next step's loads
+#pragma unroll
+    for (int q = 0; q < SM_ROWS; ++q) {
+      srcn[q] = load_src(t + 1, q);
+      const int p5 = t - 4;  // stage 5's plane in the next step
+      if (p5 >= zo0 && p5 <= zo1 && pout && yout[q]) xn[q] = ld2(x + (rowI[q] + s2 * (uint32_t)p5));
+    }
+    ringn = load_ring(t + 1);
+    // ---- stage 0 on plane t: ϵ = r·iD inside, the stored ϵ on ghost cells ----
+    if (t <= ze0) {
+      const bool zin = zinside(t);
+#pragma unroll
+      for (int q = 0; q < SM_ROWS; ++q) {
+        P2<T> v = rw[0][q];
+        if (zin && yin[q] && pin) {
+          const uint32_t I = rowI[q] + s2 * (uint32_t)t;
+          const CellCoef<T> ca = coef_u(g, LR, WL_UC, xa, I, fl(0, q)), cb = coef_u(g, LR, WL_UC, xa + 1, I + 1, fl(0, q));
+          v.a = v.a * ca.iD;
+          v.b = v.b * cb.iD;
+        }
+        const int pb = pbase(t), ry = 1 + SM_ROWS * wv + q;
+        A(pb, ry, lane) = v.a;
+        B(pb, ry, lane) = v.b;
+      }
+      if (tid < 2 * SM_RY + 256) S[pbase(t) + rlds] = ring;
+    }
+    // ---- stages 1..4: GS-RB half sweep it = s on plane t − s (src/Poisson.jl:116-128) ----
+#pragma unroll
+    for (int s = 1; s <= 4; ++s) {
+      const int p = t - s;
+      if (p >= max(1, zo0 - 5 + s) && p <= min(N2 - 2, zo1 + 5 - s) && pin) {
+#pragma unroll
+        for (int q = 0; q < SM_ROWS; ++q) {
+          if (yin[q]) {
+            const int ry = 1 + SM_ROWS * wv + q;
+            // the updated cell is xa (array A), else xa + 1 (B); the same for every lane of the wave
+            const bool first = ((xa + yq[q] + p + s) & 1) == 0;
+            const int xcell = first ? xa : xa + 1;
+            const int pb = pbase(p), pbm = pbase(p - 1), pbp = pbase(p + 1);
+            const int co = first ? 0 : SM_HP + 1;  // the updated array's offset (B is shifted by one pair)
+            const T xm = first ? T(B(pb, ry, lane - 1)) : T(A(pb, ry, lane));
+            const T xp = first ? T(B(pb, ry, lane)) : T(A(pb, ry, lane + 1));
+            const T ym = S[pb + co + (ry - 1) * SM_W + lane], yp = S[pb + co + (ry + 1) * SM_W + lane];
+            const T zm = S[pbm + co + ry * SM_W + lane], zp = S[pbp + co + ry * SM_W + lane];
+            const P2<T> rp = rw[s][q];  // a copy: the select below picks a register, not an address
+            const T rv = first ? rp.a : rp.b;
+            const uint32_t I = rowI[q] + s2 * (uint32_t)p + (first ? 0u : 1u);
+            const CellCoef<T> c = coef_u(g, LR, WL_UC, xcell, I, fl(s, q));
+            T sv = rv;
+            sv -= (xm * c.L.l[0] + xp * c.L.h[0]);
+            sv -= (ym * c.L.l[1] + yp * c.L.h[1]);
+            sv -= (zm * c.L.l[2] + zp * c.L.h[2]);
+            S[pb + co + ry * SM_W + lane] = sv * c.iD;
+          }
+        }
+      }
+    }
+    // ---- stage 5: increment! on plane t − 5 at the output cells (src/Poisson.jl:100-104) ----
+    {
+      const int p = t - 5;
+      if (p >= zo0 && p <= zo1 && pout) {
+#pragma unroll
+        for (int q = 0; q < SM_ROWS; ++q) {
+          if (yout[q]) {
+            const int ry = 1 + SM_ROWS * wv + q;
+            const uint32_t I = rowI[q] + s2 * (uint32_t)p;
+            const int pb = pbase(p), pbm = pbase(p - 1), pbp = pbase(p + 1);
+            const P2<T> e0{A(pb, ry, lane), B(pb, ry, lane)};
+            const T xl = B(pb, ry, lane - 1), xr = A(pb, ry, lane + 1);
+            const P2<T> ym{A(pb, ry - 1, lane), B(pb, ry - 1, lane)}, yp{A(pb, ry + 1, lane), B(pb, ry + 1, lane)};
+            const P2<T> zm{A(pbm, ry, lane), B(pbm, ry, lane)}, zp{A(pbp, ry, lane), B(pbp, ry, lane)};
+            const CellCoef<T> ca = coef_u(g, LR, WL_UC, xa, I, fl(5, q)), cb = coef_u(g, LR, WL_UC, xa + 1, I + 1, fl(5, q));
+            const Nb<T> va{{xl, ym.a, zm.a}, {e0.b, yp.a, zp.a}}, vb{{e0.a, ym.b, zm.b}, {xr, yp.b, zp.b}};
+            const T ra = rw[5][q].a - w * mult_nb<T>(g, ca.L, 0, ca.D, e0.a, va);
+            const T rb = rw[5][q].b - w * mult_nb<T>(g, cb.L, 0, cb.D, e0.b, vb);
+            st2(rout + I, ra, rb);
+            st2(x + I, xc[q].a + w * e0.a, xc[q].b + w * e0.b);
+            st2(ej + I, ra * ca.iD, rb * cb.iD);
+            acc += (double)ra * (double)ra;
+            acc += (double)rb * (double)rb;
+          }
+        }
+      }
+    }
+    __syncthreads();
+  }
+#undef WL_UC
+  const double tt = block_sum(acc);
+  if (tid == 0) partial[blockIdx.x] = tt;
 }
 
 // Two GS-RB half sweeps in one pass (src/Poisson.jl:116-148, it = 2q−1 then 2q, 3D): colour A

@@ -104,7 +104,37 @@ __global__ void __launch_bounds__(256) k_nn_norm(int V, const float* __restrict_
   const uint32_t c0 = f2bf(q.x * scal[1]), c1 = f2bf((q.y - scal[2]) * scal[3]), c2 = f2bf(q.z);
   *(uint4*)(in0 + 8 * (int64_t)v) = make_uint4(c0 | (c1 << 16), c2, 0u, 0u);
 }
-// x[child] += s_r · out[v][0] on every level-0 interior cell
+// block partials of Σ_v out[v][0] in FP64 (the guess's mean, which apply removes)
+__global__ void __launch_bounds__(256) k_nn_out_sum(int V, const float* __restrict__ out, int os,
+                                                    double* __restrict__ part) {
+  __shared__ double sh[256];
+  const int v = blockIdx.x * 256 + threadIdx.x;
+  sh[threadIdx.x] = v < V ? (double)out[(int64_t)v * os] : 0.0;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+}
+// scal[4] = mean of out[·][0]; one block of 256 threads, strided partial sums, then a fixed tree
+__global__ void __launch_bounds__(256) k_nn_out_mean(const double* __restrict__ part, int nblk, int V,
+                                                     float* __restrict__ scal) {
+  __shared__ double sh[256];
+  double a = 0;
+  for (int b = threadIdx.x; b < nblk; b += 256) a += part[b];
+  sh[threadIdx.x] = a;
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if ((int)threadIdx.x < st) sh[threadIdx.x] += sh[threadIdx.x + st];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) scal[4] = (float)(sh[0] / V);
+}
+// x[child] += s_r · (out[v][0] − mean out) on every level-0 interior cell.  The guess enters
+// without its mean: a constant is in the null space of the Neumann Poisson operator, so it cannot
+// reduce the residual, and it would stay in p (mom_project!'s warm start carries p from step to
+// step, src/Flow.jl:225), where the pressure force picks it up through Σ n·kern(d) ≠ 0.
 template <class T>
 __global__ void __launch_bounds__(256) k_nn_apply(Geo g, NGrid n, const float* __restrict__ out, int os,
                                                   const float* __restrict__ scal, T* __restrict__ x) {
@@ -114,7 +144,7 @@ __global__ void __launch_bounds__(256) k_nn_apply(Geo g, NGrid n, const float* _
   const int fx = (int)(i % nx0), fy = (int)((i / nx0) % ny0), fz = (int)(i / (nx0 * ny0));
   const int v = ((fz >> 1) * n.ny + (fy >> 1)) * n.nx + (fx >> 1);
   const int64_t I = at(g, 1 + fx, 1 + fy, 1 + fz);
-  x[I] = x[I] + (T)(scal[0] * out[(int64_t)v * os]);
+  x[I] = x[I] + (T)(scal[0] * (out[(int64_t)v * os] - scal[4]));
 }
 // target t[v] = (mean children x − X1[v]) / s_r
 template <class T>
@@ -163,7 +193,7 @@ struct WarmNet {
   double* part = nullptr;  // reduction partials
   double* dsc = nullptr;   // loss scalars
   float* X = nullptr;      // raw input of the last prep [V][4]
-  float* scal = nullptr;   // s_r, 1/s_r, mean x, 1/s_x
+  float* scal = nullptr;   // s_r, 1/s_r, mean x, 1/s_x, mean of the guess
   float* tgt = nullptr;    // target of the last recorded pair
   // replay store
   int cap = 0, count = 0, head = 0;
@@ -247,7 +277,7 @@ struct WarmNet {
     part = alloc<double>(4 * (int64_t)nblk256(v0));
     dsc = alloc<double>(8);
     X = alloc<float>(v0 * 4);
-    scal = alloc<float>(4);
+    scal = alloc<float>(8);
     tgt = alloc<float>(v0);
     cap = std::max(replay, 1);
     rin = alloc<bf16_t>((int64_t)cap * v0 * 8);
@@ -437,6 +467,11 @@ struct WarmNet {
   }
   template <class T>
   void apply(const Geo& g, T* x, hipStream_t st) {
+    const unsigned nb = nblk256(V());
+    hipLaunchKernelGGL(k_nn_out_sum, dim3(nb), dim3(256), 0, st, V(), (const float*)out, 8, part);
+    KCK();
+    hipLaunchKernelGGL(k_nn_out_mean, dim3(1), dim3(256), 0, st, (const double*)part, (int)nb, V(), scal);
+    KCK();
     hipLaunchKernelGGL(k_nn_apply<T>, dim3(nblk256((int64_t)V() * 8)), dim3(256), 0, st, g, R[0], (const float*)out, 8,
                        (const float*)scal, x);
     KCK();

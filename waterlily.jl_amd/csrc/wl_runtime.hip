@@ -14,6 +14,7 @@
 #include <cstdlib>
 #include <limits>
 #include <mutex>
+#include <type_traits>
 #include "wl_kernels.h"
 
 using namespace wl;
@@ -230,6 +231,7 @@ struct MG {
     free_graphs();
     dfree(x64[0]);
     dfree(x64[1]);
+    dfree(r_alt);
     for (auto& l : lv) {
       if (l.ownL) dfree(l.L);
       if (l.ownx) dfree(l.x);
@@ -270,6 +272,10 @@ struct MG {
       use_pairs = !e3 || std::atoi(e3) != 0;
       const char* e2 = std::getenv("WL_GS_PAIR");
       use_gs_pair = e2 && std::atoi(e2) != 0;
+      const char* e5 = std::getenv("WL_GS_DBL");
+      use_gs_dbl = e5 && std::atoi(e5) != 0;
+      const char* e4 = std::getenv("WL_SMOOTH4");
+      use_smooth4 = e4 && std::atoi(e4) != 0;
     }
     lv.push_back(l0);
     if (!multilevel) return;
@@ -304,8 +310,17 @@ struct MG {
     }
     if (lv.size() <= 2) throw WlErr(WL_EINVAL, "MultiLevelPoisson requires size=a2ⁿ, where n>2");
     for (size_t l = 0; l < lv.size(); ++l)  // before any graph capture: no allocation while capturing
-      if (gs_pair_ok((int)l)) lv[l].e2 = dalloc<T>(lv[l].g.n);
+      if (gs_pair_ok((int)l) || gs_dbl_ok((int)l)) lv[l].e2 = dalloc<T>(lv[l].g.n);
     if (fold_coarse_inc && pairs_ok(0)) lv[0].ej = dalloc<T>(lv[0].g.n);
+    if (smooth4_ok()) {
+      r_alt = dalloc<T>(lv[0].g.n);
+      static bool sm_attr = false;  // k_smooth4's LDS ring exceeds the default 64 KB
+      if (!sm_attr) {
+        HIPCK(hipFuncSetAttribute((const void*)k_smooth4<float>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)smooth4_lds()));
+        sm_attr = true;
+      }
+    }
     static bool attr_set = false;  // the LDS tail may use more than 64 KB of dynamic LDS
     if (!attr_set) {
       HIPCK(hipFuncSetAttribute((const void*)k_mg_tail_lds<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -479,8 +494,37 @@ struct MG {
     const Geo& g = lv[l].g;
     return use_gs_pair && perdir == 0 && g.D == 3 && g.N[2] % 2 == 0 && g.N[0] >= 3 && g.N[1] >= 3 && g.N[2] >= 4;
   }
+  // two half sweeps per pass (k_gs_dbl), with WL_GS_DBL=1: 3D levels with x pairs, no periodic
+  // directions.  Off by default: on the 512x256x256 sphere a pass took 335 µs against 2 × 120 µs for
+  // two half sweeps; it moves fewer bytes but forming the first colour of three rows per wave makes
+  // it bound by its VALU work (about 4× the arithmetic of a half sweep per cell).
+  bool use_gs_dbl = false;
+  bool gs_dbl_ok(int l) const { return use_gs_dbl && perdir == 0 && lv[l].g.D == 3 && pairs_ok(l); }
   void gs_sweeps(int l) {
     Level<T>& p = lv[l];
+    if (gs_dbl_ok(l) && p.e2) {
+      const Geo& g = p.g;
+      ZT td;
+      td.b = box_pairs(g);
+      const int npair = td.b.hi[0] + 1;
+      td.sx = 62;  // output pairs per tile (lanes 0 and 63 are a halo)
+      td.sy = 4;
+      td.nbx = 1 + (std::max(0, npair - 63) + 61) / 62;
+      td.nby = (g.N[1] - 2 + 3) / 4;
+      const int64_t per = (int64_t)td.nbx * td.nby;
+      const int nz = g.N[2] - 2;
+      td.zc = (int)std::min<int64_t>(32, std::max<int64_t>(1, (int64_t)nz * per / tgt_gs()));
+      // r at every cell, the old ϵ of the other colour, every ϵ written: 2.5 values per cell
+      ProfScope ps(P, st, "gs_dbl", l, (double)sizeof(T) * 2.5 * ncells_inside(g), 2);
+      const UniCoef<T> uc = uni_coef<T>(p.lc);
+      hipLaunchKernelGGL(k_gs_dbl<T>, zt_grid(td), kBlock, 0, st, g, td, lr(l), (const T*)p.r, (const T*)p.e,
+                         (const T*)p.e, p.e2, 1, uc);
+      KCK();
+      hipLaunchKernelGGL(k_gs_dbl<T>, zt_grid(td), kBlock, 0, st, g, td, lr(l), (const T*)p.r, (const T*)p.e2,
+                         (const T*)p.e, p.e, 3, uc);
+      KCK();
+      return;
+    }
     perbc(p.g, p.e);
     if (gs_pair_ok(l) && p.e2) {
       ZT t;
@@ -609,6 +653,53 @@ struct MG {
     KCK();
   }
 
+  // The finest smooth! + increment! in one pass (k_smooth4), with WL_SMOOTH4=1: a flow's own solver
+  // on a 3D Float32 level 0 with x pairs and an even last extent, no periodic directions, no graph
+  // capture (the residual buffers swap between calls).  Off by default: on the 512x256x256 sphere it
+  // took 885 µs against 721 µs for the four half sweeps and the increment (it moves 4× fewer bytes
+  // but is bound by its VALU work: 1.86× redundant cells in its halo, and each stage's LDS
+  // addressing and coefficient selects).
+  bool use_smooth4 = false;
+  T* r_alt = nullptr;  // the residual buffer k_smooth4 writes; swapped with lv[0].r after each call
+  bool skip_eps0 = false;  // set around a V-cycle that k_smooth4 follows
+  // LDS of k_smooth4: the ring of planes, then 32 row flags per plane of a z tile (+ 10 halo planes)
+  static constexpr int kSmooth4MaxZc = 512;
+  static size_t smooth4_lds(int zc = kSmooth4MaxZc) { return (size_t)SM_NP * SM_PL * sizeof(float) + 32 * (zc + 12); }
+  bool smooth4_ok() const {
+    return use_smooth4 && sizeof(T) == 4 && fold_coarse_inc && !use_graphs && perdir == 0 && lv[0].g.D == 3 &&
+           pairs_ok(0) && lv[0].ej != nullptr;
+  }
+  int smooth4_zc() const {  // output planes per z tile: about 256·k blocks for one block per CU
+    const Geo& g = lv[0].g;
+    const int nxy = ((g.N[0] - 2 + SM_OX - 1) / SM_OX) * ((g.N[1] - 2 + SM_OY - 1) / SM_OY);
+    static int target = [] {
+      const char* e = std::getenv("WL_SMOOTH4_BLOCKS");
+      return e ? std::max(1, std::atoi(e)) : 240;
+    }();
+    const int nz = g.N[2] - 2;
+    int chunks = std::max(1, std::min(nz, (target + nxy - 1) / nxy));
+    chunks = std::max(chunks, (nz + kSmooth4MaxZc - 1) / kSmooth4MaxZc);
+    return (nz + chunks - 1) / chunks;
+  }
+  void smooth4(const T* w, int fmode) {
+    Level<T>& p = lv[0];
+    const Geo& g = p.g;
+    const int nbx = (g.N[0] - 2 + SM_OX - 1) / SM_OX, nby = (g.N[1] - 2 + SM_OY - 1) / SM_OY;
+    const int zc = smooth4_zc(), nbz = (g.N[2] - 2 + zc - 1) / zc;
+    const int64_t nb = (int64_t)nbx * nby * nbz;
+    if (nb > B->npartial) throw WlErr(WL_EINVAL, "k_smooth4: more blocks than partial slots");
+    {
+      ProfScope ps(P, st, "smooth4", 0, (double)sizeof(T) * 5 * ncells_inside(g));
+      if constexpr (std::is_same<T, float>::value) {
+        hipLaunchKernelGGL(k_smooth4<T>, dim3((unsigned)nb), dim3(64, 32 / SM_ROWS), smooth4_lds(zc), st, g, lr(0), (const T*)p.r,
+                           r_alt, p.x, (const T*)p.e, p.ej, w, B->partial, nbx, nby, zc, uni_coef<T>(p.lc));
+        KCK();
+      }
+    }
+    std::swap(p.r, r_alt);
+    ej_written = true;
+    finish_sum(nb, 1, fmode);
+  }
   // Vcycle! (src/MultiLevelPoisson.jl:88-101) from the state "ϵ(0) = r·iD"
   void vcycle(const T* w) {
     const int Lm = (int)lv.size() - 1;
@@ -675,18 +766,22 @@ struct MG {
   // being launched; its r update is dead (the next V-cycle's restrict! overwrites r and x there).
   void prolong_level(int c, const T* w, bool fold = false) {
     Level<T>& f = lv[c - 1];
-    // fine r and x read and written, ϵ written, the coarse x (and ϵ when folded) read
+    // fine r and x read and written, ϵ written (not before k_smooth4), the coarse x (and ϵ when
+    // folded) read
+    const int fw = (c == 1 && skip_eps0) ? 4 : 5;
     ProfScope ps(P, st, "prolong_inc", c - 1,
-                 (double)sizeof(T) * (5 * ncells_inside(f.g) + (fold ? 2 : 1) * ncells_inside(lv[c].g)));
+                 (double)sizeof(T) * (fw * ncells_inside(f.g) + (fold ? 2 : 1) * ncells_inside(lv[c].g)));
     if (pairs_ok(c - 1) && f.cmask == (1 << f.g.D) - 1 && perdir == 0) {
       ZT bp = make_zt(box_pairs(f.g), tgt_rp());
-#define WL_PRO2(DD, CI)                                                                                      \
-  hipLaunchKernelGGL((k_prolong_inc2<T, DD, CI>), zt_grid(bp), kBlock, 0, st, f.g, lv[c].g, bp, lr(c - 1), f.r, f.x, \
-                     f.e, (const T*)lv[c].x, w, (const T*)lv[c].e)
-      if (f.g.D == 3) {
-        if (fold) WL_PRO2(3, true); else WL_PRO2(3, false);
+#define WL_PRO2(DD, CI, EW)                                                                                  \
+  hipLaunchKernelGGL((k_prolong_inc2<T, DD, CI, EW>), zt_grid(bp), kBlock, 0, st, f.g, lv[c].g, bp, lr(c - 1), f.r, \
+                     f.x, f.e, (const T*)lv[c].x, w, (const T*)lv[c].e)
+      if (c == 1 && skip_eps0 && f.g.D == 3) {  // k_smooth4 follows and forms ϵ itself
+        if (fold) WL_PRO2(3, true, false); else WL_PRO2(3, false, false);
+      } else if (f.g.D == 3) {
+        if (fold) WL_PRO2(3, true, true); else WL_PRO2(3, false, true);
       } else {
-        if (fold) WL_PRO2(2, true); else WL_PRO2(2, false);
+        if (fold) WL_PRO2(2, true, true); else WL_PRO2(2, false, true);
       }
 #undef WL_PRO2
       KCK();
@@ -778,6 +873,13 @@ struct MG {
   void iteration(bool first) {
     const T* w = (const T*)B->tsc + 3;
     if (!first && !ej_written) eps_init(0);
+    if (smooth4_ok()) {
+      skip_eps0 = true;  // the level-0 prolongation leaves ϵ to k_smooth4
+      vcycle(w);
+      skip_eps0 = false;
+      smooth4(w, 3);  // L₂, then the ω update in the same single-block finish
+      return;
+    }
     vcycle(w);
     gs_sweeps(0);
     increment(0, w, true, 3);  // L₂, then the ω update in the same single-block finish
