@@ -2022,7 +2022,7 @@ template <class T>
 struct TailDesc {
   int n;
   int perdir;
-  int exact;  // mixed precision: the coarsest level's smooth! in FP64
+  int exact;  // mixed precision: the coarsest level's smooth! in FP64 (1: GS-RB, 2: pcg!)
   TailLevel<T> lv[TAIL_MAX_LEVELS];
 };
 constexpr int TAIL_CG_MAX = 128;  // cells (ghosts included) of a coarsest level smoothed in FP64
@@ -2286,7 +2286,7 @@ __device__ __forceinline__ void k_mg_tail_lds_body(const TailDesc<T>& d, const T
     });
     __syncthreads();
   }
-  __shared__ double cg[2][TAIL_CG_MAX];
+  __shared__ double cg[5][TAIL_CG_MAX];
   for (int l = d.n - 1; l >= 0; --l) {  // up-leg: smooth!(l), then prolongate! + increment!(l-1)
     const SLev<T> p = lev(l);
     perbc(p, p.e);
@@ -2296,10 +2296,72 @@ __device__ __forceinline__ void k_mg_tail_lds_body(const TailDesc<T>& d, const T
     // half_rangek on the last axis; every index below is a constant or a select, so the
     // per-cell coordinates stay in registers
     const int hh[3] = {hi[0], Ld == 1 ? p.N1 / 2 - 1 : hi[1], Ld == 2 ? p.N2 / 2 - 1 : hi[2]};
-    if (d.exact && l == d.n - 1 && p.n <= TAIL_CG_MAX && !d.perdir) {
+    if (d.exact == 2 && l == d.n - 1 && p.n <= TAIL_CG_MAX && !d.perdir) {
+      // Mixed precision with WL_COARSE_PCG=1: the coarsest level is solved by pcg! (src/Poisson.jl:166-186) in double,
+      // the Jacobi-preconditioned conjugate gradient on A·δ = r from δ = 0: at most 6 iterations
+      // with the reference's early exits (|ρ| < 10eps, |α| outside [1e-2, 1e2]).  δ, rounded to T,
+      // is the level's ϵ for the increment! below, in place of the GS-RB sweeps.
+      if (threadIdx.x == 0) {
+        double *X = cg[0], *ID = cg[1], *R = cg[2], *Z = cg[3], *E = cg[4];
+        for (int I = 0; I < p.n; ++I) { X[I] = 0; ID[I] = 0; R[I] = 0; Z[I] = 0; E[I] = 0; }
+        auto diag64 = [&](int I) {
+          double sd = 0;
+          sd -= ((double)p.L[I] + (double)p.L[I + 1]);
+          sd -= ((double)p.L[p.n + I] + (double)p.L[p.n + I + p.s1]);
+          if (p.D == 3) sd -= ((double)p.L[2 * p.n + I] + (double)p.L[2 * p.n + I + p.s2]);
+          return sd;
+        };
+        auto each = [&](auto&& f) {
+          for (int k = lo[2]; k <= hi[2]; ++k)
+            for (int j = lo[1]; j <= hi[1]; ++j)
+              for (int i = lo[0]; i <= hi[0]; ++i) f(i + p.s1 * j + p.s2 * k);
+        };
+        auto dot64 = [&](const double* a, const double* b) {
+          double acc = 0;
+          each([&](int I) { acc += a[I] * b[I]; });
+          return acc;
+        };
+        each([&](int I) {
+          const double sd = diag64(I);
+          ID[I] = sd == 0.0 ? 0.0 : 1.0 / sd;
+          R[I] = (double)p.r[I];
+          Z[I] = E[I] = R[I] * ID[I];
+        });
+        const double tiny = 10.0 * 2.220446049250313e-16;
+        double rho = dot64(R, Z);
+        if (fabs(rho) >= tiny) {
+          for (int it = 1; it <= 6; ++it) {
+            each([&](int I) {  // z = A·ϵ (mult, src/Poisson.jl:71-76; ghosts of ϵ are 0)
+              double sv = E[I] * diag64(I);
+              sv += (E[I - 1] * (double)p.L[I] + E[I + 1] * (double)p.L[I + 1]);
+              sv += (E[I - p.s1] * (double)p.L[p.n + I] + E[I + p.s1] * (double)p.L[p.n + I + p.s1]);
+              if (p.D == 3) sv += (E[I - p.s2] * (double)p.L[2 * p.n + I] + E[I + p.s2] * (double)p.L[2 * p.n + I + p.s2]);
+              Z[I] = sv;
+            });
+            const double alpha = rho / dot64(Z, E);
+            if (fabs(alpha) < 1e-2 || fabs(alpha) > 1e2) break;
+            each([&](int I) {
+              X[I] += alpha * E[I];
+              R[I] -= alpha * Z[I];
+            });
+            if (it == 6) break;
+            each([&](int I) { Z[I] = R[I] * ID[I]; });
+            const double rho2 = dot64(R, Z);
+            if (fabs(rho2) < tiny) break;
+            const double beta = rho2 / rho;
+            each([&](int I) { E[I] = beta * E[I] + Z[I]; });
+            rho = rho2;
+          }
+        }
+        each([&](int I) { p.e[I] = (T)X[I]; });
+      }
+      __syncthreads();
+    } else if (d.exact == 1 && l == d.n - 1 && p.n <= TAIL_CG_MAX && !d.perdir) {
       // Mixed precision: smooth! of the coarsest level in double: ϵ = r·iD and the four GS-RB
       // half sweeps (src/Poisson.jl:141-148) with ϵ, D and iD held in double, then ϵ rounded to T
-      // for increment!.  Same algorithm and order as the T sweeps below.
+      // for increment!.  Same algorithm and order as the T sweeps below.  (pcg! here instead moved
+      // the constant the Neumann problem leaves free in p away from the reference algorithm's: on a
+      // 64x32x32 sphere, p 2.9e-3 and the force 7.5e-4 from the Float64 oracle, against 1e-4 and 5e-6.)
       if (threadIdx.x == 0) {
         double *E = cg[0], *ID = cg[1];
         for (int I = 0; I < p.n; ++I) { E[I] = 0; ID[I] = 0; }
@@ -2842,13 +2904,17 @@ __global__ void k_mg_control(double* __restrict__ dsc, T* __restrict__ st) {
 // Partials: Σr in partial[b], Σr² in partial[nb + b], both in double.  EINC: einc and wp are
 // given (the correction's last increment is fused), as a template argument so the neighbour
 // reads carry no branch.
-template <class T, bool FLOW, bool EINC = false>
+// PREP: also the next V-cycle's start as k_mixed_prep forms it when no mean is removed, ϵ = r·iD
+// into eout and the correction c = 0 into cout (a buffer this pass does not read); with a mean to
+// remove, k_mixed_fix corrects r and ϵ afterwards.
+template <class T, bool FLOW, bool EINC = false, bool PREP = false>
 __global__ void __launch_bounds__(256) k_residual64(Geo g, ZT t, LRows<T> LR, const T* __restrict__ pin, T scale,
                                                     T* __restrict__ z, const T* __restrict__ u,
                                                     const double* __restrict__ xa, const T* __restrict__ xc,
                                                     double* __restrict__ xb, T* __restrict__ r,
                                                     double* __restrict__ partial, int64_t nb,
-                                                    const T* __restrict__ einc = nullptr, const T* __restrict__ wp = nullptr) {
+                                                    const T* __restrict__ einc = nullptr, const T* __restrict__ wp = nullptr,
+                                                    T* __restrict__ eout = nullptr, T* __restrict__ cout = nullptr) {
   int c0, c1, z0, z1;
   zt_cell(t, c0, c1, z0, z1);
   double acc = 0, acc2 = 0;
@@ -2893,6 +2959,10 @@ __global__ void __launch_bounds__(256) k_residual64(Geo g, ZT t, LRows<T> LR, co
       const double rv = Dg == 0.0 ? 0.0 : (double)zv - ax;
       xb[I] = xI;
       r[I] = (T)rv;
+      if (PREP) {
+        eout[I] = (T)rv * inv_diag(diag_at<T>(g, L, I));
+        cout[I] = T(0);
+      }
       acc += rv;
       acc2 += rv * rv;
     }
@@ -2934,6 +3004,26 @@ __global__ void __launch_bounds__(256) k_mixed_prep(Geo g, ZT t, LRows<T> LR, T*
     if (sub) { rv = rv - s; r[I] = rv; }
     e[I] = rv * inv_diag(diag_at<T>(g, lcell(g, LR, c0, c1, c2), I));
     c[I] = T(0);
+  }
+}
+// The mean removal k_mixed_prep does, after a k_residual64<PREP> pass formed ϵ and c: when the
+// finish flagged a mean (|s| > 2eps), r −= s and ϵ = r·iD again; otherwise every block returns at
+// once.  A grid-stride loop over the tiles of t (nb of them).
+template <class T>
+__global__ void __launch_bounds__(256) k_mixed_fix(Geo g, ZT t, LRows<T> LR, T* __restrict__ r, T* __restrict__ e,
+                                                   const T* __restrict__ sdev, int nb) {
+  if (sdev[1] == T(0)) return;
+  const T s = sdev[0];
+  for (int lid = blockIdx.x; lid < nb; lid += gridDim.x) {
+    int c0, c1, z0, z1;
+    zt_cell_id(t, lid, c0, c1, z0, z1);
+    if (c0 > t.b.hi[0] || c1 > t.b.hi[1]) continue;
+    for (int c2 = z0; c2 <= z1; ++c2) {
+      const int64_t I = at(g, c0, c1, c2);
+      const T rv = r[I] - s;
+      r[I] = rv;
+      e[I] = rv * inv_diag(diag_at<T>(g, lcell(g, LR, c0, c1, c2), I));
+    }
   }
 }
 // x = (T)x64 on inside (the solution handed back to mom_project!)

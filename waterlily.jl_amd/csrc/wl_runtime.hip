@@ -231,6 +231,7 @@ struct MG {
     free_graphs();
     dfree(x64[0]);
     dfree(x64[1]);
+    dfree(c_alt);
     dfree(r_alt);
     for (auto& l : lv) {
       if (l.ownL) dfree(l.L);
@@ -272,6 +273,8 @@ struct MG {
       use_pairs = !e3 || std::atoi(e3) != 0;
       const char* e2 = std::getenv("WL_GS_PAIR");
       use_gs_pair = e2 && std::atoi(e2) != 0;
+      const char* e7 = std::getenv("WL_COARSE_PCG");
+      coarse_pcg = e7 && std::atoi(e7) != 0;
       const char* e5 = std::getenv("WL_GS_DBL");
       use_gs_dbl = e5 && std::atoi(e5) != 0;
       const char* e4 = std::getenv("WL_SMOOTH4");
@@ -627,11 +630,12 @@ struct MG {
     return -1;
   }
   bool exact_coarsest = false;  // set by the mixed-precision loop around its V-cycles
+  bool coarse_pcg = false;      // WL_COARSE_PCG=1: that coarsest level by FP64 pcg! instead of GS-RB
   void launch_tail(int Lt, bool in_lds, const T* w) {
     TailDesc<T> d;
     d.n = (int)lv.size() - Lt;
     d.perdir = perdir;
-    d.exact = exact_coarsest && in_lds;
+    d.exact = (exact_coarsest && in_lds) ? (coarse_pcg ? 2 : 1) : 0;
     int64_t off = 0;
     for (int k = 0; k < d.n; ++k) {
       const Level<T>& q = lv[Lt + k];
@@ -955,7 +959,15 @@ struct MG {
   void ensure_x64() {
     for (auto& b : x64)
       if (!b) b = dalloc<double>(lv[0].g.n);
+    if (mixed_fused() && !c_alt) c_alt = dalloc<T>(lv[0].g.n);
   }
+  // The FP64 residual passes also start the next V-cycle (ϵ = r·iD, correction c = 0; k_mixed_fix
+  // when a mean is removed) instead of a k_mixed_prep pass: needs a Jacobi ϵ buffer apart from ϵ
+  // (the pass reads ϵ around each cell) and no periodic directions.  c alternates between the
+  // level-0 x and c_alt (the pass reads c around each cell while it clears the other).
+  T* c_alt = nullptr;
+  bool mixed_prepped = false;  // the last FP64 residual pass formed ϵ and c
+  bool mixed_fused() const { return perdir == 0 && lv[0].ej != nullptr; }
   void finish_sum2(int64_t nb) {
     hipLaunchKernelGGL(k_finish_sum2<T>, dim3(1), dim3(256), 0, st, (const double*)B->partial, nb, B->dsc, (T*)B->tsc,
                        (double)ninside(lv[0].g));
@@ -969,14 +981,20 @@ struct MG {
     hipLaunchKernelGGL(k_mg_init<T>, dim3(1), dim3(64), 0, st, (const double*)B->dsc, (T*)B->tsc);
     KCK();
     if (hist) hist->push_back((double)(T)read_dsc(1));
+    T* const x0 = p.x;  // the level-0 x the loop ends on (c may sit in c_alt meanwhile)
     int n = 0;
     while (n < itmx) {
-      {
+      if (mixed_prepped) {
+        hipLaunchKernelGGL(k_mixed_fix<T>, dim3((unsigned)std::min<int64_t>(nb, 256)), kBlock, 0, st, p.g, bi, lr(0),
+                           p.r, jac(0), (const T*)B->tsc, (int)nb);
+        KCK();
+      } else {
         ProfScope ps(P, st, "mixed_prep", 0, (double)sizeof(T) * (p.g.D + 4) * ncells_inside(p.g));
         hipLaunchKernelGGL(k_mixed_prep<T>, zt_grid(bi), kBlock, 0, st, p.g, bi, lr(0), p.r, jac(0), p.x,
                            (const T*)B->tsc);
         KCK();
       }
+      mixed_prepped = false;
       exact_coarsest = true;
       vcycle(w);
       exact_coarsest = false;
@@ -989,9 +1007,17 @@ struct MG {
         perbc(p.g, p.x);
       }
       {
+        const bool pf = fuse && mixed_fused();
         ProfScope ps(P, st, "residual64", 0,
-                     (double)ncells_inside(p.g) * (sizeof(T) * (p.g.D + 2 + (fuse ? 1 : 0)) + 2 * sizeof(double)));
-        if (fuse)
+                     (double)ncells_inside(p.g) *
+                         (sizeof(T) * (p.g.D + 2 + (fuse ? 1 : 0) + (pf ? 2 : 0)) + 2 * sizeof(double)));
+        if (pf) {
+          hipLaunchKernelGGL((k_residual64<T, false, true, true>), zt_grid(bi), kBlock, 0, st, p.g, bi, lr(0),
+                             (const T*)nullptr, T(0), p.z, (const T*)nullptr, (const double*)x64[x64cur],
+                             (const T*)p.x, x64[1 - x64cur], p.r, B->partial, nb, (const T*)p.e, w, jac(0), c_alt);
+          std::swap(p.x, c_alt);
+          mixed_prepped = true;
+        } else if (fuse)
           hipLaunchKernelGGL((k_residual64<T, false, true>), zt_grid(bi), kBlock, 0, st, p.g, bi, lr(0),
                              (const T*)nullptr, T(0), p.z, (const T*)nullptr, (const double*)x64[x64cur],
                              (const T*)p.x, x64[1 - x64cur], p.r, B->partial, nb, (const T*)p.e, w);
@@ -1018,6 +1044,8 @@ struct MG {
         break;
       }
     }
+    if (p.x != x0) std::swap(p.x, c_alt);  // level 0's x is the flow's buffer again
+    mixed_prepped = false;
     return n;
   }
 
@@ -1446,8 +1474,16 @@ struct FlowT : FlowBase {
     mg.x64cur = 0;
     {
       ProfScope ps(&prof, st, "residual64", 0, (double)ncells_inside(g) * (sizeof(T) * (3 * D + 3) + sizeof(double)));
-      hipLaunchKernelGGL((k_residual64<T, true>), zt_grid(bi), kBlock, 0, st, g, bi, mg.lr(0), (const T*)p, dt, sigma,
-                         (const T*)u, (const double*)nullptr, (const T*)nullptr, mg.x64[0], l0.r, B.partial, nb);
+      if (mg.mixed_fused()) {  // the V-cycle's start too (l0.x is not read in this pass)
+        hipLaunchKernelGGL((k_residual64<T, true, false, true>), zt_grid(bi), kBlock, 0, st, g, bi, mg.lr(0), (const T*)p,
+                           dt, sigma, (const T*)u, (const double*)nullptr, (const T*)nullptr, mg.x64[0], l0.r, B.partial,
+                           nb, (const T*)nullptr, (const T*)nullptr, mg.jac(0), l0.x);
+        mg.mixed_prepped = true;
+      } else {
+        hipLaunchKernelGGL((k_residual64<T, true>), zt_grid(bi), kBlock, 0, st, g, bi, mg.lr(0), (const T*)p, dt, sigma,
+                           (const T*)u, (const double*)nullptr, (const T*)nullptr, mg.x64[0], l0.r, B.partial, nb);
+        mg.mixed_prepped = false;
+      }
       KCK();
     }
     mg.perbc(g, mg.x64[0]);
@@ -1524,6 +1560,7 @@ struct FlowT : FlowBase {
                          (const T*)l0.x, mg.x64[1 - mg.x64cur], l0.r, B.partial, nb, (const T*)nullptr,
                          (const T*)nullptr);
       KCK();
+      mg.mixed_prepped = false;  // a new defect: the loop's first pass starts the V-cycle itself
       mg.x64cur = 1 - mg.x64cur;
       mg.perbc(g, mg.x64[mg.x64cur]);
       mg.finish_sum2(nb);
