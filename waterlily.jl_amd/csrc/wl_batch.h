@@ -80,12 +80,15 @@ struct ARes {
   const T* u;
   T* r;
   double* partial;
+  T* eout;          // ϵ = r·iD, formed in the same pass
+  int64_t pstride;  // the Σr² partials follow the Σr ones at this offset
 };
 template <class T>
 __global__ void __launch_bounds__(256) kb_residual(const ARes<T>* __restrict__ A, const BBlk* __restrict__ blk) {
   const BBlk b = blk[blockIdx.x];
   const ARes<T>& a = A[b.sim];
-  k_residual_body<T, true>(a.g, a.t, a.L, a.p, a.x, a.scale, 1, a.z, a.u, a.r, a.partial, b.lid, b.pidx);
+  k_residual_body<T, true, true>(a.g, a.t, a.L, a.p, a.x, a.scale, 1, a.z, a.u, a.r, a.partial, b.lid, b.pidx, a.eout,
+                                 a.pstride);
 }
 
 // one block per simulation: the finish of a block reduction (k_finish_sum)
@@ -132,6 +135,15 @@ template <class T>
 __global__ void __launch_bounds__(256) kb_submean(const ALev<T>* __restrict__ A, const BBlk* __restrict__ blk) {
   const BBlk b = blk[blockIdx.x];
   const ALev<T>& a = A[b.sim];
+  k_submean_l2_body<T>(a.g, a.t, a.L, a.r, a.e, a.sdev, a.partial, b.lid, b.pidx);
+}
+// the mean removal of a member whose residual finish flagged one (|s| > 2eps); the other members'
+// blocks return at once, keeping the ϵ and Σr² partials the residual pass formed
+template <class T>
+__global__ void __launch_bounds__(256) kb_submean_cond(const ALev<T>* __restrict__ A, const BBlk* __restrict__ blk) {
+  const BBlk b = blk[blockIdx.x];
+  const ALev<T>& a = A[b.sim];
+  if (a.sdev[1] == T(0)) return;  // the same for the whole block
   k_submean_l2_body<T>(a.g, a.t, a.L, a.r, a.e, a.sdev, a.partial, b.lid, b.pidx);
 }
 template <class T>

@@ -1740,7 +1740,7 @@ struct BatchT : BatchBase {
   const ATail<T>* tails = nullptr;
   const int* tail_sims = nullptr;
   size_t tail_lds = 0;
-  const AFin *fin_res = nullptr, *fin_l2 = nullptr;
+  const AFin *fin_res = nullptr, *fin_l2 = nullptr, *fin_l2r = nullptr;
   const ACtl* ctl = nullptr;
   int *act = nullptr, *cnt = nullptr;
   T* cflout = nullptr;
@@ -1913,7 +1913,9 @@ struct BatchT : BatchBase {
         add_blocks(b, s, zt_blocks(t));
       }
       eps0 = make_launch(a, b);
-      sub0 = eps0;
+      // the conditional mean removal writes its Σr² partials where the residual pass put its own
+      for (int s = 0; s < ns; ++s) a[s].partial += zt_blocks(make_zt(box_inside(F[s]->g)));
+      sub0 = make_launch(a, b);
     }
     {  // coarse tails (MG::launch_tail)
       std::vector<ATail<T>> a;
@@ -1949,17 +1951,19 @@ struct BatchT : BatchBase {
       }
     }
     {  // reduction finishes and solver control
-      std::vector<AFin> fr, fl;
+      std::vector<AFin> fr, fl, flr;
       std::vector<ACtl> c;
       for (int s = 0; s < ns; ++s) {
         FlowT<T>* f = F[s];
         const int64_t nb = zt_blocks(make_zt(box_inside(f->g)));
         fr.push_back(AFin{f->B.partial, nb, f->B.dsc + 0, f->B.tsc, 1, (double)ncells_inside(f->g)});
         fl.push_back(AFin{f->B.partial, nb, f->B.dsc + 1, f->B.tsc, 0, 1.0});
+        flr.push_back(AFin{f->B.partial + nb, nb, f->B.dsc + 1, f->B.tsc, 0, 1.0});
         c.push_back(ACtl{f->B.dsc, f->B.tsc, f->tol, f->itmx});
       }
       fin_res = upload(fr);
       fin_l2 = upload(fl);
+      fin_l2r = upload(flr);
       ctl = upload(c);
       act = upload(std::vector<int>(ns, 0));
       cnt = upload(std::vector<int>(ns, 0));
@@ -2039,8 +2043,11 @@ struct BatchT : BatchBase {
       f->Ut(t1[s], Ub);
       bcv[s] = ABc<T>{f->g, uB, {Ub[0], Ub[1], f->g.D > 2 ? Ub[2] : 0.0}, 0, 0, bc_gx(f->g), f->g.D};
       const ZT ti = make_zt(box_inside(f->g));
-      r1[s] = ARes<T>{f->g, ti, f->mg.lr(0), f->p, f->xw, T(1) * dt, f->sigma, uB, f->mg.lv[0].r, f->B.partial};
-      r2[s] = ARes<T>{f->g, ti, f->mg.lr(0), f->p, f->xw, T(0.5) * dt, f->sigma, uB, f->mg.lv[0].r, f->B.partial};
+      const int64_t nbr = zt_blocks(ti);
+      r1[s] = ARes<T>{f->g, ti, f->mg.lr(0), f->p, f->xw, T(1) * dt, f->sigma, uB, f->mg.lv[0].r, f->B.partial,
+                      f->mg.lv[0].e, nbr};
+      r2[s] = ARes<T>{f->g, ti, f->mg.lr(0), f->p, f->xw, T(0.5) * dt, f->sigma, uB, f->mg.lv[0].r, f->B.partial,
+                      f->mg.lv[0].e, nbr};
       const ZT ta = make_zt(box_all(f->g), 2 * zt_target_default());
       k1[s] = ACorr<T>{f->g, ta, uB, f->mg.lr(0), f->xw, f->p, T(1) * dt};
       k2[s] = ACorr<T>{f->g, ta, uB, f->mg.lr(0), f->xw, f->p, T(0.5) * dt};
@@ -2186,9 +2193,9 @@ struct BatchT : BatchBase {
     KCK();
     hipLaunchKernelGGL(kb_finish_sum<T>, dim3(ns), dim3(256), 0, st, fin_res, (const int*)nullptr);
     KCK();
-    hipLaunchKernelGGL(kb_submean<T>, dim3(sub0.nblk), kBlock, 0, st, (const ALev<T>*)sub0.args, sub0.blk);
+    hipLaunchKernelGGL(kb_submean_cond<T>, dim3(sub0.nblk), kBlock, 0, st, (const ALev<T>*)sub0.args, sub0.blk);
     KCK();
-    hipLaunchKernelGGL(kb_finish_sum<T>, dim3(ns), dim3(256), 0, st, fin_l2, (const int*)nullptr);
+    hipLaunchKernelGGL(kb_finish_sum<T>, dim3(ns), dim3(256), 0, st, fin_l2r, (const int*)nullptr);
     KCK();
     // solver! (src/MultiLevelPoisson.jl:108-127) for all members at once
     hipLaunchKernelGGL(kb_mg_init<T>, dim3((ns + 63) / 64), dim3(64), 0, st, ctl, ns, act, cnt);
