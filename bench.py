@@ -362,7 +362,9 @@ def run_c4(wl, steps, warmup, seeds=(7, 8, 9)):
     if not os.path.exists(WARMNET):
         return dict(skipped=f"no trained weights at {os.path.relpath(WARMNET, ROOT)}")
     net = wl.WarmNet.load(WARMNET, grid=(128, 64, 64), replay_pairs=1)
-    res = {"plain": [], "learned": [], "learned_predictor": []}
+    # plain; the guess in both projections or the predictor's only; the mixed-precision solver alone
+    # and with the guess (the combination whose target is 1.5x the plain sim_step!/s)
+    res = {"plain": [], "learned": [], "learned_predictor": [], "mixed": [], "learned_mixed": []}
     fdiff, vc = [], {k: [] for k in res}
     fwd_ms = None
     for seed in seeds:
@@ -370,7 +372,9 @@ def run_c4(wl, steps, warmup, seeds=(7, 8, 9)):
             sim = wl.learned.random_ellipsoid_simulation(seed)
             for _ in range(warmup):
                 wl.sim_step(sim, remeasure=False)
-            if mode != "plain":
+            if mode in ("mixed", "learned_mixed"):
+                sim.pois.set_solver(mode=wl.WL_SOLVER_MIXED)
+            if mode.startswith("learned"):
                 wl.set_warmstart(sim, net, guess=True, predictor_only=(mode == "learned_predictor"))
             n0 = len(sim.pois.n)
             t0 = time.perf_counter()
@@ -402,6 +406,8 @@ def run_c4(wl, steps, warmup, seeds=(7, 8, 9)):
                 vcycles_per_step={k: round(float(np.mean(v)), 3) for k, v in vc.items()},
                 speedup_learned=round(ms["plain"] / ms["learned"], 3),
                 speedup_learned_predictor=round(ms["plain"] / ms["learned_predictor"], 3),
+                speedup_mixed=round(ms["plain"] / ms["mixed"], 3),
+                speedup_learned_mixed=round(ms["plain"] / ms["learned_mixed"], 3),
                 pressure_force_rel_diff_max=round(max(fdiff), 6),
                 net_forward_ms=None if fwd_ms is None else round(fwd_ms, 4),
                 net_roofline=None if not fwd_ms else dict(bound="mfma", achieved=round(flops / fwd_ms / 1e9, 2),
