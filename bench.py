@@ -224,7 +224,11 @@ def copy_calibration(nbytes=1 << 30, reps=5):
 PROF_KEYS = {"k_inc_restrict_full": "inc_restrict", "k_mg_tail": "mg_tail", "k_mg_tail_lds": "mg_tail", "k_bdim2": "bdim2", "k_residual": "residual", "k_submean_l2": "submean_l2",
              "k_submean_l2_cond": "submean_l2",
              "k_inc_restrict": "inc_restrict", "k_gs_half": "gs_half", "k_increment": "increment",
-             "k_prolong_inc": "prolong_inc", "k_eps_init": "eps_init", "k_correct": "correct", "k_cfl": "cfl"}
+             "k_prolong_inc": "prolong_inc", "k_eps_init": "eps_init", "k_correct": "correct", "k_cfl": "cfl",
+             # the x-pair multigrid kernels (two cells per lane)
+             "k_gs_half2": "gs_half", "k_increment2": "increment", "k_inc_restrict2": "inc_restrict",
+             "k_prolong_inc2": "prolong_inc", "k_gs_dbl": "gs_dbl", "k_smooth4": "smooth4",
+             "k_residual64": "residual64"}
 
 
 def _key_of(kname):
