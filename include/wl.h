@@ -202,7 +202,8 @@ int wl_batch_step(wl_batch_t b, const double* dt, const double* t0, const double
  *   wl_net_train: `steps` Adam steps, each on a pair drawn from the recorded ones; returns the mean
  *     loss Σ(e−ē)²/Σ(t−t̄)² before each update.
  *   wl_flow_set_warmstart: attach (net != NULL) or detach a network; flags WL_WARM_*.  A network
- *     must stay alive while attached; the flow must be 3D with the plain solver. */
+ *     must stay alive while attached; the flow must be 3D and its level-1 grid the network's grid.
+ *     Either solver mode takes the guess (WL_SOLVER_MIXED adds it to its FP64 solution). */
 typedef struct wl_net_s* wl_net_t;
 enum wl_warm_flags {
   WL_WARM_GUESS = 1,  /* add the network's guess before the solver loop */
