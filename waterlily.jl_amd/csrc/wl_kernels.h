@@ -252,6 +252,29 @@ __global__ void __launch_bounds__(64 * CBY) k_conv_bdim1(Geo g, ZT t, const T* _
   if (D == 3 && zend == z1 && outxy) finish(z1, rpart);  // top ghost plane: no z fluxes
 }
 
+// Flux of one velocity component through the lower j face of a cell (src/Flow.jl:8-11,47,56-57),
+// from the component's values at I-2e_j, I-e_j, I, I+e_j and the advecting u_j at I and I-e_i.
+// EDGE: the face may be an inlet face (isin: central difference when the face velocity is
+// positive) or an outlet face (isout: central difference when it is negative, upwind from I-e_j
+// when it is zero); otherwise it is an inner face.
+template <class T, int S, bool EDGE>
+__device__ __forceinline__ T conv_face(T vm2, T vm1, T v0, T vp1, T ujI, T ujm, bool isin, bool isout, T nu) {
+  const T ua = (ujI + ujm) / T(2);
+  bool pos = ua > T(0), cd = false;
+  if constexpr (EDGE) {
+    cd = (isin && ua > 0) || (isout && ua < 0);
+    pos = isout ? !(ua < 0) : ua > 0;
+  }
+  T fx = pos ? vm2 : vp1, fc = pos ? vm1 : v0, fd = pos ? v0 : vm1;
+  // on inner faces the selected values are made opaque: otherwise the compiler may form λ for both
+  // upwind sides and select afterwards (twice the arithmetic)
+  if constexpr (!EDGE) asm volatile("" : "+v"(fx), "+v"(fc), "+v"(fd));
+  const T lv = lam<T, S>(fx, fc, fd);
+  T phi = ua * lv;
+  if constexpr (EDGE) phi = cd ? ua * ((v0 + vm1) / T(2)) : phi;
+  return phi - nu * (v0 - vm1);
+}
+
 // The same fused conv_diff! + accelerate! + BDIM! first loop for non-periodic flows, with every
 // stencil read from LDS or registers: each thread keeps a 4-plane window (z-2..z+1) of the three
 // velocity components at its own (x,y); plane z of the tile (+2/-1 halo in x and y) is staged in
@@ -274,6 +297,9 @@ __device__ __forceinline__ void k_conv_lds_body(const Geo& g, const ZT& t, const
   const int N0 = g.N[0], N1 = g.N[1], N2 = g.N[2];
   const bool ingrid = cx <= N0 - 1 && cy <= N1 - 1;
   const bool outxy = ingrid && tx < 63 && ty < CBY - 1;
+  // the whole tile (every lane, the supplying ones included) lies at least two cells inside the
+  // x and y walls, so its faces in x and y are inner faces
+  const bool inner_xy = x0 >= 2 && x0 + 63 <= N0 - 2 && y0 >= 2 && y0 + CBY - 1 <= N1 - 2;
   const int zend = (D == 3 && z1 + 1 <= N2 - 1) ? z1 + 1 : z1;
   const int rx = tx + 2, ry = ty + 2;
   auto ldc = [&](int i, int z) -> T {
@@ -364,10 +390,10 @@ __device__ __forceinline__ void k_conv_lds_body(const Geo& g, const ZT& t, const
     T F[3][3];
     for (int i = 0; i < 3; ++i)
       for (int j = 0; j < 3; ++j) F[i][j] = 0;
-    if (valid) {
+    auto fluxes = [&](auto edge) {
+      constexpr bool EDGE = decltype(edge)::value;
       for (int j = 0; j < D; ++j) {
-        const int Nj = g.N[j];
-        const int var = c[j] == 1 ? 1 : (c[j] == Nj - 1 ? 2 : 0);
+        const bool isin = c[j] == 1, isout = c[j] == g.N[j] - 1;
         for (int i = 0; i < D; ++i) {
           // values of component i along direction j around the face: I-2, I-1, I, I+1
           T vm2, vm1, v0, vp1;
@@ -377,17 +403,14 @@ __device__ __forceinline__ void k_conv_lds_body(const Geo& g, const ZT& t, const
           // u_j at I and at I - e_i (the advecting velocity on the i-face)
           const T ujI = (j == 2) ? W[2][2] : sU[j][ry][rx];
           const T ujm = (i == 0) ? sU[j][ry][rx - 1] : (i == 1) ? sU[j][ry - 1][rx] : W[j][1];
-          const T ua = (ujI + ujm) / T(2);
-          const T fI = v0, fm = vm1;
-          const bool cd = (var == 1 && ua > 0) || (var == 2 && ua < 0);
-          const bool pos = var == 2 ? !(ua < 0) : ua > 0;
-          const T fx = pos ? vm2 : vp1;
-          const T lv = lam<T, S>(fx, pos ? fm : fI, pos ? fI : fm);
-          const T phi = cd ? ua * ((fI + fm) / T(2)) : ua * lv;
-          F[i][j] = phi - nu * (fI - fm);
+          F[i][j] = conv_face<T, S, EDGE>(vm2, vm1, v0, vp1, ujI, ujm, isin, isout, nu);
         }
       }
-    }
+    };
+    // every face of the tile's plane is an inner face (a wave-uniform test): no inlet or outlet
+    // variants to form
+    if (inner_xy && (D == 2 || (cz >= 2 && cz <= N2 - 2))) fluxes(std::false_type{});
+    else if (valid) fluxes(std::true_type{});
     for (int i = 0; i < D; ++i) sFy[i][ty][tx] = F[i][1];
     __syncthreads();
     T Fyu[3] = {0, 0, 0};
