@@ -282,14 +282,15 @@ def test_exit_bc(wl, D):
     assert abs(float(np.mean(ex, dtype=np.float64)) - float(np.mean(inl, dtype=np.float64))) < 1e-5
 
 
-@pytest.mark.parametrize("var", ["WL_LROWS", "WL_GS_PAIR", "WL_PAIRS", "WL_GS_DBL"])
+@pytest.mark.parametrize("var", ["WL_LROWS", "WL_GS_PAIR", "WL_PAIRS", "WL_GS_DBL", "WL_SPLIT"])
 def test_fast_paths_match_plain(wl, monkeypatch, var):
     """WL_LROWS: the uniform-coefficient rows (L formed, not loaded; V and μ₁ skipped where they
     vanish; on by default); WL_GS_PAIR: two GS-RB half sweeps fused per pass with LDS (off by
     default); WL_PAIRS: the x-pair multigrid kernels (two cells per lane, on by default); WL_GS_DBL:
     two half sweeps per pass, each wave forming its neighbour rows' first colour itself (off by
-    default).  Each gives the same bits as the path without it (the variable set to 0 when the
-    solver is built), over several steps."""
+    default); WL_SPLIT: the finest level's x, r, ϵ stored by colour during a projection, with
+    one-colour half sweeps (on by default).  Each gives the same bits as the path without it (the
+    variable set to 0 when the solver is built), over several steps."""
     say(f"case: {var}=1 vs 0, 3D 64x32x32 sphere, 4 steps")
     n, c = (64, 32, 32), (24, 16, 16)
     monkeypatch.setenv(var, "1")

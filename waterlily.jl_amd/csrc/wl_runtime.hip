@@ -279,6 +279,8 @@ struct MG {
       use_gs_dbl = e5 && std::atoi(e5) != 0;
       const char* e4 = std::getenv("WL_SMOOTH4");
       use_smooth4 = e4 && std::atoi(e4) != 0;
+      const char* e8 = std::getenv("WL_SPLIT");
+      use_split = !e8 || std::atoi(e8) != 0;
     }
     lv.push_back(l0);
     if (!multilevel) return;
@@ -315,6 +317,12 @@ struct MG {
     for (size_t l = 0; l < lv.size(); ++l)  // before any graph capture: no allocation while capturing
       if (gs_pair_ok((int)l) || gs_dbl_ok((int)l)) lv[l].e2 = dalloc<T>(lv[l].g.n);
     if (fold_coarse_inc && pairs_ok(0)) lv[0].ej = dalloc<T>(lv[0].g.n);
+    if (split_ok()) {  // the colour-split x, r, ϵ, ϵ_J of the finest level (ghosts stay 0)
+      const Geo& g0 = lv[0].g;
+      sg.Q = g0.N[0] / 2 + 1;
+      sg.H = (int64_t)sg.Q * g0.N[1] * g0.N[2];
+      for (auto& b : sbuf) b = dalloc<T>(2 * sg.H);
+    }
     if (smooth4_ok()) {
       r_alt = dalloc<T>(lv[0].g.n);
       static bool sm_attr = false;  // k_smooth4's LDS ring exceeds the default 64 KB
@@ -379,6 +387,39 @@ struct MG {
   }
   // the buffer Jacobi! reads ϵ from on level l
   T* jac(int l) const { return lv[l].ej ? lv[l].ej : lv[l].e; }
+  // Colour-split finest level (SplitG): a flow's own plain 3D solver without periodic directions
+  // keeps x, r, ϵ and ϵ_J of level 0 in colour-split buffers during a projection, so the GS-RB half
+  // sweeps move whole cache lines of one colour (k_gs_half_s).  These buffers carry no state from
+  // one projection to the next (the projection's residual forms x, r, ϵ_J and the correction reads
+  // x), so each projection may choose either layout.  WL_SPLIT=0 keeps the field layout.
+  bool use_split = true;
+  bool split_on = false;  // the level-0 buffers of lv[0] are the split ones (split_begin .. split_end)
+  SplitG sg{0, 0};
+  T* sbuf[4] = {nullptr, nullptr, nullptr, nullptr};  // x, r, ϵ, ϵ_J, each 2·H
+  bool split_ok() const {
+    if (!use_split || !fold_coarse_inc || lv.size() < 2 || use_graphs || use_gs_pair || use_gs_dbl || use_smooth4)
+      return false;
+    const Geo& g0 = lv[0].g;
+    if (g0.D != 3 || perdir != 0 || !pairs_ok(0) || lv[0].ej == nullptr || lv[0].cmask != 7) return false;
+    bool in_lds = false;
+    return tail_start(&in_lds) != 0;  // the coarse tail never reads level 0
+  }
+  bool split_ready() const { return sbuf[0] != nullptr && split_ok(); }
+  void split_swap() {
+    std::swap(lv[0].x, sbuf[0]);
+    std::swap(lv[0].r, sbuf[1]);
+    std::swap(lv[0].e, sbuf[2]);
+    std::swap(lv[0].ej, sbuf[3]);
+  }
+  void split_begin() {
+    split_swap();
+    split_on = true;
+  }
+  void split_end() {
+    if (!split_on) return;
+    split_swap();
+    split_on = false;
+  }
   // x-pair kernels (k_gs_half2, k_increment2): an even inside x extent, and an even last ghosted
   // extent (half_rangek then reaches every inside cell, which the pair sweep updates); WL_PAIRS=0
   // keeps the one-cell-per-thread kernels
@@ -463,7 +504,11 @@ struct MG {
     ZT b = make_zt(box_inside(p.g));
     const int64_t nb = zt_blocks(b);
     ProfScope ps(P, st, "submean_l2", 0, (double)sizeof(T) * 3 * ncells_inside(p.g));
-    if (cond)
+    if (split_on && !cond) throw WlErr(WL_EINVAL, "internal: colour-split level 0 without the projection residual");
+    if (cond && split_on)
+      hipLaunchKernelGGL((k_submean_l2_cond<T, true>), dim3((unsigned)std::min<int64_t>(nb, 256)), kBlock, 0, st,
+                         p.g, b, lr(0), p.r, jac(0), (const T*)B->tsc, B->partial + nb, (int)nb, sg);
+    else if (cond)
       hipLaunchKernelGGL(k_submean_l2_cond<T>, dim3((unsigned)std::min<int64_t>(nb, 256)), kBlock, 0, st, p.g, b,
                          lr(0), p.r, jac(0), (const T*)B->tsc, B->partial + nb, (int)nb);
     else
@@ -479,6 +524,7 @@ struct MG {
   }
 
   void eps_init(int l) {
+    if (l == 0 && split_on) throw WlErr(WL_EINVAL, "internal: colour-split level 0 without its Jacobi ϵ");
     Level<T>& p = lv[l];
     ZT b = make_zt(box_inside(p.g));
     ProfScope ps(P, st, "eps_init", l, (double)sizeof(T) * 2 * ncells_inside(p.g));
@@ -505,6 +551,15 @@ struct MG {
   bool gs_dbl_ok(int l) const { return use_gs_dbl && perdir == 0 && lv[l].g.D == 3 && pairs_ok(l); }
   void gs_sweeps(int l) {
     Level<T>& p = lv[l];
+    if (l == 0 && split_on) {
+      ZT b = make_zt(box_pairs(p.g), tgt_gs());
+      ProfScope ps(P, st, "gs_half", l, gs_bytes(l), 4);
+      for (int it = 1; it <= 4; ++it) {
+        hipLaunchKernelGGL((k_gs_half_s<T, 3>), zt_grid(b), kBlock, 0, st, p.g, sg, b, lr(l), (const T*)p.r, p.e, it);
+        KCK();
+      }
+      return;
+    }
     if (gs_dbl_ok(l) && p.e2) {
       const Geo& g = p.g;
       ZT td;
@@ -583,7 +638,11 @@ struct MG {
 #define WL_INC2(DD, L2V, EJV)                                                                                    \
   hipLaunchKernelGGL((k_increment2<T, DD, L2V, EJV>), zt_grid(b), kBlock, 0, st, p.g, b, lr(l), p.r, p.x,         \
                      (const T*)p.e, w, B->partial, p.ej)
-      if (p.g.D == 3) {
+      if (l == 0 && split_on) {
+        if (!ejw) throw WlErr(WL_EINVAL, "internal: colour-split level 0: increment without its Jacobi ϵ");
+        hipLaunchKernelGGL((k_increment2<T, 3, true, true, true>), zt_grid(b), kBlock, 0, st, p.g, b, lr(l), p.r,
+                           p.x, (const T*)p.e, w, B->partial, p.ej, sg);
+      } else if (p.g.D == 3) {
         if (ejw) WL_INC2(3, true, true); else if (l2) WL_INC2(3, true, false); else WL_INC2(3, false, false);
       } else {
         if (ejw) WL_INC2(2, true, true); else if (l2) WL_INC2(2, true, false); else WL_INC2(2, false, false);
@@ -741,7 +800,10 @@ struct MG {
       const int64_t per = (int64_t)bc.nbx * bc.nby;
       const int nz = bc.b.hi[2] - bc.b.lo[2] + 1;
       bc.zc = (int)std::min<int64_t>(32, std::max<int64_t>(1, (int64_t)nz * per / tgt_rp()));
-      if (f.g.D == 3)
+      if (l == 0 && split_on)
+        hipLaunchKernelGGL((k_inc_restrict2<T, 3, true>), zt_grid(bc), kBlock, 0, st, f.g, c.g, bc, lr(l), f.r, f.x,
+                           (const T*)fe, lr(l + 1), c.r, c.x, c.e, sg);
+      else if (f.g.D == 3)
         hipLaunchKernelGGL((k_inc_restrict2<T, 3>), zt_grid(bc), kBlock, 0, st, f.g, c.g, bc, lr(l), f.r, f.x,
                            (const T*)fe, lr(l + 1), c.r, c.x, c.e);
       else
@@ -780,7 +842,14 @@ struct MG {
 #define WL_PRO2(DD, CI, EW)                                                                                  \
   hipLaunchKernelGGL((k_prolong_inc2<T, DD, CI, EW>), zt_grid(bp), kBlock, 0, st, f.g, lv[c].g, bp, lr(c - 1), f.r, \
                      f.x, f.e, (const T*)lv[c].x, w, (const T*)lv[c].e)
-      if (c == 1 && skip_eps0 && f.g.D == 3) {  // k_smooth4 follows and forms ϵ itself
+      if (c == 1 && split_on) {
+        if (fold)
+          hipLaunchKernelGGL((k_prolong_inc2<T, 3, true, true, true>), zt_grid(bp), kBlock, 0, st, f.g, lv[c].g, bp,
+                             lr(c - 1), f.r, f.x, f.e, (const T*)lv[c].x, w, (const T*)lv[c].e, sg);
+        else
+          hipLaunchKernelGGL((k_prolong_inc2<T, 3, false, true, true>), zt_grid(bp), kBlock, 0, st, f.g, lv[c].g, bp,
+                             lr(c - 1), f.r, f.x, f.e, (const T*)lv[c].x, w, (const T*)lv[c].e, sg);
+      } else if (c == 1 && skip_eps0 && f.g.D == 3) {  // k_smooth4 follows and forms ϵ itself
         if (fold) WL_PRO2(3, true, false); else WL_PRO2(3, false, false);
       } else if (f.g.D == 3) {
         if (fold) WL_PRO2(3, true, true); else WL_PRO2(3, false, true);
@@ -1572,19 +1641,31 @@ struct FlowT : FlowBase {
   int project(T w, T dt_end, double t1) {
     if (mode == WL_SOLVER_MIXED) return project_mixed(w, dt_end, t1);
     const T dt = w * dt_end;
+    const bool use_net = net && (!(net_flags & WL_WARM_PREDICTOR) || w == T(1));
+    // the solver's level-0 x, r, ϵ, ϵ_J colour-split for this projection (the guess of a network
+    // is formed in the field layout); swapped back on every exit
+    const bool split = !use_net && mg.split_ready();
+    struct SplitScope {
+      MG<T>* m;
+      ~SplitScope() { if (m) m->split_end(); }
+    } split_scope{split ? &mg : nullptr};
+    if (split) mg.split_begin();
     Level<T>& l0 = mg.lv[0];
     ZT bi = make_zt(box_inside(g));
     const int64_t nb = zt_blocks(bi);
     {
     // with ϵ = r·iD and the Σr² partials (at partial + nb) as submean would form them unflagged
     ProfScope ps(&prof, st, "residual", 0, (double)sizeof(T) * (D + 5) * ncells_inside(g));
-    hipLaunchKernelGGL((k_residual<T, true, true>), zt_grid(bi), kBlock, 0, st, g, bi, mg.lr(0), (const T*)p, xw, dt,
-                       1, sigma, (const T*)u, l0.r, B.partial, mg.jac(0), nb);
+    if (split)
+      hipLaunchKernelGGL((k_residual<T, true, true, true>), zt_grid(bi), kBlock, 0, st, g, bi, mg.lr(0), (const T*)p,
+                         l0.x, dt, 1, sigma, (const T*)u, l0.r, B.partial, mg.jac(0), nb, mg.sg);
+    else
+      hipLaunchKernelGGL((k_residual<T, true, true>), zt_grid(bi), kBlock, 0, st, g, bi, mg.lr(0), (const T*)p, xw,
+                         dt, 1, sigma, (const T*)u, l0.r, B.partial, mg.jac(0), nb);
     KCK();
     }
     mg.perbc(g, xw);
     mg.finish_sum(nb, 0, 1, (double)MG<T>::ninside(g));
-    const bool use_net = net && (!(net_flags & WL_WARM_PREDICTOR) || w == T(1));
     // without a guess the solver's start state is set in submean's finish (the guess would change
     // Σr²); submean itself runs only where the mean is subtracted
     mg.submean_l2(!use_net, true);
@@ -1594,7 +1675,11 @@ struct FlowT : FlowBase {
     ZT ba = make_zt(box_all(g), 2 * zt_target_default());
     {
     ProfScope ps(&prof, st, "correct", 0, (double)sizeof(T) * (2 * g.n + 2 * D * ncells_inside(g)));
-    hipLaunchKernelGGL(k_correct<T>, zt_grid(ba), kBlock, 0, st, g, ba, u, mg.lr(0), (const T*)xw, p, dt);
+    if (split)
+      hipLaunchKernelGGL((k_correct<T, true>), zt_grid(ba), kBlock, 0, st, g, ba, u, mg.lr(0), (const T*)l0.x, p, dt,
+                         mg.sg);
+    else
+      hipLaunchKernelGGL(k_correct<T>, zt_grid(ba), kBlock, 0, st, g, ba, u, mg.lr(0), (const T*)xw, p, dt);
     KCK();
     }
     apply_bc_u(t1);
