@@ -66,18 +66,15 @@ struct LPtr {
   __device__ __forceinline__ T hi(int64_t I, int i, int64_t si) const { return L[I + si + i * n]; }
 };
 
-// Colour-split storage of the finest level's x, r, ϵ in a flow's own plain solver: the cells of
-// colour k = (x + y + z) mod 2 are stored apart, cell (x, y, z) at k·H + (x+1)/2 + Q·(y + N1·z)
-// with Q = N0/2 + 1.  The x pair (2q−1, 2q) of the pair kernels shares the index q (one cell in
-// each colour); the ghosts x = 0 and x = N0−1 are q = 0 and q = N0/2.  A GS-RB half sweep then
-// reads and writes whole cache lines of the one colour it updates and of the one it reads.
+// Colour-split storage of the smoother's ϵ on the finest level of a flow's own plain solver: the
+// cells of colour k = (x + y + z) mod 2 are stored apart, cell (x, y, z) at k·H + (x+1)/2 +
+// Q·(y + N1·z) with Q = N0/2 + 1.  The x pair (2q−1, 2q) of the pair kernels shares the index q (one
+// cell in each colour); the ghosts x = 0 and x = N0−1 are q = 0 and q = N0/2.  A GS-RB half sweep
+// then reads and writes whole cache lines of the colour it reads and of the one it updates.
 struct SplitG {
   int64_t H;  // colour stride
   int Q;      // x extent of a colour row
 };
-__device__ __forceinline__ int64_t sidx(const Geo& g, const SplitG& s, int x, int y, int z) {
-  return (int64_t)((x + y + z) & 1) * s.H + ((x + 1) >> 1) + (int64_t)s.Q * (y + (int64_t)g.N[1] * z);
-}
 
 template <class T>
 __device__ __forceinline__ bool same_bits(T a, T b) {
@@ -993,13 +990,12 @@ __device__ __forceinline__ void prolong_cell(const Geo& gf, const Geo& gc, int c
 // (src/Flow.jl:225).  xin·scale is written to xout; with FLOW the source z is div(u).
 // EL2: also what k_submean_l2 forms when no mean is subtracted, from the same values over the
 // same tiles: ϵ = r·iD into eout and the block Σr² into partial[pstride + pidx].
-template <class T, bool FLOW, bool EL2 = false, bool SPLIT = false>
+template <class T, bool FLOW, bool EL2 = false>
 __device__ __forceinline__ void k_residual_body(const Geo& g, const ZT& t, const LRows<T>& LR, const T* __restrict__ xin,
                                                   T* __restrict__ xout, T scale, int write_x, T* __restrict__ z,
                                                   const T* __restrict__ u, T* __restrict__ r,
                                                   double* __restrict__ partial, int lid, int pidx,
-                                                  T* __restrict__ eout = nullptr, int64_t pstride = 0,
-                                                  SplitG sg = SplitG{0, 0}) {
+                                                  T* __restrict__ eout = nullptr, int64_t pstride = 0) {
   int c0, c1, z0, z1;
   zt_cell_id(t, lid, c0, c1, z0, z1);
   double acc = 0, acc2 = 0;
@@ -1031,14 +1027,12 @@ __device__ __forceinline__ void k_residual_body(const Geo& g, const ZT& t, const
       const T Dg = diag_at<T>(g, L, I);
       const T iD = inv_diag(Dg);
       const T rv = iD == T(0) ? T(0) : zv - mult_nb<T>(g, L, I, Dg, x0, v);
-      // SPLIT: x, r and ϵ in the colour-split layout (xin, z and u keep the field layout)
-      const int64_t SI = SPLIT ? sidx(g, sg, c0, c1, c2) : I;
       if (FLOW) z[I] = zv;
-      r[SI] = rv;
-      if (write_x) xout[SI] = x0;
+      r[I] = rv;
+      if (write_x) xout[I] = x0;
       acc += (double)rv;
       if (EL2) {
-        eout[SI] = rv * iD;
+        eout[I] = rv * iD;
         acc2 += (double)rv * (double)rv;
       }
     }
@@ -1050,22 +1044,21 @@ __device__ __forceinline__ void k_residual_body(const Geo& g, const ZT& t, const
     if (threadIdx.x == 0 && threadIdx.y == 0) partial[pstride + pidx] = s2;
   }
 }
-template <class T, bool FLOW, bool EL2 = false, bool SPLIT = false>
+template <class T, bool FLOW, bool EL2 = false>
 __global__ void __launch_bounds__(256) k_residual(Geo g, ZT t, LRows<T> LR, const T* __restrict__ xin,
                                                   T* __restrict__ xout, T scale, int write_x, T* __restrict__ z,
                                                   const T* __restrict__ u, T* __restrict__ r,
                                                   double* __restrict__ partial, T* __restrict__ eout = nullptr,
-                                                  int64_t pstride = 0, SplitG sg = SplitG{0, 0}) {
-  k_residual_body<T, FLOW, EL2, SPLIT>(g, t, LR, xin, xout, scale, write_x, z, u, r, partial, xcd_tile_id(),
-                                       (int)blockIdx.x, eout, pstride, sg);
+                                                  int64_t pstride = 0) {
+  k_residual_body<T, FLOW, EL2>(g, t, LR, xin, xout, scale, write_x, z, u, r, partial, xcd_tile_id(),
+                                (int)blockIdx.x, eout, pstride);
 }
 
 // r -= s (when |s| > 2eps) ; Σr² partials ; ϵ = r·iD (Jacobi!/GS init of the first V-cycle)
-template <class T, bool SPLIT = false>
+template <class T>
 __device__ __forceinline__ void k_submean_l2_body(const Geo& g, const ZT& t, const LRows<T>& LR, T* __restrict__ r,
                                                     T* __restrict__ e, const T* __restrict__ sdev,
-                                                    double* __restrict__ partial, int lid, int pidx,
-                                                    SplitG sg = SplitG{0, 0}) {
+                                                    double* __restrict__ partial, int lid, int pidx) {
   int c0, c1, z0, z1;
   zt_cell_id(t, lid, c0, c1, z0, z1);
   double acc = 0;
@@ -1074,10 +1067,9 @@ __device__ __forceinline__ void k_submean_l2_body(const Geo& g, const ZT& t, con
     const bool sub = sdev[1] != T(0);
     for (int c2 = z0; c2 <= z1; ++c2) {
       const int64_t I = at(g, c0, c1, c2);
-      const int64_t SI = SPLIT ? sidx(g, sg, c0, c1, c2) : I;  // r and ϵ (colour-split with SPLIT)
-      T rv = r[SI];
-      if (sub) { rv = rv - s; r[SI] = rv; }
-      e[SI] = rv * inv_diag(diag_at<T>(g, lcell(g, LR, c0, c1, c2), I));
+      T rv = r[I];
+      if (sub) { rv = rv - s; r[I] = rv; }
+      e[I] = rv * inv_diag(diag_at<T>(g, lcell(g, LR, c0, c1, c2), I));
       acc += (double)rv * (double)rv;
     }
   }
@@ -1093,14 +1085,13 @@ __global__ void __launch_bounds__(256) k_submean_l2(Geo g, ZT t, LRows<T> LR, T*
 // The same, only when the mean is subtracted, over a small grid: each block walks the blocks
 // b = blockIdx.x + k·gridDim.x of the nb-block launch above (same tiles, same partial slots), so an
 // unflagged call costs one small dispatch.
-template <class T, bool SPLIT = false>
+template <class T>
 __global__ void __launch_bounds__(256) k_submean_l2_cond(Geo g, ZT t, LRows<T> LR, T* __restrict__ r,
                                                          T* __restrict__ e, const T* __restrict__ sdev,
-                                                         double* __restrict__ partial, int nb,
-                                                         SplitG sg = SplitG{0, 0}) {
+                                                         double* __restrict__ partial, int nb) {
   if (sdev[1] == T(0)) return;
   for (int b = (int)blockIdx.x; b < nb; b += (int)gridDim.x)
-    k_submean_l2_body<T, SPLIT>(g, t, LR, r, e, sdev, partial, xcd_tile_of(b, nb), b, sg);
+    k_submean_l2_body<T>(g, t, LR, r, e, sdev, partial, xcd_tile_of(b, nb), b);
 }
 
 // ϵ = r·iD on inside (Jacobi!/GS-RB init, src/Poisson.jl:112,142)
@@ -1280,9 +1271,9 @@ __device__ __forceinline__ void st2(T* __restrict__ p, T a, T b) {
   v.y = b;
   *(typename PairV<T>::type*)p = v;
 }
-// The x pair (xa, xa+1) of a row as the pair kernels see it, in the field layout (I: the index of
-// xa) or colour-split (SPLIT, I: the shared colour index q + Q·(y + N1·z)); par = (y + z) mod 2 of
-// the row, so the colour of xa (odd) is 1 − par.  The single cells xa − 1 and xa + 2 beside the
+// The x pair (xa, xa+1) of a row of ϵ as the pair kernels see it, in the field layout (I: the index
+// of xa) or colour-split (SPLIT, I: the shared colour index q + Q·(y + N1·z)); par = (y + z) mod 2
+// of the row, so the colour of xa (odd) is 1 − par.  The single cells xa − 1 and xa + 2 beside the
 // pair are the second cell of the pair before and the first of the pair after.
 template <class T, bool SPLIT>
 struct PairMem {
@@ -1384,12 +1375,12 @@ __global__ void __launch_bounds__(256) k_gs_half2(Geo g, ZT t, LRows<T> LR, cons
   }
 }
 
-// The same half sweep on the colour-split layout: lane q updates the one cell of colour
-// k = (it + D + 1) mod 2 of the x pair (2q−1, 2q) of its row, reading r of that colour and ϵ of the
-// other colour only, so every load and store is a whole run of one colour's row.  The other
-// colour's ϵ at the lane's own index is the cell beside the updated one in x and its neighbour in
-// y and z; the second x neighbour is the next or previous lane's (a wave shuffle, or memory at the
-// wave's ends).  The arithmetic is k_gs_half2's, in the same order.
+// The same half sweep on the colour-split ϵ: lane q updates the one cell of colour k = (it + D + 1)
+// mod 2 of the x pair (2q−1, 2q) of its row (r is read as the pair, in the field layout), reading ϵ
+// of the other colour only, so every ϵ load and store is a whole run of one colour's row.  The
+// other colour's ϵ at the lane's own index is the cell beside the updated one in x and its
+// neighbour in y and z; the second x neighbour is the next or previous lane's (a wave shuffle, or
+// memory at the wave's ends).  The arithmetic is k_gs_half2's, in the same order.
 template <class T, int D>
 __global__ void __launch_bounds__(256) k_gs_half_s(Geo g, SplitG sg, ZT t, LRows<T> LR, const T* __restrict__ r,
                                                    T* __restrict__ e, int it) {
@@ -1399,9 +1390,9 @@ __global__ void __launch_bounds__(256) k_gs_half_s(Geo g, SplitG sg, ZT t, LRows
   const int k = (it + D + 1) & 1;                   // the colour this sweep updates
   const T* __restrict__ O = e + (k ? 0 : sg.H);     // ϵ of the other colour (read only)
   T* __restrict__ E = e + (k ? sg.H : 0);           // ϵ of this colour (written only)
-  const T* __restrict__ R = r + (k ? sg.H : 0);
   const int64_t s1 = sg.Q, s2 = (int64_t)sg.Q * g.N[1];
   const int64_t rowJ = ((c.xa + 1) >> 1) + s1 * c.y;
+  const int64_t rowI = c.xa + g.s[1] * c.y;  // r's row (field layout)
   T om = T(0), op = T(0);
   T o0 = O[rowJ + s2 * c.z0];
   if (D == 3) {
@@ -1410,7 +1401,7 @@ __global__ void __launch_bounds__(256) k_gs_half_s(Geo g, SplitG sg, ZT t, LRows
   }
   for (int z = c.z0; z <= c.z1; ++z) {
     const int64_t J = rowJ + s2 * z;
-    const T rv = R[J];
+    const P2<T> rr = ld2(r + rowI + g.s[2] * z);
     const T ym = O[J - s1], yp = O[J + s1];
     T on = T(0);
     if (D == 3 && z < c.z1) on = O[J + 2 * s2];
@@ -1428,7 +1419,7 @@ __global__ void __launch_bounds__(256) k_gs_half_s(Geo g, SplitG sg, ZT t, LRows
     const int ca = first ? c.xa : c.xa + 1;
     const LC<T> L = lcell(g, LR, ca, c.y, z);
     const T Dg = diag_at<T>(g, L, 0);
-    T sv = rv;
+    T sv = first ? rr.a : rr.b;
     sv -= (xm * L.l[0] + xp * L.h[0]);
     sv -= (ym * L.l[1] + yp * L.h[1]);
     if (D == 3) sv -= (om * L.l[2] + op * L.h[2]);
@@ -1659,37 +1650,40 @@ __global__ void __launch_bounds__(256) k_increment2(Geo g, ZT t, LRows<T> LR, T*
                                                     SplitG sg = SplitG{0, 0}) {
   const T w = *wp;
   const PairCell c = pair_cell(t);
-  const PairMem<T, SPLIT> M{sg.H};  // r, x, ϵ and ϵ_J colour-split with SPLIT
+  const PairMem<T, SPLIT> M{sg.H};  // ϵ colour-split with SPLIT (r, x, ϵ_J in the field layout)
   double acc = 0;
   if (c.y <= t.b.hi[1]) {
     const int lane = threadIdx.x;
-    const int64_t s1 = SPLIT ? (int64_t)sg.Q : g.s[1], s2 = SPLIT ? (int64_t)sg.Q * g.N[1] : g.s[2];
-    const int64_t rowI = (SPLIT ? (c.xa + 1) >> 1 : c.xa) + s1 * c.y;
+    const int64_t s1 = g.s[1], s2 = g.s[2];
+    const int64_t rowI = c.xa + s1 * c.y;
+    // ϵ's strides and row base (colour-split with SPLIT)
+    const int64_t t1 = SPLIT ? (int64_t)sg.Q : s1, t2 = SPLIT ? (int64_t)sg.Q * g.N[1] : s2;
+    const int64_t rowE = (SPLIT ? (c.xa + 1) >> 1 : c.xa) + t1 * c.y;
     P2<T> em{T(0), T(0)}, ep{T(0), T(0)};
-    P2<T> e0 = M.ld(e, rowI + s2 * c.z0, c.y + c.z0);
+    P2<T> e0 = M.ld(e, rowE + t2 * c.z0, c.y + c.z0);
     if (D == 3) {
-      em = M.ld(e, rowI + s2 * (c.z0 - 1), c.y + c.z0 + 1);
-      ep = M.ld(e, rowI + s2 * (c.z0 + 1), c.y + c.z0 + 1);
+      em = M.ld(e, rowE + t2 * (c.z0 - 1), c.y + c.z0 + 1);
+      ep = M.ld(e, rowE + t2 * (c.z0 + 1), c.y + c.z0 + 1);
     }
     for (int z = c.z0; z <= c.z1; ++z) {
-      const int64_t I = rowI + s2 * z;
+      const int64_t I = rowI + s2 * z, IE = rowE + t2 * z;
       const int par = c.y + z;
-      const P2<T> rr = M.ld(r, I, par), xx = M.ld(x, I, par);
-      const P2<T> ym = M.ld(e, I - s1, par + 1), yp = M.ld(e, I + s1, par + 1);
+      const P2<T> rr = ld2(r + I), xx = ld2(x + I);
+      const P2<T> ym = M.ld(e, IE - t1, par + 1), yp = M.ld(e, IE + t1, par + 1);
       P2<T> en{T(0), T(0)};
-      if (D == 3 && z < c.z1) en = M.ld(e, I + 2 * s2, par);
+      if (D == 3 && z < c.z1) en = M.ld(e, IE + 2 * t2, par);
       T xl = __shfl_up(e0.b, 1, 64), xr = __shfl_down(e0.a, 1, 64);
-      if (lane == 0) xl = M.left(e, I, par);
-      if (c.edge_r) xr = M.right(e, I, par);
+      if (lane == 0) xl = M.left(e, IE, par);
+      if (c.edge_r) xr = M.right(e, IE, par);
       const LC<T> La = lcell(g, LR, c.xa, c.y, z), Lb = lcell(g, LR, c.xa + 1, c.y, z);
       const Nb<T> va{{xl, ym.a, em.a}, {e0.b, yp.a, ep.a}}, vb{{e0.a, ym.b, em.b}, {xr, yp.b, ep.b}};
       const T Da = diag_at<T>(g, La, 0), Db = diag_at<T>(g, Lb, 0);
       const T ra = rr.a - w * mult_nb<T>(g, La, 0, Da, e0.a, va);
       const T rb = rr.b - w * mult_nb<T>(g, Lb, 0, Db, e0.b, vb);
       if (c.vx) {
-        M.st(r, I, par, ra, rb);
-        M.st(x, I, par, xx.a + w * e0.a, xx.b + w * e0.b);
-        if (EJ) M.st(ej, I, par, ra * inv_diag(Da), rb * inv_diag(Db));
+        st2(r + I, ra, rb);
+        st2(x + I, xx.a + w * e0.a, xx.b + w * e0.b);
+        if (EJ) st2(ej + I, ra * inv_diag(Da), rb * inv_diag(Db));
         if (L2) {
           acc += (double)ra * (double)ra;
           acc += (double)rb * (double)rb;
@@ -1722,7 +1716,7 @@ __global__ void __launch_bounds__(256) k_prolong_inc2(Geo gf, Geo gc, ZT t, LRow
                                                       const T* __restrict__ ec, SplitG sg = SplitG{0, 0}) {
   const T w = *wp;
   const PairCell c = pair_cell(t);
-  const PairMem<T, SPLIT> M{sg.H};  // fine r, x, ϵ colour-split with SPLIT
+  const PairMem<T, SPLIT> M{sg.H};  // fine ϵ colour-split with SPLIT (r, x in the field layout)
   if (c.y > t.b.hi[1]) return;
   const int N0 = gf.N[0], N1 = gf.N[1], N2 = gf.N[2];
   const int64_t cs1 = gc.s[1], cs2 = gc.s[2];
@@ -1735,9 +1729,10 @@ __global__ void __launch_bounds__(256) k_prolong_inc2(Geo gf, Geo gc, ZT t, LRow
   const int qx = (c.xa + 1) >> 1, qy = (c.y + 1) >> 1;
   const bool hl = c.xa - 1 >= 1, hr = c.xa + 2 <= N0 - 2, hym = c.y - 1 >= 1, hyp = c.y + 1 <= N1 - 2;
   const int qym = c.y >> 1, qyp = (c.y + 2) >> 1;
-  const int64_t s1 = SPLIT ? (int64_t)sg.Q : gf.s[1], s2 = SPLIT ? (int64_t)sg.Q * N1 : gf.s[2];
+  const int64_t s1 = gf.s[1], s2 = gf.s[2];
   for (int z = c.z0; z <= c.z1; ++z) {
-    const int64_t I = (SPLIT ? (c.xa + 1) >> 1 : c.xa) + s1 * c.y + s2 * z;
+    const int64_t I = c.xa + s1 * c.y + s2 * z;
+    const int64_t IE = SPLIT ? ((c.xa + 1) >> 1) + (int64_t)sg.Q * (c.y + (int64_t)N1 * z) : I;
     const int par = c.y + z;
     const int qz = D == 3 ? (z + 1) >> 1 : 0;
     const bool hzm = D == 3 && z - 1 >= 1, hzp = D == 3 && z + 1 <= N2 - 2;
@@ -1745,16 +1740,16 @@ __global__ void __launch_bounds__(256) k_prolong_inc2(Geo gf, Geo gc, ZT t, LRow
     const T xl = cv(qx - 1, qy, qz, hl), xr = cv(qx + 1, qy, qz, hr);
     const T ym = cv(qx, qym, qz, hym), yp = cv(qx, qyp, qz, hyp);
     const T zm = cv(qx, qy, D == 3 ? z >> 1 : 0, hzm), zp = cv(qx, qy, D == 3 ? (z + 2) >> 1 : 0, hzp);
-    const P2<T> rr = M.ld(rf, I, par), xx = M.ld(xf, I, par);
+    const P2<T> rr = ld2(rf + I), xx = ld2(xf + I);
     const LC<T> La = lcell(gf, LR, c.xa, c.y, z), Lb = lcell(gf, LR, c.xa + 1, c.y, z);
     const Nb<T> va{{xl, ym, zm}, {own, yp, zp}}, vb{{own, ym, zm}, {xr, yp, zp}};
     const T Da = diag_at<T>(gf, La, 0), Db = diag_at<T>(gf, Lb, 0);
     const T ra = rr.a - w * mult_nb<T>(gf, La, 0, Da, own, va);
     const T rb = rr.b - w * mult_nb<T>(gf, Lb, 0, Db, own, vb);
     if (c.vx) {
-      M.st(rf, I, par, ra, rb);
-      M.st(xf, I, par, xx.a + w * own, xx.b + w * own);
-      if (EW) M.st(ef, I, par, ra * inv_diag(Da), rb * inv_diag(Db));
+      st2(rf + I, ra, rb);
+      st2(xf + I, xx.a + w * own, xx.b + w * own);
+      if (EW) M.st(ef, IE, par, ra * inv_diag(Da), rb * inv_diag(Db));
     }
   }
 }
@@ -1765,12 +1760,11 @@ __global__ void __launch_bounds__(256) k_prolong_inc2(Geo gf, Geo gc, ZT t, LRow
 // 32..63 row 2Q of coarse row Q, so a coarse cell's four children of a fine plane are in lanes l
 // and l+32 and are summed with one shuffle, in the reference's z, y, x order.  Tiles are in coarse
 // units: 32 coarse x cells by 4 coarse rows (one per wave), z in coarse planes.
-template <class T, int D, bool SPLIT = false>
+template <class T, int D>
 __global__ void __launch_bounds__(256) k_inc_restrict2(Geo gf, Geo gc, ZT t, LRows<T> Lf, T* __restrict__ rf,
                                                        T* __restrict__ xf, const T* __restrict__ ef,
                                                        LRows<T> Lc, T* __restrict__ rc, T* __restrict__ xc,
-                                                       T* __restrict__ ec, SplitG sg = SplitG{0, 0}) {
-  const PairMem<T, SPLIT> M{sg.H};  // fine r, x, ϵ colour-split with SPLIT
+                                                       T* __restrict__ ec) {
   int cx, cy, z0, z1;
   zt_cell(t, cx, cy, z0, z1);  // threadIdx.x (0..63) is remapped below: the tile is 32 x wide
   const int lane = threadIdx.x, half = lane >> 5;
@@ -1781,33 +1775,32 @@ __global__ void __launch_bounds__(256) k_inc_restrict2(Geo gf, Geo gc, ZT t, LRo
   const int xa = 2 * q - 1, fy = 2 * cy - 1 + half;
   const bool lead = vx && half == 0;
   const bool edge_l = (lane & 31) == 0, edge_r = (lane & 31) == 31 || cx >= t.b.hi[0];
-  const int64_t s1 = SPLIT ? (int64_t)sg.Q : gf.s[1], s2 = SPLIT ? (int64_t)sg.Q * gf.N[1] : gf.s[2];
+  const int64_t s1 = gf.s[1], s2 = gf.s[2];
   constexpr int NZ = D == 3 ? 2 : 1;
   for (int kc = z0; kc <= z1; ++kc) {
     T ra[NZ], rb[NZ];
 #pragma unroll
     for (int qz = 0; qz < NZ; ++qz) {
       const int fz = D == 3 ? 2 * kc - 1 + qz : 0;
-      const int64_t I = (SPLIT ? q : xa) + s1 * fy + s2 * fz;
-      const int par = fy + fz;
-      const P2<T> e0 = M.ld(ef, I, par), ym = M.ld(ef, I - s1, par + 1), yp = M.ld(ef, I + s1, par + 1);
+      const int64_t I = xa + s1 * fy + s2 * fz;
+      const P2<T> e0 = ld2(ef + I), ym = ld2(ef + I - s1), yp = ld2(ef + I + s1);
       P2<T> zm{T(0), T(0)}, zp{T(0), T(0)};
       if (D == 3) {
-        zm = M.ld(ef, I - s2, par + 1);
-        zp = M.ld(ef, I + s2, par + 1);
+        zm = ld2(ef + I - s2);
+        zp = ld2(ef + I + s2);
       }
-      const P2<T> rr = M.ld(rf, I, par), xx = M.ld(xf, I, par);
+      const P2<T> rr = ld2(rf + I), xx = ld2(xf + I);
       T xl = __shfl_up(e0.b, 1, 64), xr = __shfl_down(e0.a, 1, 64);
-      if (edge_l) xl = M.left(ef, I, par);
-      if (edge_r) xr = M.right(ef, I, par);
+      if (edge_l) xl = ef[I - 1];
+      if (edge_r) xr = ef[I + 2];
       const LC<T> La = lcell(gf, Lf, xa, fy, fz), Lb = lcell(gf, Lf, xa + 1, fy, fz);
       const Nb<T> va{{xl, ym.a, zm.a}, {e0.b, yp.a, zp.a}}, vb{{e0.a, ym.b, zm.b}, {xr, yp.b, zp.b}};
       const T Da = diag_at<T>(gf, La, 0), Db = diag_at<T>(gf, Lb, 0);
       ra[qz] = rr.a - T(1) * mult_nb<T>(gf, La, 0, Da, e0.a, va);
       rb[qz] = rr.b - T(1) * mult_nb<T>(gf, Lb, 0, Db, e0.b, vb);
       if (vx) {
-        M.st(rf, I, par, ra[qz], rb[qz]);
-        M.st(xf, I, par, xx.a + T(1) * e0.a, xx.b + T(1) * e0.b);
+        st2(rf + I, ra[qz], rb[qz]);
+        st2(xf + I, xx.a + T(1) * e0.a, xx.b + T(1) * e0.b);
       }
     }
     // children summed in z, y, x order (restrict, src/MultiLevelPoisson.jl:13-19)
@@ -2606,10 +2599,9 @@ __global__ void __launch_bounds__(1024) k_mg_tail_lds(TailDesc<T> d, const T* __
 
 // mom_project! correction (src/Flow.jl:227-230): u[I,i] -= L[I,i]·(x[I]-x[I-e_i]) on inside,
 // p = x/dt on every cell.
-template <class T, bool SPLIT = false>
+template <class T>
 __device__ __forceinline__ void k_correct_body(const Geo& g, const ZT& t, T* __restrict__ u, const LRows<T>& LR,
-                                                 const T* __restrict__ x, T* __restrict__ p, T dt, int lid,
-                                                 SplitG sg = SplitG{0, 0}) {
+                                                 const T* __restrict__ x, T* __restrict__ p, T dt, int lid) {
   int c[3], z0, z1;
   zt_cell_id(t, lid, c[0], c[1], z0, z1);
   if (c[0] > t.b.hi[0] || c[1] > t.b.hi[1]) return;
@@ -2617,8 +2609,7 @@ __device__ __forceinline__ void k_correct_body(const Geo& g, const ZT& t, T* __r
     const int64_t I = at(g, c[0], c[1], c[2]);
     bool in = true;
     for (int d = 0; d < g.D; ++d) in &= c[d] >= 1 && c[d] <= g.N[d] - 2;
-    // x in the colour-split layout with SPLIT
-    const T xI = x[SPLIT ? sidx(g, sg, c[0], c[1], c[2]) : I];
+    const T xI = x[I];
     if (in) {
       // the components' u and x(I − e_i) are read before any store (a 2D level rereads direction 1)
       T uv[3], xm[3];
@@ -2626,7 +2617,7 @@ __device__ __forceinline__ void k_correct_body(const Geo& g, const ZT& t, T* __r
       for (int i = 0; i < 3; ++i) {
         const int k = dir_or_1(g, i);
         uv[i] = u[I + k * g.n];
-        xm[i] = x[SPLIT ? sidx(g, sg, c[0] - (k == 0), c[1] - (k == 1), c[2] - (k == 2)) : I - g.s[k]];
+        xm[i] = x[I - g.s[k]];
       }
       const LC<T> L = lcell(g, LR, c[0], c[1], c[2]);
       u[I] = uv[0] - L.l[0] * (xI - xm[0]);
@@ -2636,11 +2627,10 @@ __device__ __forceinline__ void k_correct_body(const Geo& g, const ZT& t, T* __r
     p[I] = xI / dt;
   }
 }
-template <class T, bool SPLIT = false>
+template <class T>
 __global__ void __launch_bounds__(256) k_correct(Geo g, ZT t, T* __restrict__ u, LRows<T> LR,
-                                                 const T* __restrict__ x, T* __restrict__ p, T dt,
-                                                 SplitG sg = SplitG{0, 0}) {
-  k_correct_body<T, SPLIT>(g, t, u, LR, x, p, dt, xcd_tile_id(), sg);
+                                                 const T* __restrict__ x, T* __restrict__ p, T dt) {
+  k_correct_body<T>(g, t, u, LR, x, p, dt, xcd_tile_id());
 }
 
 // CFL (src/Flow.jl:234-244): σ = flux_out(u) on inside; block maxima of σ over every cell.

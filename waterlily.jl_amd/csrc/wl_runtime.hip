@@ -317,11 +317,11 @@ struct MG {
     for (size_t l = 0; l < lv.size(); ++l)  // before any graph capture: no allocation while capturing
       if (gs_pair_ok((int)l) || gs_dbl_ok((int)l)) lv[l].e2 = dalloc<T>(lv[l].g.n);
     if (fold_coarse_inc && pairs_ok(0)) lv[0].ej = dalloc<T>(lv[0].g.n);
-    if (split_ok()) {  // the colour-split x, r, ϵ, ϵ_J of the finest level (ghosts stay 0)
+    if (split_ok()) {  // the colour-split ϵ of the finest level (its ghosts stay 0)
       const Geo& g0 = lv[0].g;
       sg.Q = g0.N[0] / 2 + 1;
       sg.H = (int64_t)sg.Q * g0.N[1] * g0.N[2];
-      for (auto& b : sbuf) b = dalloc<T>(2 * sg.H);
+      se = dalloc<T>(2 * sg.H);
     }
     if (smooth4_ok()) {
       r_alt = dalloc<T>(lv[0].g.n);
@@ -387,15 +387,16 @@ struct MG {
   }
   // the buffer Jacobi! reads ϵ from on level l
   T* jac(int l) const { return lv[l].ej ? lv[l].ej : lv[l].e; }
-  // Colour-split finest level (SplitG): a flow's own plain 3D solver without periodic directions
-  // keeps x, r, ϵ and ϵ_J of level 0 in colour-split buffers during a projection, so the GS-RB half
-  // sweeps move whole cache lines of one colour (k_gs_half_s).  These buffers carry no state from
-  // one projection to the next (the projection's residual forms x, r, ϵ_J and the correction reads
-  // x), so each projection may choose either layout.  WL_SPLIT=0 keeps the field layout.
+  // Colour-split smoother ϵ on the finest level (SplitG): a flow's own plain 3D solver without
+  // periodic directions keeps level 0's ϵ (the GS-RB smoother's, written by the prolongation, read
+  // by the increment) colour-split during a projection, so the half sweeps move whole cache lines
+  // of one colour (k_gs_half_s).  That ϵ carries nothing from one projection to the next (each
+  // V-cycle's prolongation forms it anew), so each projection may choose either layout; x, r and
+  // the Jacobi ϵ keep the field layout.  WL_SPLIT=0 keeps the field layout for ϵ too.
   bool use_split = true;
-  bool split_on = false;  // the level-0 buffers of lv[0] are the split ones (split_begin .. split_end)
+  bool split_on = false;  // lv[0].e is the split buffer (split_begin .. split_end)
   SplitG sg{0, 0};
-  T* sbuf[4] = {nullptr, nullptr, nullptr, nullptr};  // x, r, ϵ, ϵ_J, each 2·H
+  T* se = nullptr;  // 2·H values
   bool split_ok() const {
     if (!use_split || !fold_coarse_inc || lv.size() < 2 || use_graphs || use_gs_pair || use_gs_dbl || use_smooth4)
       return false;
@@ -404,13 +405,8 @@ struct MG {
     bool in_lds = false;
     return tail_start(&in_lds) != 0;  // the coarse tail never reads level 0
   }
-  bool split_ready() const { return sbuf[0] != nullptr && split_ok(); }
-  void split_swap() {
-    std::swap(lv[0].x, sbuf[0]);
-    std::swap(lv[0].r, sbuf[1]);
-    std::swap(lv[0].e, sbuf[2]);
-    std::swap(lv[0].ej, sbuf[3]);
-  }
+  bool split_ready() const { return se != nullptr && split_ok(); }
+  void split_swap() { std::swap(lv[0].e, se); }
   void split_begin() {
     split_swap();
     split_on = true;
@@ -504,11 +500,7 @@ struct MG {
     ZT b = make_zt(box_inside(p.g));
     const int64_t nb = zt_blocks(b);
     ProfScope ps(P, st, "submean_l2", 0, (double)sizeof(T) * 3 * ncells_inside(p.g));
-    if (split_on && !cond) throw WlErr(WL_EINVAL, "internal: colour-split level 0 without the projection residual");
-    if (cond && split_on)
-      hipLaunchKernelGGL((k_submean_l2_cond<T, true>), dim3((unsigned)std::min<int64_t>(nb, 256)), kBlock, 0, st,
-                         p.g, b, lr(0), p.r, jac(0), (const T*)B->tsc, B->partial + nb, (int)nb, sg);
-    else if (cond)
+    if (cond)
       hipLaunchKernelGGL(k_submean_l2_cond<T>, dim3((unsigned)std::min<int64_t>(nb, 256)), kBlock, 0, st, p.g, b,
                          lr(0), p.r, jac(0), (const T*)B->tsc, B->partial + nb, (int)nb);
     else
@@ -524,7 +516,6 @@ struct MG {
   }
 
   void eps_init(int l) {
-    if (l == 0 && split_on) throw WlErr(WL_EINVAL, "internal: colour-split level 0 without its Jacobi ϵ");
     Level<T>& p = lv[l];
     ZT b = make_zt(box_inside(p.g));
     ProfScope ps(P, st, "eps_init", l, (double)sizeof(T) * 2 * ncells_inside(p.g));
@@ -800,10 +791,7 @@ struct MG {
       const int64_t per = (int64_t)bc.nbx * bc.nby;
       const int nz = bc.b.hi[2] - bc.b.lo[2] + 1;
       bc.zc = (int)std::min<int64_t>(32, std::max<int64_t>(1, (int64_t)nz * per / tgt_rp()));
-      if (l == 0 && split_on)
-        hipLaunchKernelGGL((k_inc_restrict2<T, 3, true>), zt_grid(bc), kBlock, 0, st, f.g, c.g, bc, lr(l), f.r, f.x,
-                           (const T*)fe, lr(l + 1), c.r, c.x, c.e, sg);
-      else if (f.g.D == 3)
+      if (f.g.D == 3)
         hipLaunchKernelGGL((k_inc_restrict2<T, 3>), zt_grid(bc), kBlock, 0, st, f.g, c.g, bc, lr(l), f.r, f.x,
                            (const T*)fe, lr(l + 1), c.r, c.x, c.e);
       else
@@ -1642,8 +1630,8 @@ struct FlowT : FlowBase {
     if (mode == WL_SOLVER_MIXED) return project_mixed(w, dt_end, t1);
     const T dt = w * dt_end;
     const bool use_net = net && (!(net_flags & WL_WARM_PREDICTOR) || w == T(1));
-    // the solver's level-0 x, r, ϵ, ϵ_J colour-split for this projection (the guess of a network
-    // is formed in the field layout); swapped back on every exit
+    // the smoother's level-0 ϵ colour-split for this projection (not with a network's guess, whose
+    // solve keeps the field layout); swapped back on every exit
     const bool split = !use_net && mg.split_ready();
     struct SplitScope {
       MG<T>* m;
@@ -1656,12 +1644,8 @@ struct FlowT : FlowBase {
     {
     // with ϵ = r·iD and the Σr² partials (at partial + nb) as submean would form them unflagged
     ProfScope ps(&prof, st, "residual", 0, (double)sizeof(T) * (D + 5) * ncells_inside(g));
-    if (split)
-      hipLaunchKernelGGL((k_residual<T, true, true, true>), zt_grid(bi), kBlock, 0, st, g, bi, mg.lr(0), (const T*)p,
-                         l0.x, dt, 1, sigma, (const T*)u, l0.r, B.partial, mg.jac(0), nb, mg.sg);
-    else
-      hipLaunchKernelGGL((k_residual<T, true, true>), zt_grid(bi), kBlock, 0, st, g, bi, mg.lr(0), (const T*)p, xw,
-                         dt, 1, sigma, (const T*)u, l0.r, B.partial, mg.jac(0), nb);
+    hipLaunchKernelGGL((k_residual<T, true, true>), zt_grid(bi), kBlock, 0, st, g, bi, mg.lr(0), (const T*)p, xw, dt,
+                       1, sigma, (const T*)u, l0.r, B.partial, mg.jac(0), nb);
     KCK();
     }
     mg.perbc(g, xw);
@@ -1675,11 +1659,7 @@ struct FlowT : FlowBase {
     ZT ba = make_zt(box_all(g), 2 * zt_target_default());
     {
     ProfScope ps(&prof, st, "correct", 0, (double)sizeof(T) * (2 * g.n + 2 * D * ncells_inside(g)));
-    if (split)
-      hipLaunchKernelGGL((k_correct<T, true>), zt_grid(ba), kBlock, 0, st, g, ba, u, mg.lr(0), (const T*)l0.x, p, dt,
-                         mg.sg);
-    else
-      hipLaunchKernelGGL(k_correct<T>, zt_grid(ba), kBlock, 0, st, g, ba, u, mg.lr(0), (const T*)xw, p, dt);
+    hipLaunchKernelGGL(k_correct<T>, zt_grid(ba), kBlock, 0, st, g, ba, u, mg.lr(0), (const T*)xw, p, dt);
     KCK();
     }
     apply_bc_u(t1);
