@@ -868,10 +868,17 @@ struct MG {
   void autotune(int reps, const T* w) {
     if (lv.size() < 2 || reps < 1) return;
     free_graphs();  // captured iterations hold the old tile shapes
+    // the kernels a projection runs: with the colour-split ϵ when the flow's projections use it
+    const bool sp = split_ready();
+    struct SplitScope {
+      MG* m;
+      ~SplitScope() { if (m) m->split_end(); }
+    } split_scope{sp ? this : nullptr};
+    if (sp) split_begin();
     Level<T>& f = lv[0];
     Level<T>& c = lv[1];
     T* bufs[6] = {f.e, f.r, f.x, c.e, c.r, c.x};
-    const int64_t ns[6] = {f.g.n, f.g.n, f.g.n, c.g.n, c.g.n, c.g.n};
+    const int64_t ns[6] = {sp ? 2 * sg.H : f.g.n, f.g.n, f.g.n, c.g.n, c.g.n, c.g.n};
     int64_t tot = 0;
     for (int64_t v : ns) tot += v;
     T* save = dalloc<T>(tot);
