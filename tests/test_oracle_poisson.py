@@ -164,3 +164,33 @@ def test_restrict_prolong_adjoint():
         J = tuple(slice(2 * I[d] - 1, 2 * I[d] + 1) for d in range(3))
         assert rc[I] == pytest.approx(rf[J].sum(), rel=1e-15)
         assert np.all(P[J] == xc[I])
+
+
+def test_residual_mean_threshold_follows_the_exact_mean():
+    """residual! (src/Poisson.jl:91-97) subtracts the mean s = sum(r)/length(inside) unless
+    |s| <= 2eps.  The oracle (and the device, whose Σr finish it shares) sums r in FP64 and rounds
+    s to T once; the reference's `sum` adds Float32 values in an order of its own choosing.  At the
+    threshold the branch can therefore depend on the summation: here r = z (x = 0) on 64 inside
+    cells, one +64, one -64 and 62 equal small terms, has an exact mean of 2.5·eps(Float32), just
+    above 2eps, while a sequential Float32 sum absorbs every small term into the 64 and gives 0.
+    The oracle follows the exact mean and subtracts it; a Float32 order may not (a deliberate
+    deviation: the branch is parity-unpinned against the reference at this threshold)."""
+    N = (6, 6, 6)
+    pois = O.OPois(unit_coeffs(N), multilevel=False)
+    n, eps = 64, np.finfo(np.float32).eps
+    vals = np.full(n, np.float32(2.5 * eps * n / (n - 2)), dtype=np.float32)
+    vals[0], vals[-1] = np.float32(64.0), np.float32(-64.0)
+    z = np.zeros(N, np.float32, order="F")
+    z[1:-1, 1:-1, 1:-1] = vals.reshape((4, 4, 4), order="F")
+    pois.set("z", z)
+    pois.set("x", np.zeros(N, np.float32, order="F"))
+    pois.op(0)  # residual!
+    r = pois.get("r")[1:-1, 1:-1, 1:-1].reshape(-1, order="F")
+    exact = float(np.sum(vals.astype(np.float64))) / n
+    seq = np.float32(0)
+    for v in vals:
+        seq = np.float32(seq + v)
+    assert abs(exact) > 2 * eps  # the exact mean asks for the subtraction
+    assert abs(np.float32(seq) / np.float32(n)) <= 2 * eps  # a sequential Float32 sum does not
+    s = np.float32(np.float32(np.sum(vals.astype(np.float64))) / np.float32(n))
+    assert s != 0 and np.array_equal(r, (vals - s).astype(np.float32))  # the oracle subtracted it
