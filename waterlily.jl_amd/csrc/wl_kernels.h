@@ -57,6 +57,35 @@ __device__ __forceinline__ LC<T> lcell(const Geo& g, const LRows<T>& a, int x, i
   }
   return c;
 }
+// lcell with the row's uniform flag already known (a wave-uniform branch where a wave is one row):
+// a uniform row forms its coefficients and issues no L load at all; the values are lcell's.
+template <class T>
+__device__ __forceinline__ LC<T> lcell_row(const Geo& g, const LRows<T>& a, int x, int y, int z, bool u) {
+  LC<T> c;
+  if (u) {
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      c.l[i] = (i == 0 && x == 1) ? T(0) : a.c[i];
+      c.h[i] = (i == 0 && x + 1 == g.N[0] - 1) ? T(0) : a.c[i];
+    }
+  } else {
+    const int64_t I = at(g, x, y, z);
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      const int k = dir_or_1(g, i);
+      const int64_t Ih = I + g.s[k] < g.n ? I + g.s[k] : I;
+      c.l[i] = a.L[I + k * g.n];
+      c.h[i] = a.L[Ih + k * g.n];
+    }
+  }
+  return c;
+}
+// the uniform flag of row (y, z), false without flags
+template <class T>
+__device__ __forceinline__ bool row_uniform(const Geo& g, const LRows<T>& a, int y, int z) {
+  return a.rows != nullptr && a.rows[y + (int64_t)g.N[1] * z] != 0;
+}
+
 // plain pointer view (every coefficient loaded)
 template <class T>
 struct LPtr {
@@ -1379,8 +1408,8 @@ __global__ void __launch_bounds__(256) k_gs_half2(Geo g, ZT t, LRows<T> LR, cons
 // mod 2 of the x pair (2q−1, 2q) of its row (r is read as the pair, in the field layout), reading ϵ
 // of the other colour only, so every ϵ load and store is a whole run of one colour's row.  The
 // other colour's ϵ at the lane's own index is the cell beside the updated one in x and its
-// neighbour in y and z; the second x neighbour is the next or previous lane's (a wave shuffle, or
-// memory at the wave's ends).  The arithmetic is k_gs_half2's, in the same order.
+// neighbour in y and z; the second x neighbour is at the index before or after.  The arithmetic is
+// k_gs_half2's, in the same order.
 template <class T, int D>
 __global__ void __launch_bounds__(256) k_gs_half_s(Geo g, SplitG sg, ZT t, LRows<T> LR, const T* __restrict__ r,
                                                    T* __restrict__ e, int it) {
@@ -1399,25 +1428,26 @@ __global__ void __launch_bounds__(256) k_gs_half_s(Geo g, SplitG sg, ZT t, LRows
     om = O[rowJ + s2 * (c.z0 - 1)];
     op = O[rowJ + s2 * (c.z0 + 1)];
   }
+  // the uniform flags of this wave's row over its planes, lane j holding plane z0 + j (tiles have
+  // at most 32 planes), read once: each plane takes its flag with v_readlane, so no wait on a flag
+  // load (which the in-order memory counter would make a wait for the plane's field loads) sits in
+  // the loop, and a uniform row issues no L load
+  const int nz = c.z1 - c.z0 + 1;
+  const uint32_t flv = (lane < nz && row_uniform(g, LR, c.y, c.z0 + lane)) ? 1u : 0u;
   for (int z = c.z0; z <= c.z1; ++z) {
+    const bool uni = z - c.z0 < 64 ? __builtin_amdgcn_readlane(flv, z - c.z0) != 0u : row_uniform(g, LR, c.y, z);
     const int64_t J = rowJ + s2 * z;
+    const bool first = ((1 + c.y + z) & 1) == k;  // the updated cell is xa (wave-uniform)
+    // this plane's loads first (r, the other colour in y and its second x neighbour, the same
+    // cache lines as the neighbouring lanes' own), the next-but-one plane's last
     const P2<T> rr = ld2(r + rowI + g.s[2] * z);
     const T ym = O[J - s1], yp = O[J + s1];
+    const T xo = O[first ? J - 1 : J + 1];
     T on = T(0);
     if (D == 3 && z < c.z1) on = O[J + 2 * s2];
-    const bool first = ((1 + c.y + z) & 1) == k;  // the updated cell is xa (wave-uniform)
-    T xm, xp;
-    if (first) {
-      xm = __shfl_up(o0, 1, 64);
-      if (lane == 0) xm = O[J - 1];
-      xp = o0;
-    } else {
-      xm = o0;
-      xp = __shfl_down(o0, 1, 64);
-      if (c.edge_r) xp = O[J + 1];
-    }
+    const T xm = first ? xo : o0, xp = first ? o0 : xo;
     const int ca = first ? c.xa : c.xa + 1;
-    const LC<T> L = lcell(g, LR, ca, c.y, z);
+    const LC<T> L = lcell_row(g, LR, ca, c.y, z, uni);
     const T Dg = diag_at<T>(g, L, 0);
     T sv = first ? rr.a : rr.b;
     sv -= (xm * L.l[0] + xp * L.h[0]);
