@@ -1695,7 +1695,11 @@ __global__ void __launch_bounds__(256) k_increment2(Geo g, ZT t, LRows<T> LR, T*
       em = M.ld(e, rowE + t2 * (c.z0 - 1), c.y + c.z0 + 1);
       ep = M.ld(e, rowE + t2 * (c.z0 + 1), c.y + c.z0 + 1);
     }
+    // the row's uniform flags of this wave's planes, one per lane (see k_gs_half_s)
+    const int nz = c.z1 - c.z0 + 1;
+    const uint32_t flv = (lane < nz && row_uniform(g, LR, c.y, c.z0 + lane)) ? 1u : 0u;
     for (int z = c.z0; z <= c.z1; ++z) {
+      const bool uni = z - c.z0 < 64 ? __builtin_amdgcn_readlane(flv, z - c.z0) != 0u : row_uniform(g, LR, c.y, z);
       const int64_t I = rowI + s2 * z, IE = rowE + t2 * z;
       const int par = c.y + z;
       const P2<T> rr = ld2(r + I), xx = ld2(x + I);
@@ -1705,7 +1709,7 @@ __global__ void __launch_bounds__(256) k_increment2(Geo g, ZT t, LRows<T> LR, T*
       T xl = __shfl_up(e0.b, 1, 64), xr = __shfl_down(e0.a, 1, 64);
       if (lane == 0) xl = M.left(e, IE, par);
       if (c.edge_r) xr = M.right(e, IE, par);
-      const LC<T> La = lcell(g, LR, c.xa, c.y, z), Lb = lcell(g, LR, c.xa + 1, c.y, z);
+      const LC<T> La = lcell_row(g, LR, c.xa, c.y, z, uni), Lb = lcell_row(g, LR, c.xa + 1, c.y, z, uni);
       const Nb<T> va{{xl, ym.a, em.a}, {e0.b, yp.a, ep.a}}, vb{{e0.a, ym.b, em.b}, {xr, yp.b, ep.b}};
       const T Da = diag_at<T>(g, La, 0), Db = diag_at<T>(g, Lb, 0);
       const T ra = rr.a - w * mult_nb<T>(g, La, 0, Da, e0.a, va);
@@ -1760,7 +1764,11 @@ __global__ void __launch_bounds__(256) k_prolong_inc2(Geo gf, Geo gc, ZT t, LRow
   const bool hl = c.xa - 1 >= 1, hr = c.xa + 2 <= N0 - 2, hym = c.y - 1 >= 1, hyp = c.y + 1 <= N1 - 2;
   const int qym = c.y >> 1, qyp = (c.y + 2) >> 1;
   const int64_t s1 = gf.s[1], s2 = gf.s[2];
+  // the row's uniform flags of this wave's planes, one per lane (see k_gs_half_s)
+  const int nz = c.z1 - c.z0 + 1;
+  const uint32_t flv = ((int)threadIdx.x < nz && row_uniform(gf, LR, c.y, c.z0 + (int)threadIdx.x)) ? 1u : 0u;
   for (int z = c.z0; z <= c.z1; ++z) {
+    const bool uni = z - c.z0 < 64 ? __builtin_amdgcn_readlane(flv, z - c.z0) != 0u : row_uniform(gf, LR, c.y, z);
     const int64_t I = c.xa + s1 * c.y + s2 * z;
     const int64_t IE = SPLIT ? ((c.xa + 1) >> 1) + (int64_t)sg.Q * (c.y + (int64_t)N1 * z) : I;
     const int par = c.y + z;
@@ -1771,7 +1779,7 @@ __global__ void __launch_bounds__(256) k_prolong_inc2(Geo gf, Geo gc, ZT t, LRow
     const T ym = cv(qx, qym, qz, hym), yp = cv(qx, qyp, qz, hyp);
     const T zm = cv(qx, qy, D == 3 ? z >> 1 : 0, hzm), zp = cv(qx, qy, D == 3 ? (z + 2) >> 1 : 0, hzp);
     const P2<T> rr = ld2(rf + I), xx = ld2(xf + I);
-    const LC<T> La = lcell(gf, LR, c.xa, c.y, z), Lb = lcell(gf, LR, c.xa + 1, c.y, z);
+    const LC<T> La = lcell_row(gf, LR, c.xa, c.y, z, uni), Lb = lcell_row(gf, LR, c.xa + 1, c.y, z, uni);
     const Nb<T> va{{xl, ym, zm}, {own, yp, zp}}, vb{{own, ym, zm}, {xr, yp, zp}};
     const T Da = diag_at<T>(gf, La, 0), Db = diag_at<T>(gf, Lb, 0);
     const T ra = rr.a - w * mult_nb<T>(gf, La, 0, Da, own, va);
