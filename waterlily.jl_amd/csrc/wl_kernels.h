@@ -1815,11 +1815,22 @@ __global__ void __launch_bounds__(256) k_inc_restrict2(Geo gf, Geo gc, ZT t, LRo
   const bool edge_l = (lane & 31) == 0, edge_r = (lane & 31) == 31 || cx >= t.b.hi[0];
   const int64_t s1 = gf.s[1], s2 = gf.s[2];
   constexpr int NZ = D == 3 ? 2 : 1;
+  // uniform-row flags taken with v_readlane (see k_gs_half_s): lane l holds its half's fine row over
+  // fine plane 2·z0 − 1 + (l & 31) (flA) or 32 planes further (flB; a tile has at most 32 coarse
+  // planes), and the coarse row over coarse plane z0 + l (flc)
+  const int nzc = z1 - z0 + 1, nzf = D == 3 ? 2 * nzc : 1, jl = lane & 31;
+  const uint32_t flA = (jl < nzf && row_uniform(gf, Lf, fy, D == 3 ? 2 * z0 - 1 + jl : 0)) ? 1u : 0u;
+  const uint32_t flB = (32 + jl < nzf && row_uniform(gf, Lf, fy, D == 3 ? 2 * z0 + 31 + jl : 0)) ? 1u : 0u;
+  const uint32_t flc = (lane < nzc && row_uniform(gc, Lc, cy, D == 3 ? z0 + lane : 0)) ? 1u : 0u;
   for (int kc = z0; kc <= z1; ++kc) {
     T ra[NZ], rb[NZ];
 #pragma unroll
     for (int qz = 0; qz < NZ; ++qz) {
       const int fz = D == 3 ? 2 * kc - 1 + qz : 0;
+      const int jf = D == 3 ? 2 * (kc - z0) + qz : 0;  // < 64
+      const uint32_t fv = jf < 32 ? flA : flB;
+      const bool uni = half ? __builtin_amdgcn_readlane(fv, 32 + (jf & 31)) != 0u
+                            : __builtin_amdgcn_readlane(fv, jf & 31) != 0u;
       const int64_t I = xa + s1 * fy + s2 * fz;
       const P2<T> e0 = ld2(ef + I), ym = ld2(ef + I - s1), yp = ld2(ef + I + s1);
       P2<T> zm{T(0), T(0)}, zp{T(0), T(0)};
@@ -1831,7 +1842,7 @@ __global__ void __launch_bounds__(256) k_inc_restrict2(Geo gf, Geo gc, ZT t, LRo
       T xl = __shfl_up(e0.b, 1, 64), xr = __shfl_down(e0.a, 1, 64);
       if (edge_l) xl = ef[I - 1];
       if (edge_r) xr = ef[I + 2];
-      const LC<T> La = lcell(gf, Lf, xa, fy, fz), Lb = lcell(gf, Lf, xa + 1, fy, fz);
+      const LC<T> La = lcell_row(gf, Lf, xa, fy, fz, uni), Lb = lcell_row(gf, Lf, xa + 1, fy, fz, uni);
       const Nb<T> va{{xl, ym.a, zm.a}, {e0.b, yp.a, zp.a}}, vb{{e0.a, ym.b, zm.b}, {xr, yp.b, zp.b}};
       const T Da = diag_at<T>(gf, La, 0), Db = diag_at<T>(gf, Lb, 0);
       ra[qz] = rr.a - T(1) * mult_nb<T>(gf, La, 0, Da, e0.a, va);
@@ -1851,12 +1862,13 @@ __global__ void __launch_bounds__(256) k_inc_restrict2(Geo gf, Geo gc, ZT t, LRo
       s += a1;
       s += b1;
     }
+    const bool unic = __builtin_amdgcn_readlane(flc, D == 3 ? kc - z0 : 0) != 0u;
     if (lead) {
       const int ccz = D == 3 ? kc : 0;
       const int64_t Ic = at(gc, cx, cy, ccz);
       rc[Ic] = s;
       xc[Ic] = 0;
-      ec[Ic] = s * inv_diag(diag_at<T>(gc, lcell(gc, Lc, cx, cy, ccz), Ic));
+      ec[Ic] = s * inv_diag(diag_at<T>(gc, lcell_row(gc, Lc, cx, cy, ccz, unic), Ic));
     }
   }
 }
