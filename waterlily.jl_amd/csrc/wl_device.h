@@ -155,16 +155,21 @@ __device__ __forceinline__ float med3<float>(float a, float b, float c) {
   return __builtin_amdgcn_fmed3f(a, b, c);
 }
 // x/6 rounded to nearest.  For float, a reciprocal multiply with one FMA correction (Markstein)
-// equals the IEEE quotient for every |x| in [2^-100, 2^100) (checked for all floats on the CPU
-// and on the device by wl_self_test); outside that range the division is done directly.
+// equals the IEEE quotient whenever that result is a normal number (checked for all 2^32 floats
+// on the CPU, and on the device by wl_self_test): a normal quotient needs |x| >= 6*2^-126, where
+// the correction step is exact.  Zero, subnormal, infinite and NaN results take the division;
+// the test is one class compare on the quotient.
 __device__ __forceinline__ float div6(float x) {
-  const float ax = fabsf(x);
-  if (ax >= 0x1p-100f && ax < 0x1p100f) {
-    const float r = 0.16666667f;
-    const float q0 = x * r;
-    return __builtin_fmaf(__builtin_fmaf(-q0, 6.0f, x), r, q0);
+  const float r = 0.16666667f;
+  const float q0 = x * r;
+  float q = __builtin_fmaf(__builtin_fmaf(-q0, 6.0f, x), r, q0);
+  if (!__builtin_isnormal(q)) {
+    // the empty asm keeps the division inside the (rarely taken) branch: without it the compiler
+    // evaluates the full IEEE division for every lane and selects
+    asm("" : "+v"(x));
+    q = x / 6.0f;
   }
-  return x / 6.0f;
+  return q;
 }
 __device__ __forceinline__ double div6(double x) { return x / 6.0; }
 
