@@ -282,15 +282,16 @@ def test_exit_bc(wl, D):
     assert abs(float(np.mean(ex, dtype=np.float64)) - float(np.mean(inl, dtype=np.float64))) < 1e-5
 
 
-@pytest.mark.parametrize("var", ["WL_LROWS", "WL_GS_PAIR", "WL_PAIRS", "WL_GS_DBL", "WL_SPLIT"])
+@pytest.mark.parametrize("var", ["WL_LROWS", "WL_GS_PAIR", "WL_PAIRS", "WL_GS_DBL", "WL_SPLIT", "WL_FUSE_BDIM"])
 def test_fast_paths_match_plain(wl, monkeypatch, var):
     """WL_LROWS: the uniform-coefficient rows (L formed, not loaded; V and μ₁ skipped where they
     vanish; on by default); WL_GS_PAIR: two GS-RB half sweeps fused per pass with LDS (off by
     default); WL_PAIRS: the x-pair multigrid kernels (two cells per lane, on by default); WL_GS_DBL:
     two half sweeps per pass, each wave forming its neighbour rows' first colour itself (off by
-    default); WL_SPLIT: the finest level's x, r, ϵ stored by colour during a projection, with
-    one-colour half sweeps (on by default).  Each gives the same bits as the path without it (the
-    variable set to 0 when the solver is built), over several steps."""
+    default); WL_SPLIT: the finest level's smoother ϵ stored by colour during a projection, with
+    one-colour half sweeps (on by default); WL_FUSE_BDIM: BDIM! of the rows that need only f done by
+    the convection kernel (on by default).  Each gives the same bits as the path without it (the
+    variable set to 0 when the flow is built), over several steps."""
     say(f"case: {var}=1 vs 0, 3D 64x32x32 sphere, 4 steps")
     n, c = (64, 32, 32), (24, 16, 16)
     monkeypatch.setenv(var, "1")
@@ -300,7 +301,40 @@ def test_fast_paths_match_plain(wl, monkeypatch, var):
     for k in range(4):
         wl.sim_step(a, remeasure=False)
         wl.sim_step(b, remeasure=False)
-        for name in ("u", "p", "f"):
+        for name in ("u", "u0", "p", "f", "sigma"):
+            assert np.array_equal(a.flow.get(name), b.flow.get(name)), (k, name)
+    assert a.flow.Δt == b.flow.Δt and a.pois.n == b.pois.n
+
+
+@pytest.mark.parametrize("D", [2, 3])
+def test_fused_bdim_exit_and_bands(wl, monkeypatch, D):
+    """The convection kernel's BDIM! rows (WL_FUSE_BDIM, on by default) against the separate
+    BDIM! pass, bit for bit on every field: with exitBC (the exit plane the corrector's BC! keeps),
+    a body velocity V in a band of rows and μ₀ changed in a few cells (rows that are no longer
+    uniform), over several steps with remeasurement off."""
+    say(f"case: WL_FUSE_BDIM=1 vs 0, exitBC, V band, D={D}")
+    n, c, R = ((96, 48), (30, 24), 8) if D == 2 else ((64, 32, 32), (24, 16, 16), 6)
+    uBC = (1,) + (0,) * (D - 1)
+    sims = []
+    for v in ("1", "0"):
+        monkeypatch.setenv("WL_FUSE_BDIM", v)
+        sims.append(wl.Simulation(n, uBC, 2 * R, ν=2 * R / 300, body=wl.Sphere(c, R), T=np.float32, exitBC=True))
+    a, b = sims
+    V = a.flow.get("V").copy()
+    m0 = a.flow.get("mu0").copy()
+    rng = np.random.default_rng(11)
+    band = (slice(40, 60), slice(6, 12)) if D == 2 else (slice(40, 56), slice(4, 10), slice(20, 26))
+    V[band] = rng.uniform(-0.1, 0.1, size=V[band].shape).astype(V.dtype)
+    cell = (70, 30) if D == 2 else (50, 25, 8)
+    m0[cell] = 0.75
+    for s in sims:
+        s.flow.set("V", V)
+        s.flow.set("mu0", m0)
+        s.pois.update()
+    for k in range(5):
+        wl.sim_step(a, remeasure=False)
+        wl.sim_step(b, remeasure=False)
+        for name in ("u", "u0", "p", "f", "sigma"):
             assert np.array_equal(a.flow.get(name), b.flow.get(name)), (k, name)
     assert a.flow.Δt == b.flow.Δt and a.pois.n == b.pois.n
 

@@ -318,11 +318,19 @@ __device__ __forceinline__ T conv_face(T vm2, T vm1, T v0, T vp1, T ujI, T ujm, 
 // stencil read from LDS or registers: each thread keeps a 4-plane window (z-2..z+1) of the three
 // velocity components at its own (x,y); plane z of the tile (+2/-1 halo in x and y) is staged in
 // LDS, so the 9 face fluxes of a cell cost no global address arithmetic.
-template <class T, int S, bool PRED, int D, bool PF>
+//
+// FZ: BDIM!'s second loop (and the corrector's scale_u!) is done here for the cells of the x-rows
+// that need nothing but f itself: rows flagged with bit 1 of vrows hold no nonzero μ₁, V is +0
+// on them and μ₀ takes its row constants (mc0..mc2; +0 for the x-component at x = 1).  There
+// k_bdim2 would form u = u⁰·0 + (+0 + μ₀·f) (predictor) or (u + (+0 + μ₀·f))·½ (corrector) from
+// the f just computed, so it is formed here with the same operations and stored to un, a buffer
+// no thread of this launch reads; k_bdim2 then skips those rows.  Bit 0 of vrows marks V ≠ +0.
+template <class T, int S, bool PRED, int D, bool PF, bool FZ = false>
 __device__ __forceinline__ void k_conv_lds_body(const Geo& g, const ZT& t, const T* __restrict__ u, const T* __restrict__ u0,
                                                        T* __restrict__ f, T* __restrict__ sigma,
                                                        const T* __restrict__ V, const uint8_t* __restrict__ vrows,
-                                                       T nu, T dt, const Accel& acc, int lid) {
+                                                       T nu, T dt, const Accel& acc, int lid, T* __restrict__ un = nullptr,
+                                                       T mc0 = T(0), T mc1 = T(0), T mc2 = T(0)) {
   constexpr int RX = 64 + 3, RY = CBY + 3;  // staged x range x0-2..x0+64, y range y0-2..y0+CBY
   constexpr int NH = 3 * RY + 3 * 64;       // halo cells per component and plane
   constexpr int NHT = (D * NH + 64 * CBY - 1) / (64 * CBY);  // halo loads per thread
@@ -372,12 +380,15 @@ __device__ __forceinline__ void k_conv_lds_body(const Geo& g, const ZT& t, const
   for (int k = 0; k < NHT; ++k) Hc[k] = ld_halo(k, z0);
   T rpart[3] = {0, 0, 0}, Fzp[3] = {0, 0, 0}, Vp[3] = {0, 0, 0};
   T U0c[3] = {0, 0, 0};  // the corrector's u⁰ of the plane this iteration finishes
+  bool fzn = false, fzp = false;  // fused-row flags of planes cz and cz-1
 
   struct T3 { T v[3]; };
   // the corrector's u⁰ is read at the start of the iteration that finishes its plane, so the
   // load overlaps the flux evaluations (a plane earlier would need three more live registers)
-  auto finish = [&](int cz, T3 rr, T3 uvals, T3 Vv, T3 u0vals) {
+  auto finish = [&](int cz, T3 rr, T3 uvals, T3 Vv, T3 u0vals, bool fzr) {
     const int64_t I = at(g, cx, cy, cz);
+    // a fused row's cells on inside (the row itself is an inside row)
+    const bool fzc = FZ && fzr && cx >= 1 && cx <= N0 - 2;
 #pragma unroll
     for (int i = 0; i < D; ++i) {
       T r = rr.v[i];
@@ -389,7 +400,13 @@ __device__ __forceinline__ void k_conv_lds_body(const Geo& g, const ZT& t, const
       if (acc.field) r += ((const T*)acc.field)[I + i * n];
       const T uv = uvals.v[i];
       const T u0v = PRED ? uv : u0vals.v[i];
-      f[I + i * n] = u0v + dt * r - Vv.v[i];
+      const T fv = u0v + dt * r - Vv.v[i];
+      f[I + i * n] = fv;
+      if (FZ && fzc) {
+        const T l = i == 0 ? (cx == 1 ? T(0) : mc0) : i == 1 ? mc1 : mc2;
+        const T inc = (T(0) / T(2) + T(0)) + l * fv;
+        un[I + i * n] = PRED ? uv * T(0) + inc : (uv + inc) * T(0.5);
+      }
     }
   };
   auto ld_cell = [&](const T* a, int i, int z) -> T { return (outxy && z <= N2 - 1) ? a[at(g, cx, cy, z) + i * n] : T(0); };
@@ -419,9 +436,12 @@ __device__ __forceinline__ void k_conv_lds_body(const Geo& g, const ZT& t, const
       for (int k = 0; k < NHT; ++k) Hn[k] = ld_halo(k, cz + 1);
     }
     {
-      const bool vnz = outxy && cz <= N2 - 1 && vrows[cy + (int64_t)N1 * cz];  // V is +0 in unflagged rows
+      // the row's flags are the same for every lane that uses them: taken from lane 0 (whose outxy
+      // is false only when every lane's is), they are scalars
+      const int rf = __builtin_amdgcn_readfirstlane((outxy && cz <= N2 - 1) ? (int)vrows[cy + (int64_t)N1 * cz] : 0);
+      const bool vnz = (rf & 1) != 0;  // V is +0 in unflagged rows
+      fzn = FZ && (rf & 2) != 0;
       for (int i = 0; i < D; ++i) Vn[i] = vnz ? ld_cell(V, i, cz) : T(0);
-
     }
     __syncthreads();
     const int c[3] = {cx, cy, cz};
@@ -468,7 +488,7 @@ __device__ __forceinline__ void k_conv_lds_body(const Geo& g, const ZT& t, const
     if (D == 2) {
       if (outxy)
         finish(cz, T3{{rxy[0], rxy[1], rxy[2]}}, T3{{W[0][2], W[1][2], W[2][2]}}, T3{{Vn[0], Vn[1], Vn[2]}},
-               T3{{U0c[0], U0c[1], U0c[2]}});
+               T3{{U0c[0], U0c[1], U0c[2]}}, fzn);
     } else {
       if (cz > z0 && outxy) {  // plane cz-1 is complete: add its z fluxes
         const bool zin = cx >= 1 && cy >= 1 && cz - 1 >= 1 && cz - 1 <= N2 - 2;
@@ -478,9 +498,10 @@ __device__ __forceinline__ void k_conv_lds_body(const Geo& g, const ZT& t, const
           if (zin) { rr[i] += Fzp[i]; rr[i] -= F[i][2]; }
         }
         finish(cz - 1, T3{{rr[0], rr[1], rr[2]}}, T3{{W[0][1], W[1][1], W[2][1]}}, T3{{Vp[0], Vp[1], Vp[2]}},
-               T3{{U0c[0], U0c[1], U0c[2]}});
+               T3{{U0c[0], U0c[1], U0c[2]}}, fzp);
       }
       for (int i = 0; i < 3; ++i) { rpart[i] = rxy[i]; Fzp[i] = F[i][2]; Vp[i] = Vn[i]; }
+      fzp = fzn;
     }
     // Φ on the top ghost layers (component D, last direction whose inner range holds the cell)
     if (valid && outxy && cz <= z1) {
@@ -503,15 +524,16 @@ __device__ __forceinline__ void k_conv_lds_body(const Geo& g, const ZT& t, const
     if (!PRED)
       for (int i = 0; i < D; ++i) U0c[i] = ld_cell(u0, i, z1);
     finish(z1, T3{{rpart[0], rpart[1], rpart[2]}}, T3{{W[0][2], W[1][2], W[2][2]}}, T3{{Vp[0], Vp[1], Vp[2]}},
-           T3{{U0c[0], U0c[1], U0c[2]}});
+           T3{{U0c[0], U0c[1], U0c[2]}}, fzp);
   }
 }
-template <class T, int S, bool PRED, int D, bool PF>
+template <class T, int S, bool PRED, int D, bool PF, bool FZ = false>
 __global__ void __launch_bounds__(64 * CBY, sizeof(T) == 4 ? 4 : 2) k_conv_lds(Geo g, ZT t, const T* __restrict__ u, const T* __restrict__ u0,
                                                        T* __restrict__ f, T* __restrict__ sigma,
                                                        const T* __restrict__ V, const uint8_t* __restrict__ vrows,
-                                                       T nu, T dt, Accel acc) {
-  k_conv_lds_body<T, S, PRED, D, PF>(g, t, u, u0, f, sigma, V, vrows, nu, dt, acc, xcd_tile_id());
+                                                       T nu, T dt, Accel acc, T* __restrict__ un, T mc0, T mc1, T mc2) {
+  k_conv_lds_body<T, S, PRED, D, PF, FZ>(g, t, u, u0, f, sigma, V, vrows, nu, dt, acc, xcd_tile_id(), un, mc0, mc1,
+                                         mc2);
 }
 
 // BDIM! second half + scale_u!: u += μddn(μ₁,f) + V + μ₀·f on inside (src/Flow.jl:179,211-214).
@@ -521,24 +543,18 @@ __global__ void __launch_bounds__(64 * CBY, sizeof(T) == 4 ? 4 : 2) k_conv_lds(G
 // ±0·(finite difference), so the sum stays +0 and neither μ₁ nor the f-stencil is read there
 // (identical results for finite f). A wave covers 64 cells of one row, so the branch is uniform.
 // μ₀ is read through the finest level's LRows (μ₀ is its L) and V only in rows where it is not +0.
-template <class T, bool PRED, bool ADD = false>
-__device__ __forceinline__ void k_bdim2_body(const Geo& g, const ZT& t, T* __restrict__ u, const T* __restrict__ uold,
-                                               const T* __restrict__ f, const T* __restrict__ V,
-                                               const LRows<T>& mu0, const T* __restrict__ mu1,
-                                               const uint8_t* __restrict__ rows, const uint8_t* __restrict__ vrows, int lid) {
-  int c0, c1, z0, z1;
-  zt_cell_id(t, lid, c0, c1, z0, z1);
-  if (c0 > t.b.hi[0] || c1 > t.b.hi[1]) return;
-  for (int c2 = z0; c2 <= z1; ++c2) {
-  const int D = g.D;
+// One cell (c0, c1, c2) of it, every component; band and vnz are the row's flags, m0 its μ₀.
+// OUT: the corrector writes its result to uo instead of updating u in place.  DT: the dimension
+// when known at compile time (the μ₁ terms' loads are then all issued together).
+template <class T, bool PRED, bool ADD, bool OUT, int DT>
+__device__ __forceinline__ void bdim2_cell(const Geo& g, int c0, int c1, int c2, bool band, bool vnz, const LC<T>& m0,
+                                           T* __restrict__ u, const T* __restrict__ uold, const T* __restrict__ f,
+                                           const T* __restrict__ V, const T* __restrict__ mu1, T* __restrict__ uo) {
+  const int D = DT ? DT : g.D;
   const int64_t I = at(g, c0, c1, c2);
   const int64_t n = g.n;
-  const bool band = rows[c1 + (int64_t)g.N[1] * c2] != 0;
-  const bool vnz = vrows[c1 + (int64_t)g.N[1] * c2] != 0;
-  // μ₀ first (its loads are issued only for rows without the row constants), then every
-  // component's f and u (a 2D level rereads component 1), V and the μ₁ band terms where the row
-  // has them: all reads come before the first store
-  const LC<T> m0 = lcell(g, mu0, c0, c1, c2);
+  // every component's f and u (a 2D level rereads component 1), V and the μ₁ band terms where
+  // the row has them: all reads come before the first store
   T fv[3], uv[3], vv[3] = {T(0), T(0), T(0)}, sb[3] = {T(0), T(0), T(0)};
 #pragma unroll
   for (int i = 0; i < 3; ++i) {
@@ -556,6 +572,7 @@ __device__ __forceinline__ void k_bdim2_body(const Geo& g, const ZT& t, T* __res
       if (i >= D) continue;
       const int64_t k = I + i * n;
       T s = 0;
+#pragma unroll
       for (int j = 0; j < D; ++j) s += mu1[I + (i + (int64_t)D * j) * n] * (f[k + g.s[j]] - f[k - g.s[j]]);
       sb[i] = s;
     }
@@ -567,8 +584,24 @@ __device__ __forceinline__ void k_bdim2_body(const Geo& g, const ZT& t, T* __res
     const T inc = sb[i] / T(2) + (vnz ? vv[i] : T(0)) + m0.l[i] * fv[i];
     if (ADD) u[k] = uv[i] + inc;
     else if (PRED) u[k] = uv[i] * T(0) + inc;
+    else if (OUT) uo[k] = (uv[i] + inc) * T(0.5);
     else u[k] = (uv[i] + inc) * T(0.5);
   }
+}
+template <class T, bool PRED, bool ADD = false>
+__device__ __forceinline__ void k_bdim2_body(const Geo& g, const ZT& t, T* __restrict__ u, const T* __restrict__ uold,
+                                               const T* __restrict__ f, const T* __restrict__ V,
+                                               const LRows<T>& mu0, const T* __restrict__ mu1,
+                                               const uint8_t* __restrict__ rows, const uint8_t* __restrict__ vrows, int lid) {
+  int c0, c1, z0, z1;
+  zt_cell_id(t, lid, c0, c1, z0, z1);
+  if (c0 > t.b.hi[0] || c1 > t.b.hi[1]) return;
+  for (int c2 = z0; c2 <= z1; ++c2) {
+    const bool band = rows[c1 + (int64_t)g.N[1] * c2] != 0;
+    const bool vnz = vrows[c1 + (int64_t)g.N[1] * c2] != 0;
+    // μ₀ first: its loads are issued only for rows without the row constants
+    const LC<T> m0 = lcell(g, mu0, c0, c1, c2);
+    bdim2_cell<T, PRED, ADD, false, 0>(g, c0, c1, c2, band, vnz, m0, u, uold, f, V, mu1, (T*)nullptr);
   }
 }
 template <class T, bool PRED, bool ADD = false>
@@ -577,6 +610,56 @@ __global__ void __launch_bounds__(256) k_bdim2(Geo g, ZT t, T* __restrict__ u, c
                                                LRows<T> mu0, const T* __restrict__ mu1,
                                                const uint8_t* __restrict__ rows, const uint8_t* __restrict__ vrows) {
   k_bdim2_body<T, PRED, ADD>(g, t, u, uold, f, V, mu0, mu1, rows, vrows, xcd_tile_id());
+}
+// The same on the inside rows listed in rlist (the rows the convection kernel did not complete,
+// k_conv_rows): one wave per listed row and 64-cell x segment (blockIdx.y), each cell once, so no
+// wave walks a chain of planes.  The row's flags come from crows (bit 0 V, bit 2 μ₁, bit 3
+// uniform μ₀).  The corrector writes to uo, the buffer the convection kernel wrote the other
+// rows to.
+template <class T, bool PRED, int DT>
+__global__ void __launch_bounds__(256) k_bdim2_rows(Geo g, const int* __restrict__ rlist, int nlist, T* __restrict__ u,
+                                                    const T* __restrict__ uold, const T* __restrict__ f,
+                                                    const T* __restrict__ V, LRows<T> mu0, const T* __restrict__ mu1,
+                                                    const uint8_t* __restrict__ crows, T* __restrict__ uo) {
+  const int e = (int)blockIdx.x * (int)blockDim.y + (int)threadIdx.y;
+  const int c0 = 1 + 64 * (int)blockIdx.y + (int)threadIdx.x;
+  if (e >= nlist || c0 > g.N[0] - 2) return;
+  const int r = rlist[e];
+  const int c1 = r % g.N[1], c2 = r / g.N[1];
+  const int pf = crows[r];
+  const LC<T> m0 = lcell_row(g, mu0, c0, c1, c2, (pf & 8) != 0);
+  bdim2_cell<T, PRED, false, !PRED, DT>(g, c0, c1, c2, (pf & 4) != 0, (pf & 1) != 0, m0, u, uold, f, V, mu1, uo);
+}
+
+// Flags of the convection kernel's rows (k_conv_lds_body): bit 0 where V is not +0 (vrows), bit 1
+// on the inside rows with no nonzero μ₁, V +0 and uniform μ₀ (lrows, the finest level's uniform-row
+// flags; none without them), whose BDIM! the convection kernel completes; for k_bdim2's rows,
+// bit 2 where μ₁ is nonzero (rows) and bit 3 where μ₀ is uniform.
+// The inside rows without bit 1 are appended to rlist (in no particular order: each row's BDIM! is
+// independent of the others'); *cnt, zeroed beforehand, ends as their number.
+__global__ void __launch_bounds__(256) k_conv_rows(Geo g, const uint8_t* __restrict__ rows,
+                                                   const uint8_t* __restrict__ vrows,
+                                                   const uint8_t* __restrict__ lrows, uint8_t* __restrict__ crows,
+                                                   int* __restrict__ rlist, int* __restrict__ cnt) {
+  const int64_t nr = (int64_t)g.N[1] * g.N[2];
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nr) return;
+  const int y = (int)(r % g.N[1]), z = (int)(r / g.N[1]);
+  const bool inside = y >= 1 && y <= g.N[1] - 2 && (g.D == 2 || (z >= 1 && z <= g.N[2] - 2));
+  const bool v = vrows[r] != 0;
+  const bool lu = lrows != nullptr && lrows[r] != 0, b = rows[r] != 0;
+  const bool fz = inside && lu && !b && !v;
+  crows[r] = (uint8_t)((v ? 1 : 0) | (fz ? 2 : 0) | (b ? 4 : 0) | (lu ? 8 : 0));
+  if (inside && !fz) rlist[atomicAdd(cnt, 1)] = (int)r;
+}
+// the x = N0-1 plane of component 0 (every y, z), from a to b: the exit plane that BC! keeps
+// under exitBC (src/core.jl:212)
+template <class T>
+__global__ void __launch_bounds__(256) k_copy_exit_plane(Geo g, const T* __restrict__ a, T* __restrict__ b) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= (int64_t)g.N[1] * g.N[2]) return;
+  const int64_t I = (g.N[0] - 1) + (int64_t)g.N[0] * q;
+  b[I] = a[I];
 }
 
 // The step's remaining f-passes when a udf! runs between conv_diff! and accelerate! (src/Flow.jl:

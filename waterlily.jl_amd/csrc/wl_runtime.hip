@@ -368,10 +368,12 @@ struct MG {
 
   // Uniform-row flags of level l from its current L (WL_LROWS=0 disables the fast rows).
   bool use_lrows = true;
+  int lrow_epoch0 = 0;  // counts the rebuilds of the finest level's flags
   void build_lrows(int l) {
     Level<T>& p = lv[l];
     const int64_t nr = (int64_t)p.g.N[1] * p.g.N[2];
     p.lrow_ok = false;
+    if (l == 0) ++lrow_epoch0;
     if (!use_lrows) return;
     hipLaunchKernelGGL(k_lrow_bits<T>, dim3((unsigned)((nr + 3) / 4)), dim3(256), 0, st, p.g, (const T*)p.L,
                        p.lc[0], p.lc[1], p.lc[2], p.lrow);
@@ -1236,7 +1238,15 @@ struct FlowT : FlowBase {
   Geo g;
   int D;
   T *u, *u0, *f, *p, *sigma, *V, *mu0, *mu1, *xw;
-  uint8_t* rows = nullptr;  // [0, N1·N2): x-rows holding a nonzero μ₁; then rows where V is not +0
+  uint8_t* rows = nullptr;  // [0, N1·N2): x-rows holding a nonzero μ₁; then rows where V is not +0;
+                            // then the convection kernel's row flags (k_conv_rows)
+  // BDIM! of the rows that need only f is done by the convection kernel (WL_FUSE_BDIM=0 disables):
+  // the corrector writes those rows' u to u1, k_bdim2 the others, and u1 becomes u
+  T* u1 = nullptr;
+  int* rlist = nullptr;  // the inside rows left to k_bdim2_rows, then their count
+  int nlist = 0;
+  bool crows_ok = false;
+  int crows_epoch = -1;
   MG<T> mg;
   Bufs B;
   hipStream_t st;
@@ -1255,7 +1265,14 @@ struct FlowT : FlowBase {
     u = dalloc<T>(n * D); u0 = dalloc<T>(n * D); f = dalloc<T>(n * D); V = dalloc<T>(n * D);
     mu0 = dalloc<T>(n * D); mu1 = dalloc<T>(n * D * D);
     p = dalloc<T>(n); sigma = dalloc<T>(n); xw = dalloc<T>(n);
-    rows = dalloc<uint8_t>(2 * (int64_t)g.N[1] * g.N[2]);
+    rows = dalloc<uint8_t>(3 * (int64_t)g.N[1] * g.N[2]);
+    {
+      const char* e = std::getenv("WL_FUSE_BDIM");
+      if (!e || std::atoi(e) != 0) {
+        u1 = dalloc<T>(n * D);
+        rlist = dalloc<int>((int64_t)g.N[1] * g.N[2] + 1);
+      }
+    }
     alloc_bufs(B, 3 * nblocks(box_all(g)) + 16, sizeof(T));
     mg.st = st;
     mg.B = &B;
@@ -1279,7 +1296,8 @@ struct FlowT : FlowBase {
   }
   ~FlowT() override {
     (void)hipStreamSynchronize(st);
-    dfree(u); dfree(u0); dfree(f); dfree(V); dfree(mu0); dfree(mu1); dfree(p); dfree(sigma); dfree(xw); dfree(rows);
+    dfree(u); dfree(u0); dfree(u1); dfree(rlist); dfree(f); dfree(V); dfree(mu0); dfree(mu1); dfree(p); dfree(sigma); dfree(xw);
+    dfree(rows);
     dfree(mpart); dfree(fn_ubc); dfree(fn_acc[0]); dfree(fn_acc[1]);
     free_bufs(B);
     mg.lv[0].ownL = mg.lv[0].ownx = mg.lv[0].ownz = false;
@@ -1336,6 +1354,26 @@ struct FlowT : FlowBase {
     sync();
   }
   uint8_t* vrows() const { return rows + (int64_t)g.N[1] * g.N[2]; }
+  uint8_t* crows() const { return rows + 2 * (int64_t)g.N[1] * g.N[2]; }
+  // the fused BDIM! path: the non-periodic convection kernel, no udf between conv_diff! and BDIM!
+  bool fuse_bdim() const { return u1 != nullptr && desc.perdir_mask == 0 && udf == nullptr; }
+  // the row flags and the list of rows left to k_bdim2_rows; the list's length is read back (once
+  // per change of μ₀, μ₁ or V)
+  void conv_rows() {
+    const int64_t nr = (int64_t)g.N[1] * g.N[2];
+    const LRows<T> m0 = mg.lr(0);
+    int* cnt = rlist + nr;
+    hipLaunchKernelGGL(k_fill<int>, dim3(1), dim3(256), 0, st, cnt, (int64_t)1, 0);
+    KCK();
+    hipLaunchKernelGGL(k_conv_rows, dim3((unsigned)((nr + 255) / 256)), dim3(256), 0, st, g, (const uint8_t*)rows,
+                       (const uint8_t*)vrows(), m0.rows, crows(), rlist, cnt);
+    KCK();
+    HIPCK(hipMemcpyAsync(B.htsc, cnt, sizeof(int), hipMemcpyDeviceToHost, st));
+    HIPCK(hipStreamSynchronize(st));
+    std::memcpy(&nlist, B.htsc, sizeof(int));
+    crows_ok = true;
+    crows_epoch = mg.lrow_epoch0;
+  }
   void field_uploaded(int field) override {
     if (field == WL_MU1 || field == WL_V) mu1_dirty = true;
     if (field == WL_MU0) mg.lv[0].lrow_ok = false;  // L of the finest level is μ₀
@@ -1349,6 +1387,7 @@ struct FlowT : FlowBase {
     hipLaunchKernelGGL(k_mu1_rows<T>, grid_of(ba), kBlock, 0, st, g, ba, (const T*)mu1, (const T*)V, rows, rows + nr);
     KCK();
     mu1_dirty = false;
+    crows_ok = false;
   }
   void pois_update() override {
     mg.update();
@@ -1431,8 +1470,16 @@ struct FlowT : FlowBase {
     acc.raw = udf != nullptr;
     const T nu = (T)desc.nu;
     const dim3 cblk(64, CBY, 1);
+    const bool fz = fuse_bdim();
+    if (fz && (!crows_ok || crows_epoch != mg.lrow_epoch0)) conv_rows();
+    // the fused rows' u: the predictor's goes to u⁰'s buffer, which becomes u after the swap below
+    // (the predictor's convection does not read u⁰); the corrector's to u1
+    T* un = fz ? (pred ? u0 : u1) : nullptr;
+    const LRows<T> m0 = mg.lr(0);
+    const uint8_t* vr = fz ? (const uint8_t*)crows() : (const uint8_t*)vrows();
     {
-      ProfScope ps(&prof, st, pred ? "conv_pred" : "conv_corr", 0, (double)sizeof(T) * (pred ? 2 : 3) * D * g.n);
+      ProfScope ps(&prof, st, pred ? "conv_pred" : "conv_corr", 0,
+                   (double)sizeof(T) * ((pred ? 2 : 3) + (fz ? 1 : 0)) * D * g.n);
       if (desc.perdir_mask) {
         if (pred)
           hipLaunchKernelGGL((k_conv_bdim1<T, S, true, DD>), zt_grid(ba), cblk, 0, st, g, ba, (const T*)u, (const T*)u0, f,
@@ -1440,20 +1487,27 @@ struct FlowT : FlowBase {
         else
           hipLaunchKernelGGL((k_conv_bdim1<T, S, false, DD>), zt_grid(ba), cblk, 0, st, g, ba, (const T*)u, (const T*)u0, f,
                              sigma, (const T*)V, (const uint8_t*)vrows(), nu, dt, desc.perdir_mask, acc);
+      } else if (fz) {
+        if (pred)
+          hipLaunchKernelGGL((k_conv_lds<T, S, true, DD, true, true>), zt_grid(ba), cblk, 0, st, g, ba, (const T*)u,
+                             (const T*)u0, f, sigma, (const T*)V, vr, nu, dt, acc, un, m0.c[0], m0.c[1], m0.c[2]);
+        else
+          hipLaunchKernelGGL((k_conv_lds<T, S, false, DD, true, true>), zt_grid(ba), cblk, 0, st, g, ba, (const T*)u,
+                             (const T*)u0, f, sigma, (const T*)V, vr, nu, dt, acc, un, m0.c[0], m0.c[1], m0.c[2]);
       } else if (conv_prefetch()) {
         if (pred)
           hipLaunchKernelGGL((k_conv_lds<T, S, true, DD, true>), zt_grid(ba), cblk, 0, st, g, ba, (const T*)u, (const T*)u0, f,
-                             sigma, (const T*)V, (const uint8_t*)vrows(), nu, dt, acc);
+                             sigma, (const T*)V, vr, nu, dt, acc, (T*)nullptr, T(0), T(0), T(0));
         else
           hipLaunchKernelGGL((k_conv_lds<T, S, false, DD, true>), zt_grid(ba), cblk, 0, st, g, ba, (const T*)u, (const T*)u0, f,
-                             sigma, (const T*)V, (const uint8_t*)vrows(), nu, dt, acc);
+                             sigma, (const T*)V, vr, nu, dt, acc, (T*)nullptr, T(0), T(0), T(0));
       } else {
         if (pred)
           hipLaunchKernelGGL((k_conv_lds<T, S, true, DD, false>), zt_grid(ba), cblk, 0, st, g, ba, (const T*)u, (const T*)u0, f,
-                             sigma, (const T*)V, (const uint8_t*)vrows(), nu, dt, acc);
+                             sigma, (const T*)V, vr, nu, dt, acc, (T*)nullptr, T(0), T(0), T(0));
         else
           hipLaunchKernelGGL((k_conv_lds<T, S, false, DD, false>), zt_grid(ba), cblk, 0, st, g, ba, (const T*)u, (const T*)u0, f,
-                             sigma, (const T*)V, (const uint8_t*)vrows(), nu, dt, acc);
+                             sigma, (const T*)V, vr, nu, dt, acc, (T*)nullptr, T(0), T(0), T(0));
       }
       KCK();
     }
@@ -1464,14 +1518,42 @@ struct FlowT : FlowBase {
       run_udf(pred, dt, t);
       return;
     }
+    if (fz) {
+      {
+        ProfScope ps(&prof, st, "bdim2", 0, (double)sizeof(T) * 3 * D * (double)nlist * (g.N[0] - 2));
+        if (nlist > 0) {
+          const dim3 grid((unsigned)((nlist + BY - 1) / BY), (unsigned)((g.N[0] - 2 + 63) / 64));
+          if (pred)
+            hipLaunchKernelGGL((k_bdim2_rows<T, true, DD>), grid, kBlock, 0, st, g, (const int*)rlist, nlist, u,
+                               (const T*)u0, (const T*)f, (const T*)V, m0, (const T*)mu1, (const uint8_t*)crows(),
+                               (T*)nullptr);
+          else
+            hipLaunchKernelGGL((k_bdim2_rows<T, false, DD>), grid, kBlock, 0, st, g, (const int*)rlist, nlist, u,
+                               (const T*)u0, (const T*)f, (const T*)V, m0, (const T*)mu1, (const uint8_t*)crows(), u1);
+          KCK();
+        }
+      }
+      if (!pred) {
+        // u1 holds the corrector's u on inside; BC! rewrites every ghost from inside values except
+        // the exit plane it keeps under exitBC (src/core.jl:212), which is carried over first
+        if (desc.exit_bc) {
+          const int64_t np = (int64_t)g.N[1] * g.N[2];
+          hipLaunchKernelGGL(k_copy_exit_plane<T>, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, st, g,
+                             (const T*)u, u1);
+          KCK();
+        }
+        std::swap(u, u1);
+      }
+      return;
+    }
     ZT bi = make_zt(box_inside(g), 2 * zt_target_default());
     ProfScope ps(&prof, st, "bdim2", 0, (double)sizeof(T) * 3 * D * ncells_inside(g));
     if (pred)
       hipLaunchKernelGGL((k_bdim2<T, true>), zt_grid(bi), kBlock, 0, st, g, bi, u, (const T*)u0, (const T*)f,
-                         (const T*)V, mg.lr(0), (const T*)mu1, (const uint8_t*)rows, (const uint8_t*)vrows());
+                         (const T*)V, m0, (const T*)mu1, (const uint8_t*)rows, (const uint8_t*)vrows());
     else
       hipLaunchKernelGGL((k_bdim2<T, false>), zt_grid(bi), kBlock, 0, st, g, bi, u, (const T*)u0, (const T*)f,
-                         (const T*)V, mg.lr(0), (const T*)mu1, (const uint8_t*)rows, (const uint8_t*)vrows());
+                         (const T*)V, m0, (const T*)mu1, (const uint8_t*)rows, (const uint8_t*)vrows());
     KCK();
   }
   // A udf! (src/Flow.jl:193,207,255-257) between conv_diff! and accelerate!: the flow holds the
