@@ -324,7 +324,9 @@ __device__ __forceinline__ T conv_face(T vm2, T vm1, T v0, T vp1, T ujI, T ujm, 
 // on them and μ₀ takes its row constants (mc0..mc2; +0 for the x-component at x = 1).  There
 // k_bdim2 would form u = u⁰·0 + (+0 + μ₀·f) (predictor) or (u + (+0 + μ₀·f))·½ (corrector) from
 // the f just computed, so it is formed here with the same operations and stored to un, a buffer
-// no thread of this launch reads; k_bdim2 then skips those rows.  Bit 0 of vrows marks V ≠ +0.
+// no thread of this launch reads; k_bdim2 then skips those rows.  Bit 0 of vrows marks V ≠ +0;
+// bit 4 marks the fused rows whose f the predictor's k_bdim2_rows does not read (no μ₁ row among
+// their y and z neighbours), where the predictor stores no f.
 template <class T, int S, bool PRED, int D, bool PF, bool FZ = false>
 __device__ __forceinline__ void k_conv_lds_body(const Geo& g, const ZT& t, const T* __restrict__ u, const T* __restrict__ u0,
                                                        T* __restrict__ f, T* __restrict__ sigma,
@@ -380,15 +382,18 @@ __device__ __forceinline__ void k_conv_lds_body(const Geo& g, const ZT& t, const
   for (int k = 0; k < NHT; ++k) Hc[k] = ld_halo(k, z0);
   T rpart[3] = {0, 0, 0}, Fzp[3] = {0, 0, 0}, Vp[3] = {0, 0, 0};
   T U0c[3] = {0, 0, 0};  // the corrector's u⁰ of the plane this iteration finishes
-  bool fzn = false, fzp = false;  // fused-row flags of planes cz and cz-1
+  int fzn = 0, fzp = 0;  // the row flags of planes cz and cz-1 (FZ)
 
   struct T3 { T v[3]; };
   // the corrector's u⁰ is read at the start of the iteration that finishes its plane, so the
   // load overlaps the flux evaluations (a plane earlier would need three more live registers)
-  auto finish = [&](int cz, T3 rr, T3 uvals, T3 Vv, T3 u0vals, bool fzr) {
+  auto finish = [&](int cz, T3 rr, T3 uvals, T3 Vv, T3 u0vals, int fr) {
     const int64_t I = at(g, cx, cy, cz);
     // a fused row's cells on inside (the row itself is an inside row)
-    const bool fzc = FZ && fzr && cx >= 1 && cx <= N0 - 2;
+    const bool fzc = FZ && (fr & 2) != 0 && cx >= 1 && cx <= N0 - 2;
+    // the predictor's f of a fused row that no μ₁ row reads is not stored: the corrector's
+    // convection overwrites f on every cell before anything else reads it
+    const bool keepf = !(FZ && PRED && (fr & 16) != 0);
 #pragma unroll
     for (int i = 0; i < D; ++i) {
       T r = rr.v[i];
@@ -401,7 +406,7 @@ __device__ __forceinline__ void k_conv_lds_body(const Geo& g, const ZT& t, const
       const T uv = uvals.v[i];
       const T u0v = PRED ? uv : u0vals.v[i];
       const T fv = u0v + dt * r - Vv.v[i];
-      f[I + i * n] = fv;
+      if (keepf) f[I + i * n] = fv;
       if (FZ && fzc) {
         const T l = i == 0 ? (cx == 1 ? T(0) : mc0) : i == 1 ? mc1 : mc2;
         const T inc = (T(0) / T(2) + T(0)) + l * fv;
@@ -440,7 +445,7 @@ __device__ __forceinline__ void k_conv_lds_body(const Geo& g, const ZT& t, const
       // is false only when every lane's is), they are scalars
       const int rf = __builtin_amdgcn_readfirstlane((outxy && cz <= N2 - 1) ? (int)vrows[cy + (int64_t)N1 * cz] : 0);
       const bool vnz = (rf & 1) != 0;  // V is +0 in unflagged rows
-      fzn = FZ && (rf & 2) != 0;
+      fzn = FZ ? rf : 0;
       for (int i = 0; i < D; ++i) Vn[i] = vnz ? ld_cell(V, i, cz) : T(0);
     }
     __syncthreads();
@@ -634,7 +639,8 @@ __global__ void __launch_bounds__(256) k_bdim2_rows(Geo g, const int* __restrict
 // Flags of the convection kernel's rows (k_conv_lds_body): bit 0 where V is not +0 (vrows), bit 1
 // on the inside rows with no nonzero μ₁, V +0 and uniform μ₀ (lrows, the finest level's uniform-row
 // flags; none without them), whose BDIM! the convection kernel completes; for k_bdim2's rows,
-// bit 2 where μ₁ is nonzero (rows) and bit 3 where μ₀ is uniform.
+// bit 2 where μ₁ is nonzero (rows) and bit 3 where μ₀ is uniform; bit 4 on the fused rows with
+// no μ₁ row among their y and z neighbours.
 // The inside rows without bit 1 are appended to rlist (in no particular order: each row's BDIM! is
 // independent of the others'); *cnt, zeroed beforehand, ends as their number.
 __global__ void __launch_bounds__(256) k_conv_rows(Geo g, const uint8_t* __restrict__ rows,
@@ -649,7 +655,10 @@ __global__ void __launch_bounds__(256) k_conv_rows(Geo g, const uint8_t* __restr
   const bool v = vrows[r] != 0;
   const bool lu = lrows != nullptr && lrows[r] != 0, b = rows[r] != 0;
   const bool fz = inside && lu && !b && !v;
-  crows[r] = (uint8_t)((v ? 1 : 0) | (fz ? 2 : 0) | (b ? 4 : 0) | (lu ? 8 : 0));
+  // a μ₁ row reads f of its y and z neighbour rows (μddn, src/Flow.jl:20-24)
+  bool nb = (y >= 1 && rows[r - 1] != 0) || (y + 1 < g.N[1] && rows[r + 1] != 0);
+  if (g.D == 3) nb = nb || (z >= 1 && rows[r - g.N[1]] != 0) || (z + 1 < g.N[2] && rows[r + g.N[1]] != 0);
+  crows[r] = (uint8_t)((v ? 1 : 0) | (fz ? 2 : 0) | (b ? 4 : 0) | (lu ? 8 : 0) | (fz && !nb ? 16 : 0));
   if (inside && !fz) rlist[atomicAdd(cnt, 1)] = (int)r;
 }
 // the x = N0-1 plane of component 0 (every y, z), from a to b: the exit plane that BC! keeps
