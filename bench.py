@@ -228,7 +228,9 @@ PROF_KEYS = {"k_inc_restrict_full": "inc_restrict", "k_mg_tail": "mg_tail", "k_m
              # the x-pair multigrid kernels (two cells per lane)
              "k_gs_half2": "gs_half", "k_increment2": "increment", "k_inc_restrict2": "inc_restrict",
              "k_prolong_inc2": "prolong_inc", "k_gs_half_s": "gs_half", "k_gs_dbl": "gs_dbl", "k_smooth4": "smooth4",
-             "k_residual64": "residual64"}
+             "k_residual64": "residual64",
+             # BDIM!'s rows the convection kernel leaves (a list of rows), and their flags
+             "k_bdim2_rows": "bdim2", "k_conv_rows": "conv_rows", "k_copy_exit_plane": "exit_plane"}
 
 
 def _key_of(kname):

@@ -1478,8 +1478,10 @@ struct FlowT : FlowBase {
     const LRows<T> m0 = mg.lr(0);
     const uint8_t* vr = fz ? (const uint8_t*)crows() : (const uint8_t*)vrows();
     {
+      // algorithmic bytes: u read and f written (the predictor), also u⁰ read (the corrector); fused,
+      // the predictor writes u instead of f (f is kept only next to μ₁ rows) and the corrector both
       ProfScope ps(&prof, st, pred ? "conv_pred" : "conv_corr", 0,
-                   (double)sizeof(T) * ((pred ? 2 : 3) + (fz ? 1 : 0)) * D * g.n);
+                   (double)sizeof(T) * (pred ? 2 : (fz ? 4 : 3)) * D * g.n);
       if (desc.perdir_mask) {
         if (pred)
           hipLaunchKernelGGL((k_conv_bdim1<T, S, true, DD>), zt_grid(ba), cblk, 0, st, g, ba, (const T*)u, (const T*)u0, f,
