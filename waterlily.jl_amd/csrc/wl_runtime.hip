@@ -1773,7 +1773,8 @@ struct FlowT : FlowBase {
   // CFL(a) (src/Flow.jl:234-237) of the current u: min(10, 1/(max flux out of a cell + 5ν)),
   // formed in T; σ is its scratch (its stale ghosts included, as in the reference)
   double cfl() override {
-    ZT ba = make_zt(box_all(g));
+    // shorter z runs than the default tiles (9 planes at 512x256x256): 211 -> 186 us there
+    ZT ba = make_zt(box_all(g), 4 * zt_target_default());
     const int64_t nb = zt_blocks(ba);
     T* tpart = (T*)B.partial;
     {
