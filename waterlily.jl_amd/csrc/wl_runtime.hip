@@ -107,6 +107,14 @@ static ZT make_zt(const Box& b, int target = 0) {
   t.zc = (int)std::min<int64_t>(32, std::max<int64_t>(1, zc));
   return t;
 }
+// planes per convection tile (WL_CONV_ZC overrides the choice of conv_zt, for tuning); 0: unset
+static int conv_zc_override() {
+  static int v = [] {
+    const char* e = std::getenv("WL_CONV_ZC");
+    return e ? std::max(1, std::min(32, std::atoi(e))) : 0;
+  }();
+  return v;
+}
 // conv kernel variant: prefetch the next plane one iteration ahead (WL_CONV_PREFETCH=0 disables)
 static bool conv_prefetch() {
   static bool v = [] {
@@ -1401,8 +1409,12 @@ struct FlowT : FlowBase {
     ba.sy = CBY - 1;
     ba.nbx = (g.N[0] + 62) / 63;
     ba.nby = (g.N[1] + CBY - 2) / (CBY - 1);
-    const int64_t per = (int64_t)ba.nbx * ba.nby;
-    ba.zc = (int)std::min<int64_t>(32, std::max<int64_t>(1, (int64_t)g.N[2] * per / (zt_target_default() / 2)));
+    // planes per tile: about 13 tiles along z, measured best at both sizes swept on the MI355X
+    // (512x256x256: 20 planes, predictor 626 us against 685 at 32 and 660 at 18; 256x128x128:
+    // 10 planes, 102 us against 110 at 7 and 113 at 15).  A model of rounds of resident blocks
+    // times the planes each loads picked 29 and 15, right for the first only.
+    ba.zc = std::max(1, std::min(32, (g.N[2] + 12) / 13));
+    if (conv_zc_override()) ba.zc = conv_zc_override();
     return ba;
   }
   // function-valued uBC and forcing (host-evaluated fields, wl_flow_set_field_fn): uBC at t₁,
