@@ -91,8 +91,9 @@ static int zt_target_default() {
   return v;
 }
 // z-marching tiles over a box: enough planes per tile to keep ~target blocks in flight.  The
-// restriction, prolongation, BDIM and correction kernels take twice the default (shorter z runs):
-// measured 2-11% faster for them on the 256x128x128 sphere, slower for the others.
+// restriction, prolongation and BDIM kernels take twice the default (shorter z runs): measured
+// 2-11% faster for them on the 256x128x128 sphere, slower for the others.  The correction kernel
+// takes eight times, the CFL kernel four (512x256x256: correction 303 -> 262 us, CFL 211 -> 185 us).
 static ZT make_zt(const Box& b, int target = 0) {
   if (target <= 0) target = zt_target_default();
   ZT t;
@@ -1655,7 +1656,7 @@ struct FlowT : FlowBase {
     KCK();
     mg.perbc(g, xw);
     if (use_net && (net_flags & WL_WARM_RECORD)) net->record<T>(g, (const T*)xw, st);
-    ZT ba = make_zt(box_all(g), 2 * zt_target_default());
+    ZT ba = make_zt(box_all(g), 8 * zt_target_default());
     {
     ProfScope ps(&prof, st, "correct", 0, (double)sizeof(T) * (2 * g.n + 2 * D * ncells_inside(g)));
     hipLaunchKernelGGL(k_correct<T>, zt_grid(ba), kBlock, 0, st, g, ba, u, mg.lr(0), (const T*)xw, p, dt);
@@ -1759,7 +1760,7 @@ struct FlowT : FlowBase {
     if (use_net) warm_guess();
     const int nit = mg.mg_loop(tol, itmx, &r2hist, !use_net);
     if (use_net && (net_flags & WL_WARM_RECORD)) net->record<T>(g, (const T*)xw, st);
-    ZT ba = make_zt(box_all(g), 2 * zt_target_default());
+    ZT ba = make_zt(box_all(g), 8 * zt_target_default());
     {
     ProfScope ps(&prof, st, "correct", 0, (double)sizeof(T) * (2 * g.n + 2 * D * ncells_inside(g)));
     hipLaunchKernelGGL(k_correct<T>, zt_grid(ba), kBlock, 0, st, g, ba, u, mg.lr(0), (const T*)xw, p, dt);
