@@ -2855,10 +2855,12 @@ __device__ __forceinline__ void mg_control_state(double r2sum, double* __restric
   dsc[6] = (double)r2;
 }
 template <class T>
-__global__ void __launch_bounds__(256) k_finish_sum(const double* __restrict__ partial, int64_t np,
-                                                    double* __restrict__ outd, T* __restrict__ outT, int mode,
-                                                    double ninside, double* __restrict__ dsc = nullptr) {
+__global__ void __launch_bounds__(1024) k_finish_sum(const double* __restrict__ partial, int64_t np,
+                                                     double* __restrict__ outd, T* __restrict__ outT, int mode,
+                                                     double ninside, double* __restrict__ dsc = nullptr) {
+  // each thread's partials in index order; the loads run ahead of the dependent adds
   double acc = 0;
+#pragma unroll 8
   for (int64_t q = threadIdx.x; q < np; q += blockDim.x) acc += partial[q];
   const double s = block_sum(acc);
   if (threadIdx.x == 0) {
@@ -2874,8 +2876,9 @@ __global__ void __launch_bounds__(256) k_finish_sum(const double* __restrict__ p
   }
 }
 template <class T>
-__global__ void __launch_bounds__(256) k_finish_max(const T* __restrict__ partial, int64_t np, T* __restrict__ out) {
+__global__ void __launch_bounds__(1024) k_finish_max(const T* __restrict__ partial, int64_t np, T* __restrict__ out) {
   T m = -T(INFINITY);
+#pragma unroll 8
   for (int64_t q = threadIdx.x; q < np; q += blockDim.x) m = partial[q] > m ? partial[q] : m;
   m = block_max<T>(m, -T(INFINITY));
   if (threadIdx.x == 0) out[0] = m;
@@ -3250,9 +3253,10 @@ __global__ void __launch_bounds__(256) k_residual64(Geo g, ZT t, LRows<T> LR, co
 // Σr, the mean s = Σr/N and Σ(r − s)² = Σr² − N·s² (the mean is removed when |s| > 2eps(T),
 // src/Poisson.jl:95-97): dsc[0] = Σr, dsc[1] = the residual norm; st[0] = (T)s, st[1] = flag.
 template <class T>
-__global__ void __launch_bounds__(256) k_finish_sum2(const double* __restrict__ partial, int64_t nb,
-                                                     double* __restrict__ dsc, T* __restrict__ st, double ninside) {
+__global__ void __launch_bounds__(1024) k_finish_sum2(const double* __restrict__ partial, int64_t nb,
+                                                      double* __restrict__ dsc, T* __restrict__ st, double ninside) {
   double a = 0, b = 0;
+#pragma unroll 4
   for (int64_t q = threadIdx.x; q < nb; q += blockDim.x) { a += partial[q]; b += partial[nb + q]; }
   a = block_sum(a);
   b = block_sum(b);
