@@ -639,7 +639,8 @@ struct MG {
     Level<T>& p = lv[l];
     perbc(p.g, p.e);
     if (pairs_ok(l)) {
-      ZT b = make_zt(box_pairs(p.g));
+      // shorter z runs than the default (4x the target): 214 -> 207 us at 512x256x256
+      ZT b = make_zt(box_pairs(p.g), 4 * zt_target_default());
       const bool ejw = l2 && p.ej != nullptr;
       ProfScope ps(P, st, "increment", l, (double)sizeof(T) * (5 + (ejw ? 1 : 0)) * ncells_inside(p.g));
 #define WL_INC2(DD, L2V, EJV)                                                                                    \
