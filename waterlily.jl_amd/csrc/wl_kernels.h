@@ -641,12 +641,10 @@ __global__ void __launch_bounds__(256) k_bdim2_rows(Geo g, const int* __restrict
 // flags; none without them), whose BDIM! the convection kernel completes; for k_bdim2's rows,
 // bit 2 where μ₁ is nonzero (rows) and bit 3 where μ₀ is uniform; bit 4 on the fused rows with
 // no μ₁ row among their y and z neighbours.
-// The inside rows without bit 1 are appended to rlist (in no particular order: each row's BDIM! is
-// independent of the others'); *cnt, zeroed beforehand, ends as their number.
+// k_row_list then lists the inside rows without bit 1.
 __global__ void __launch_bounds__(256) k_conv_rows(Geo g, const uint8_t* __restrict__ rows,
                                                    const uint8_t* __restrict__ vrows,
-                                                   const uint8_t* __restrict__ lrows, uint8_t* __restrict__ crows,
-                                                   int* __restrict__ rlist, int* __restrict__ cnt) {
+                                                   const uint8_t* __restrict__ lrows, uint8_t* __restrict__ crows) {
   const int64_t nr = (int64_t)g.N[1] * g.N[2];
   const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= nr) return;
@@ -659,7 +657,38 @@ __global__ void __launch_bounds__(256) k_conv_rows(Geo g, const uint8_t* __restr
   bool nb = (y >= 1 && rows[r - 1] != 0) || (y + 1 < g.N[1] && rows[r + 1] != 0);
   if (g.D == 3) nb = nb || (z >= 1 && rows[r - g.N[1]] != 0) || (z + 1 < g.N[2] && rows[r + g.N[1]] != 0);
   crows[r] = (uint8_t)((v ? 1 : 0) | (fz ? 2 : 0) | (b ? 4 : 0) | (lu ? 8 : 0) | (fz && !nb ? 16 : 0));
-  if (inside && !fz) rlist[atomicAdd(cnt, 1)] = (int)r;
+}
+// The inside rows of crows without bit 1, in increasing row order, into rlist, and their number
+// into *cnt: one 1024-thread block, 1024 rows per pass, each row's place from a block-wide prefix
+// count (wave ballots, then the waves' totals in LDS).  In row order, the rows k_bdim2_rows
+// does next to each other share their neighbour rows in the cache.
+__global__ void __launch_bounds__(1024) k_row_list(Geo g, const uint8_t* __restrict__ crows, int* __restrict__ rlist,
+                                                   int* __restrict__ cnt) {
+  __shared__ int wsum[16];
+  const int64_t nr = (int64_t)g.N[1] * g.N[2];
+  const int tid = (int)threadIdx.x, lane = tid & 63, wv = tid >> 6, nw = (int)blockDim.x >> 6;
+  int base = 0;
+  for (int64_t r0 = 0; r0 < nr; r0 += blockDim.x) {
+    const int64_t r = r0 + tid;
+    bool take = false;
+    if (r < nr) {
+      const int y = (int)(r % g.N[1]), z = (int)(r / g.N[1]);
+      const bool inside = y >= 1 && y <= g.N[1] - 2 && (g.D == 2 || (z >= 1 && z <= g.N[2] - 2));
+      take = inside && (crows[r] & 2) == 0;
+    }
+    const uint64_t m = __ballot(take);
+    const int below = __popcll(m & ((1ull << lane) - 1ull));
+    if (lane == 0) wsum[wv] = __popcll(m);
+    __syncthreads();
+    int off = base;
+    for (int k = 0; k < wv; ++k) off += wsum[k];
+    int tot = 0;
+    for (int k = 0; k < nw; ++k) tot += wsum[k];
+    if (take) rlist[off + below] = (int)r;
+    base += tot;
+    __syncthreads();  // wsum is rewritten by the next pass
+  }
+  if (tid == 0) cnt[0] = base;
 }
 // the x = N0-1 plane of component 0 (every y, z), from a to b: the exit plane that BC! keeps
 // under exitBC (src/core.jl:212)

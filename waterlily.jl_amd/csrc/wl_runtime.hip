@@ -1378,10 +1378,10 @@ struct FlowT : FlowBase {
     const int64_t nr = (int64_t)g.N[1] * g.N[2];
     const LRows<T> m0 = mg.lr(0);
     int* cnt = rlist + nr;
-    hipLaunchKernelGGL(k_fill<int>, dim3(1), dim3(256), 0, st, cnt, (int64_t)1, 0);
-    KCK();
     hipLaunchKernelGGL(k_conv_rows, dim3((unsigned)((nr + 255) / 256)), dim3(256), 0, st, g, (const uint8_t*)rows,
-                       (const uint8_t*)vrows(), m0.rows, crows(), rlist, cnt);
+                       (const uint8_t*)vrows(), m0.rows, crows());
+    KCK();
+    hipLaunchKernelGGL(k_row_list, dim3(1), dim3(1024), 0, st, g, (const uint8_t*)crows(), rlist, cnt);
     KCK();
     HIPCK(hipMemcpyAsync(B.htsc, cnt, sizeof(int), hipMemcpyDeviceToHost, st));
     HIPCK(hipStreamSynchronize(st));
