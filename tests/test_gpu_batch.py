@@ -69,3 +69,25 @@ def test_batch_matches_single(wl, D, graphs, monkeypatch):
     for a, c in zip(batched, single):
         assert a.flow.Δt == c.flow.Δt and a.pois.n == c.pois.n
     say("  n =", [a.pois.n[-2:] for a in batched])
+
+
+def test_batch_large_3d_member(wl):
+    """A 128³ member: large enough that the reduction kernels' z-marching tiles hold more than one
+    plane, so the batch's per-member partial-sum slots, its reduction finishes and the single
+    flow's must all agree on the tile count; 3 steps against stepping each member alone."""
+    say("case: batch with a 128x128x128 member vs single stepping, 3 steps")
+    specs = [dict(n=(128, 128, 128), L=16.0, nu=16.0 / 500, body=wl.Sphere((32, 64, 64), 8)),
+             dict(n=(64, 32, 32), L=8.0, nu=8.0 / 300, body=wl.Sphere((16, 16, 16), 4))]
+    batched = [make(wl, s) for s in specs]
+    single = [make(wl, s) for s in specs]
+    b = wl.Batch(batched)
+    for k in range(3):
+        b.step()
+        for s in single:
+            wl.sim_step(s, remeasure=False)
+        for a, c in zip(batched, single):
+            for name in ("u", "p"):
+                assert np.array_equal(a.flow.get(name), c.flow.get(name)), (k, name, a.flow.N)
+    for a, c in zip(batched, single):
+        assert a.flow.Δt == c.flow.Δt and a.pois.n == c.pois.n
+    say("  n =", [a.pois.n[-2:] for a in batched])

@@ -2084,13 +2084,13 @@ struct BatchT : BatchBase {
       std::vector<ALev<T>> a;
       std::vector<BBlk> b;
       for (int s = 0; s < ns; ++s) {
-        const ZT t = make_zt(box_inside(F[s]->g));
+        const ZT t = zt_inside(F[s]->g);
         a.push_back(lev_args(F[s], 0, t));
         add_blocks(b, s, zt_blocks(t));
       }
       eps0 = make_launch(a, b);
       // the conditional mean removal writes its Σr² partials where the residual pass put its own
-      for (int s = 0; s < ns; ++s) a[s].partial += zt_blocks(make_zt(box_inside(F[s]->g)));
+      for (int s = 0; s < ns; ++s) a[s].partial += zt_blocks(zt_inside(F[s]->g));
       sub0 = make_launch(a, b);
     }
     {  // coarse tails (MG::launch_tail)
@@ -2131,7 +2131,7 @@ struct BatchT : BatchBase {
       std::vector<ACtl> c;
       for (int s = 0; s < ns; ++s) {
         FlowT<T>* f = F[s];
-        const int64_t nb = zt_blocks(make_zt(box_inside(f->g)));
+        const int64_t nb = zt_blocks(zt_inside(f->g));
         fr.push_back(AFin{f->B.partial, nb, f->B.dsc + 0, f->B.tsc, 1, (double)ncells_inside(f->g)});
         fl.push_back(AFin{f->B.partial, nb, f->B.dsc + 1, f->B.tsc, 0, 1.0});
         flr.push_back(AFin{f->B.partial + nb, nb, f->B.dsc + 1, f->B.tsc, 0, 1.0});
@@ -2155,7 +2155,7 @@ struct BatchT : BatchBase {
         add_blocks(bc_, s, zt_blocks(f->conv_zt()));
         add_blocks(bb, s, zt_blocks(make_zt(box_inside(f->g), 2 * zt_target_default())));
         add_blocks(bbc, s, (int64_t)bc_gx(f->g) * f->g.D * (2 * f->g.D + 1), false);
-        add_blocks(br, s, zt_blocks(make_zt(box_inside(f->g))));
+        add_blocks(br, s, zt_blocks(zt_inside(f->g)));
         add_blocks(bco, s, zt_blocks(make_zt(box_all(f->g), 2 * zt_target_default())));
         add_blocks(bcf, s, zt_blocks(make_zt(box_all(f->g))));
       }
@@ -2218,7 +2218,7 @@ struct BatchT : BatchBase {
       double Ub[3];
       f->Ut(t1[s], Ub);
       bcv[s] = ABc<T>{f->g, uB, {Ub[0], Ub[1], f->g.D > 2 ? Ub[2] : 0.0}, 0, 0, bc_gx(f->g), f->g.D};
-      const ZT ti = make_zt(box_inside(f->g));
+      const ZT ti = zt_inside(f->g);
       const int64_t nbr = zt_blocks(ti);
       r1[s] = ARes<T>{f->g, ti, f->mg.lr(0), f->p, f->xw, T(1) * dt, f->sigma, uB, f->mg.lv[0].r, f->B.partial,
                       f->mg.lv[0].e, nbr};
