@@ -391,6 +391,28 @@ def test_mixed_precision_solver(wl):
     assert um < 1e-4 and pm < 1e-4 and worst_m < 1e-3
 
 
+def test_mixed_split_eps_matches_field_layout(wl, monkeypatch):
+    """The mixed-precision solver with the finest smoother ϵ colour-split (WL_SPLIT, on by default;
+    its FP64 residual reads that ϵ by colour) against the field layout (WL_SPLIT=0): the same bits
+    in u, p, f, Δt and the V-cycle counts over 4 steps of a 3D sphere."""
+    say("case: mixed solver, WL_SPLIT=1 vs 0, 3D 64x32x32 sphere, 4 steps")
+    n, c, R, nu = (64, 32, 32), (20, 16, 16), 6, 12 / 400
+    sims = []
+    for v in ("1", "0"):
+        monkeypatch.setenv("WL_SPLIT", v)
+        s = wl.Simulation(n, (1, 0, 0), 2 * R, ν=nu, body=wl.Sphere(c, R), T=np.float32)
+        s.pois.set_solver(mode=wl.WL_SOLVER_MIXED)
+        sims.append(s)
+    a, b = sims
+    for k in range(4):
+        wl.sim_step(a, remeasure=False)
+        wl.sim_step(b, remeasure=False)
+        for name in ("u", "p", "f"):
+            assert np.array_equal(a.flow.get(name), b.flow.get(name)), (k, name)
+    assert a.flow.Δt == b.flow.Δt and a.pois.n == b.pois.n
+    say("  n =", a.pois.n[-4:])
+
+
 def test_mixed_coarse_pcg(wl, monkeypatch):
     """WL_COARSE_PCG=1: the mixed-precision solver with the coarsest level solved by FP64 pcg!
     (src/Poisson.jl:166-186) instead of FP64 GS-RB.  The same 3D sphere (64x32x32, Re 400) for 20

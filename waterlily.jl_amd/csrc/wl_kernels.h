@@ -3215,14 +3215,17 @@ __global__ void k_mg_control(double* __restrict__ dsc, T* __restrict__ st) {
 // PREP: also the next V-cycle's start as k_mixed_prep forms it when no mean is removed, ϵ = r·iD
 // into eout and the correction c = 0 into cout (a buffer this pass does not read); with a mean to
 // remove, k_mixed_fix corrects r and ϵ afterwards.
-template <class T, bool FLOW, bool EINC = false, bool PREP = false>
+// SPLIT (3D): einc is the colour-split smoother ϵ (SplitG: cell (x,y,z) at k·H + (x+1)/2 +
+// Q·(y + N1·z), k = (x+y+z) mod 2), read at each stencil cell by its own colour.
+template <class T, bool FLOW, bool EINC = false, bool PREP = false, bool SPLIT = false>
 __global__ void __launch_bounds__(256) k_residual64(Geo g, ZT t, LRows<T> LR, const T* __restrict__ pin, T scale,
                                                     T* __restrict__ z, const T* __restrict__ u,
                                                     const double* __restrict__ xa, const T* __restrict__ xc,
                                                     double* __restrict__ xb, T* __restrict__ r,
                                                     double* __restrict__ partial, int64_t nb,
                                                     const T* __restrict__ einc = nullptr, const T* __restrict__ wp = nullptr,
-                                                    T* __restrict__ eout = nullptr, T* __restrict__ cout = nullptr) {
+                                                    T* __restrict__ eout = nullptr, T* __restrict__ cout = nullptr,
+                                                    SplitG sg = SplitG{0, 0}) {
   int c0, c1, z0, z1;
   zt_cell(t, c0, c1, z0, z1);
   double acc = 0, acc2 = 0;
@@ -3253,8 +3256,26 @@ __global__ void __launch_bounds__(256) k_residual64(Geo g, ZT t, LRows<T> LR, co
         return EINC ? xa[J] + (double)(xc[J] + wp[0] * einc[J]) : xa[J] + (double)xc[J];
       };
       // x and its neighbours, then L, before the arithmetic
-      const double xI = xv(I);
-      const Nb<double> v = gather_nb<double>(g, I, xv);
+      double xI;
+      Nb<double> v;
+      if constexpr (EINC && SPLIT) {
+        const int64_t rowq = (int64_t)sg.Q * (c1 + (int64_t)g.N[1] * c2);
+        auto xs = [&](int64_t J, int x, int64_t q, int par) -> double {
+          return xa[J] + (double)(xc[J] + wp[0] * einc[(int64_t)((x + par) & 1) * sg.H + ((x + 1) >> 1) + q]);
+        };
+        const int par = (c1 + c2) & 1;
+        const int64_t dq1 = sg.Q, dq2 = (int64_t)sg.Q * g.N[1];
+        xI = xs(I, c0, rowq, par);
+        v.m[0] = xs(I - 1, c0 - 1, rowq, par);
+        v.p[0] = xs(I + 1, c0 + 1, rowq, par);
+        v.m[1] = xs(I - g.s[1], c0, rowq - dq1, par ^ 1);
+        v.p[1] = xs(I + g.s[1], c0, rowq + dq1, par ^ 1);
+        v.m[2] = xs(I - g.s[2], c0, rowq - dq2, par ^ 1);
+        v.p[2] = xs(I + g.s[2], c0, rowq + dq2, par ^ 1);
+      } else {
+        xI = xv(I);
+        v = gather_nb<double>(g, I, xv);
+      }
       const LC<T> L = lcell(g, LR, c0, c1, c2);
       double Dg = 0;
       Dg -= ((double)L.l[0] + (double)L.h[0]);

@@ -1092,16 +1092,30 @@ struct MG {
         ProfScope ps(P, st, "residual64", 0,
                      (double)ncells_inside(p.g) *
                          (sizeof(T) * (p.g.D + 2 + (fuse ? 1 : 0) + (pf ? 2 : 0)) + 2 * sizeof(double)));
-        if (pf) {
+        if (pf && split_on) {  // the smoother's ϵ colour-split (split_begin)
+          hipLaunchKernelGGL((k_residual64<T, false, true, true, true>), zt_grid(bi), kBlock, 0, st, p.g, bi, lr(0),
+                             (const T*)nullptr, T(0), p.z, (const T*)nullptr, (const double*)x64[x64cur],
+                             (const T*)p.x, x64[1 - x64cur], p.r, B->partial, nb, (const T*)p.e, w, jac(0), c_alt,
+                             sg);
+          std::swap(p.x, c_alt);
+          mixed_prepped = true;
+        } else if (pf) {
           hipLaunchKernelGGL((k_residual64<T, false, true, true>), zt_grid(bi), kBlock, 0, st, p.g, bi, lr(0),
                              (const T*)nullptr, T(0), p.z, (const T*)nullptr, (const double*)x64[x64cur],
                              (const T*)p.x, x64[1 - x64cur], p.r, B->partial, nb, (const T*)p.e, w, jac(0), c_alt);
           std::swap(p.x, c_alt);
           mixed_prepped = true;
-        } else if (fuse)
+        } else if (fuse && split_on)
+          hipLaunchKernelGGL((k_residual64<T, false, true, false, true>), zt_grid(bi), kBlock, 0, st, p.g, bi, lr(0),
+                             (const T*)nullptr, T(0), p.z, (const T*)nullptr, (const double*)x64[x64cur],
+                             (const T*)p.x, x64[1 - x64cur], p.r, B->partial, nb, (const T*)p.e, w, (T*)nullptr,
+                             (T*)nullptr, sg);
+        else if (fuse)
           hipLaunchKernelGGL((k_residual64<T, false, true>), zt_grid(bi), kBlock, 0, st, p.g, bi, lr(0),
                              (const T*)nullptr, T(0), p.z, (const T*)nullptr, (const double*)x64[x64cur],
                              (const T*)p.x, x64[1 - x64cur], p.r, B->partial, nb, (const T*)p.e, w);
+        else if (split_on)
+          throw WlErr(WL_EINVAL, "internal: colour-split level 0 needs the fused mixed residual");
         else
           hipLaunchKernelGGL((k_residual64<T, false, false>), zt_grid(bi), kBlock, 0, st, p.g, bi, lr(0),
                              (const T*)nullptr, T(0), p.z, (const T*)nullptr, (const double*)x64[x64cur],
@@ -1657,7 +1671,17 @@ struct FlowT : FlowBase {
     mg.finish_sum2(nb);
     const bool use_net = net && (!(net_flags & WL_WARM_PREDICTOR) || w == T(1));
     if (use_net) warm_guess_mixed(bi, nb);
-    const int nit = mg.mixed_loop(tol, itmx, &r2hist);
+    // the smoother's level-0 ϵ colour-split for the solve, as in the plain projection
+    int nit;
+    {
+      const bool split = !use_net && mg.split_ready();
+      struct SplitScope {
+        MG<T>* m;
+        ~SplitScope() { if (m) m->split_end(); }
+      } split_scope{split ? &mg : nullptr};
+      if (split) mg.split_begin();
+      nit = mg.mixed_loop(tol, itmx, &r2hist);
+    }
     hipLaunchKernelGGL(k_round_x<T>, zt_grid(bi), kBlock, 0, st, g, bi, (const double*)mg.x64[mg.x64cur], xw);
     KCK();
     mg.perbc(g, xw);
